@@ -1,0 +1,218 @@
+"""Throughput of the MI355X sosfilt hot path (one JSON line on stdout).
+
+Default workload (N=1): batched sosfilt FP32, x [4096 channels, 65536
+samples] ~ N(0,1) (seeded, generated on the device), one 6-section order-12
+Butterworth bandpass [0.05, 0.3] shared by all channels.  A step is one call
+of the drop-in ``sosfilt(sos, x)`` over the whole batch; inputs are resident
+in HBM before timing starts.
+
+metric  G sample*sections/s = B*T*S / t / 1e9
+roofline.achieved  algorithmic bytes (B*T*(4+4) + sos bytes) / mean step time
+         measured with HIP events on the launch stream over the timed steps
+cpu_baseline  the FP64 C oracle (FP32 I/O, OpenMP over channels, all host
+         cores) on a bounded sample of the same workload (a subset of
+         channels, full T), scaled to the same unit.
+
+Other workloads (--workload): c2b (per-channel order-8 lowpass filters),
+c2c (x [8, 2^24]), c5 (order-16 Chebyshev I, x [2048, 2^20], fp32 / bf16io),
+c3 (forward+backward of the differentiation workload).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = ("G sample·sections/s (1 GPU) vs plain; serving p50/p99 latency at "
+          "≤1e-5 rel err")
+HBM_PEAK_GBS = 8000.0
+# traffic per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE (profiles/), by workload
+TRAFFIC_BYTES = {}
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--workload", default="c2a", choices=["c2a", "c2b", "c2c", "c5", "c5bf16", "c3"])
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=10.0)
+    return p.parse_args()
+
+
+def make_workload(name, dev):
+    import numpy as np
+    import torch
+    from torchscience_amd.signal_processing.filter._design import (
+        batched_butterworth_sos, butterworth_sos, chebyshev1_sos)
+    g = torch.Generator(device=dev)
+    if name == "c2a":
+        B, T = 4096, 65536
+        g.manual_seed(1)
+        sos = butterworth_sos(6, [0.05, 0.3], "bandpass")
+        desc = ("batched sosfilt fp32: x[4096,65536] N(0,1), 6-section order-12 Butterworth "
+                "bandpass [0.05,0.3] shared across channels")
+        cpf = B
+    elif name == "c2b":
+        B, T = 4096, 65536
+        g.manual_seed(1)
+        rng = np.random.default_rng(2)
+        sos = batched_butterworth_sos(8, rng.uniform(0.02, 0.45, B))
+        desc = ("batched sosfilt fp32: x[4096,65536] N(0,1), per-channel order-8 Butterworth "
+                "lowpass, cutoff U[0.02,0.45] (4 sections each)")
+        cpf = 1
+    elif name == "c2c":
+        B, T = 8, 1 << 24
+        g.manual_seed(3)
+        sos = butterworth_sos(6, [0.05, 0.3], "bandpass")
+        desc = "long-sequence sosfilt fp32: x[8,2^24] N(0,1), 6-section bandpass [0.05,0.3]"
+        cpf = B
+    elif name in ("c5", "c5bf16"):
+        B, T = 2048, 1 << 20
+        g.manual_seed(5)
+        sos = chebyshev1_sos(16, 0.01, 1.0)
+        desc = ("precision stress sosfilt: x[2048,2^20], order-16 Chebyshev I 1 dB cutoff 0.01 "
+                "(8 sections), " + ("bf16 I/O fp32 arithmetic" if name == "c5bf16" else "fp32"))
+        cpf = B
+    else:  # c3
+        B, T = 1024, 16384
+        g.manual_seed(3)
+        sos = butterworth_sos(8, 0.2)
+        desc = ("differentiation: sosfilt fp32 forward+backward, x[1024,16384], learnable "
+                "4-section SOS (order-8 Butterworth lowpass 0.2), zi (1024,4,2)")
+        cpf = B
+    x = torch.randn(B, T, device=dev, generator=g, dtype=torch.float32)
+    if name == "c5bf16":
+        x = x.bfloat16()
+    return dict(B=B, T=T, sos=sos, x=x, desc=desc, cpf=cpf)
+
+
+def cpu_baseline(w, budget_s):
+    import numpy as np
+    from oracle import oracle
+    B, T, sos = w["B"], w["T"], w["sos"]
+    S = sos.shape[-2]
+    threads = max(1, os.cpu_count() or 1)
+    # probe with a few channels, then size the sample to ~budget_s
+    probe = min(B, threads)
+    xs = w["x"][:probe].float().cpu().numpy()
+    sub = sos if sos.ndim == 2 else sos[:probe]
+    t0 = time.perf_counter()
+    oracle.sosfilt_rows_f32io(sub, xs, chans_per_filter=1 if sos.ndim == 3 else None,
+                              n_threads=threads)
+    dt = max(time.perf_counter() - t0, 1e-6)
+    n = int(min(B, max(probe, probe * budget_s / dt)))
+    n = max(probe, n // threads * threads)
+    xs = w["x"][:n].float().cpu().numpy()
+    sub = sos if sos.ndim == 2 else sos[:n]
+    t0 = time.perf_counter()
+    oracle.sosfilt_rows_f32io(sub, xs, chans_per_filter=1 if sos.ndim == 3 else None,
+                              n_threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": n * T * S / dt / 1e9, "unit": "G sample·sections/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n} of {B} channels x full T={T}, {S} sections, FP64 DF2T C oracle, "
+                      f"fp32 I/O, OpenMP {threads} threads, {dt:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (this build has no multi-GPU path); "
+              f"got --gpus {args.gpus}", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from torchscience_amd.signal_processing.filter import sosfilt, batched_filter_apply
+
+    dev = torch.device("cuda:0")
+    w = make_workload(args.workload, dev)
+    B, T, sos_np = w["B"], w["T"], w["sos"]
+    S = sos_np.shape[-2]
+    x = w["x"]
+    per_channel = sos_np.ndim == 3
+    sos = torch.tensor(sos_np, dtype=torch.float32, device=dev)
+    io_bytes = 2 if x.dtype == torch.bfloat16 else 4
+
+    if args.workload == "c3":
+        sos_p = sos.clone().requires_grad_(True)
+        xg = x.clone().requires_grad_(True)
+        zi = torch.zeros(B, S, 2, device=dev, requires_grad=True)
+
+        def step():
+            y, zf = sosfilt(sos_p, xg, zi=zi)
+            y.backward(torch.ones_like(y))
+            return y
+        algo_bytes = B * T * 4 * 2 + B * T * 4 * 3
+    elif per_channel:
+        def step():
+            return batched_filter_apply(sos, x, broadcast=False)
+        algo_bytes = B * T * io_bytes * 2 + sos.numel() * 4
+    else:
+        prec = "bf16io" if x.dtype == torch.bfloat16 else None
+
+        def step():
+            return sosfilt(sos, x, precision=prec)
+        algo_bytes = B * T * io_bytes * 2 + sos.numel() * 4
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream(dev)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = ev0.elapsed_time(ev1) / args.steps
+    ms = wall * 1e3 / args.steps
+    value = B * T * S / (ms * 1e-3) / 1e9
+    achieved = algo_bytes / (ev_ms * 1e-3) / 1e9
+
+    from torchscience_amd import _lib
+    assert _lib.workspaces.header_error(dev) == 0, "look-back spin limit hit"
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "G sample·sections/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16" if x.dtype == torch.bfloat16 else "f32",
+        "data": "synthetic: x ~ N(0,1) from a seeded device generator; coefficients from the "
+                "FP64 IIR designer",
+        "config": {"workload": w["desc"], "channels": B, "samples": T, "sections": S,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": TRAFFIC_BYTES.get(args.workload),
+                     "event_ms_per_step": round(ev_ms, 4),
+                     "algorithmic_bytes": algo_bytes},
+    }
+    if not args.no_cpu_baseline and args.workload != "c3":
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

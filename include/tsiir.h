@@ -1,0 +1,146 @@
+/*
+ * tsiir.h - C ABI of the MI355X (gfx950) IIR filtering library.
+ *
+ * This is the drop-in boundary for torchscience's IIR filtering hot path.
+ * The reference implements the path in Python over ATen ops:
+ *   sosfilt              src/torchscience/signal_processing/filter/_sosfilt.py:13-133
+ *   lfilter (DF2T core)  src/torchscience/signal_processing/filter/_lfilter.py:12-178
+ *   batched_filter_apply src/torchscience/signal_processing/filter/_batched.py:449-566
+ * and has no native entry point for it, so each function below replaces the
+ * Python loop it names: a ctypes (or any C FFI) caller hands over device
+ * pointers, sizes and a HIP stream, and the library launches hand-written HIP
+ * kernels on that stream.
+ *
+ * Conventions
+ *  - Plain C: no HIP, torch or C++ types.  `stream` is a hipStream_t passed
+ *    as void* (NULL = the default stream).
+ *  - All data pointers are device pointers.  The library never allocates,
+ *    frees or synchronises; it launches asynchronously and keeps no pointer
+ *    after returning.
+ *  - Errors are returned as tsiir_status; nothing is thrown across the ABI.
+ *  - Every call is reentrant.  Calls that share one workspace must not run
+ *    concurrently (they would race on its scan status words).
+ *
+ * Data layout
+ *  - Signals are (channels, samples) with unit stride along time and a row
+ *    stride in elements between channels.
+ *  - Channel c of the output reads input row (c % x_rows), filter
+ *    (c / chans_per_filter) of `sos`, and initial-state row c of `zi`
+ *    (row 0 when TSIIR_ZI_BROADCAST is set).
+ *  - sos is (n_filters, n_sections, 6) contiguous, rows [b0 b1 b2 a0 a1 a2];
+ *    each row is normalised by a0 when a0 != 1, as _lfilter.py:96-98 does.
+ *  - zi / zf are (channels, n_sections, 2) contiguous, in the compute type
+ *    (float for TSIIR_F32 / TSIIR_BF16 I/O, double for TSIIR_F64).
+ */
+#ifndef TSIIR_H
+#define TSIIR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TSIIR_ABI_VERSION 1
+
+typedef enum tsiir_status {
+  TSIIR_OK = 0,
+  TSIIR_EINVAL = 1,        /* bad descriptor, pointer or size */
+  TSIIR_EUNSUPPORTED = 2,  /* valid request this build does not implement */
+  TSIIR_ELAUNCH = 3,       /* a kernel launch failed */
+  TSIIR_EWORKSPACE = 4     /* workspace missing or too small */
+} tsiir_status;
+
+typedef enum tsiir_dtype {
+  TSIIR_F32 = 0,
+  TSIIR_F64 = 1,
+  TSIIR_BF16 = 2
+} tsiir_dtype;
+
+/* Arithmetic form of the recurrence (the compute type follows io_dtype). */
+typedef enum tsiir_mode {
+  TSIIR_MODE_PLAIN = 0,        /* DF2T in the compute type                 */
+  TSIIR_MODE_COMPENSATED = 1   /* FP32 DF2T with error-free state updates  */
+} tsiir_mode;
+
+/* flags */
+#define TSIIR_HAS_ZI 0x1u        /* zi holds the initial state             */
+#define TSIIR_WRITE_ZF 0x2u      /* write the final state to zf            */
+#define TSIIR_ZI_BROADCAST 0x4u  /* one (n_sections, 2) zi for every channel */
+#define TSIIR_REVERSE 0x8u       /* run along decreasing time (adjoint)    */
+
+typedef struct tsiir_sosfilt_desc {
+  int64_t channels;          /* output channels B                          */
+  int64_t n_samples;         /* T                                          */
+  int32_t n_sections;        /* S (>= 1)                                   */
+  int32_t io_dtype;          /* tsiir_dtype of x and y                     */
+  int32_t coef_dtype;        /* tsiir_dtype of sos (F32 or F64)            */
+  int32_t mode;              /* tsiir_mode                                 */
+  uint32_t flags;            /* TSIIR_* flags                              */
+  int32_t reserved;
+  int64_t x_rows;            /* distinct input rows (channel c reads c % x_rows) */
+  int64_t x_row_stride;      /* elements between input rows               */
+  int64_t y_row_stride;      /* elements between output rows              */
+  int64_t chans_per_filter;  /* channels that share one sos filter (>= 1) */
+} tsiir_sosfilt_desc;
+
+/* Library ABI version (TSIIR_ABI_VERSION of the build). */
+int tsiir_version(void);
+
+/* Static text for a status code. */
+const char* tsiir_status_string(tsiir_status status);
+
+/* Device-memory workspace the forward call needs for this descriptor.
+ * Replaces nothing in the reference (its Python loop allocates per step);
+ * callers size one buffer with it and reuse it. */
+size_t tsiir_sosfilt_workspace_bytes(const tsiir_sosfilt_desc* desc);
+
+/* Prepare a workspace for first use: zero-fills it on `stream`.  Needed once
+ * per allocation; later calls keep it consistent themselves. */
+tsiir_status tsiir_workspace_init(void* workspace, size_t workspace_bytes,
+                                  void* stream);
+
+/* y = sosfilt(sos, x, zi) along time; zf = final state (when requested).
+ * Replaces the section loop _sosfilt.py:117-128 and the per-sample DF2T loop
+ * _lfilter.py:153-166 it calls, and the per-filter loop of
+ * batched_filter_apply(filter_format="sos") _batched.py:513-531 (via
+ * x_rows / chans_per_filter).  Also serves lfilter for filters of order <= 2
+ * (a single section), _lfilter.py:12-178. */
+tsiir_status tsiir_sosfilt_forward(const tsiir_sosfilt_desc* desc,
+                                   const void* x, void* y, const void* sos,
+                                   const void* zi, void* zf,
+                                   void* workspace, size_t workspace_bytes,
+                                   void* stream);
+
+/* y = lfilter(b, a, x, zi) for a filter of order N = desc->n_sections
+ * (1..64), coefficients shared by every channel.  `coef` holds b[0..N] then
+ * a[0..N] (2N+2 values, zero-padded to a common length as
+ * _lfilter.py:114-117 does) in coef_dtype == io_dtype (F32 or F64); they are
+ * normalised by a[0] when a[0] != 1.  zi / zf are (channels, N).
+ * Replaces the per-sample DF2T loop _lfilter.py:153-166 for orders above 2
+ * (orders <= 2 go through tsiir_sosfilt_forward as one section). */
+tsiir_status tsiir_lfilter_forward(const tsiir_sosfilt_desc* desc, const void* coef,
+                                   const void* x, void* y, const void* zi, void* zf,
+                                   void* stream);
+
+/* Gradients of sosfilt (FP32 or FP64 compute).  Given dy (and dzf when zf
+ * was an output), writes dx, dsos (n_filters, n_sections, 6) and dzi
+ * (channels, n_sections, 2).  Replaces the autograd tape the reference
+ * records through _lfilter.py:153-166 (gradients reach x, b0..b2, a1, a2 and
+ * zi; the a0 column receives 0 when a0 == 1, as in the reference).
+ * dsos is accumulated per channel in `dsos_partial` workspace and reduced;
+ * dzi / dzf / dsos may be NULL when not wanted. */
+size_t tsiir_sosfilt_backward_workspace_bytes(const tsiir_sosfilt_desc* desc);
+tsiir_status tsiir_sosfilt_backward(const tsiir_sosfilt_desc* desc,
+                                    const void* x, const void* sos,
+                                    const void* zi, const void* dy,
+                                    const void* dzf, void* dx, void* dsos,
+                                    void* dzi, void* workspace,
+                                    size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TSIIR_H */

@@ -1,0 +1,329 @@
+"""FP64 CPU oracle for torchscience's IIR filtering path.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker: tests/, the smoke
+check of ``__graft_entry__`` and the ``cpu_baseline`` leg of bench.py import
+it; the product package (``torchscience_amd``) never does.
+
+It restates, in numpy over the C DF2T core in ``oracle.c``:
+
+* ``sosfilt``  - src/torchscience/signal_processing/filter/_sosfilt.py:84-133
+  (shape check and message, axis move, zi broadcast of an (S, 2) zi over a
+  batched x, section cascade, zf per section);
+* ``lfilter``  - src/torchscience/signal_processing/filter/_lfilter.py:86-178
+  (1-D checks and messages, a0 normalisation, b/a padding, order-0 gain,
+  zi reshape, DF2T loop);
+* ``batched_filter_apply`` - src/torchscience/signal_processing/filter/_batched.py:449-566;
+* ``sosfilt_zi`` / ``lfilter_zi`` - _sosfilt_zi.py:54-86 / _lfilter_zi.py:70-128.
+
+Deliberate difference from the reference: zi with ``axis != -1`` follows the
+documented meaning (zi batch dims are x's batch dims in order), not the
+reference's ``moveaxis(zi, axis, -1)`` (_lfilter.py:137), which misorders
+state for x of two or more dims and is not covered by any reference test.
+
+All results are float64 (complex128 for complex input); the product's dtype
+promotion is checked separately.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_f64p = ctypes.POINTER(ctypes.c_double)
+_f32p = ctypes.POINTER(ctypes.c_float)
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "liboracle.so"],
+                       check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.oracle_lfilter_f64.argtypes = [_f64p, ctypes.c_int, _f64p, ctypes.c_int,
+                                         _f64p, _f64p, ctypes.c_int64, ctypes.c_int64,
+                                         _f64p, _f64p, ctypes.c_int]
+        L.oracle_sosfilt_f64.argtypes = [_f64p, ctypes.c_int, ctypes.c_int64,
+                                         _f64p, _f64p, ctypes.c_int64, ctypes.c_int64,
+                                         _f64p, _f64p, ctypes.c_int]
+        L.oracle_sosfilt_f32io.argtypes = [_f64p, ctypes.c_int, ctypes.c_int64,
+                                           _f32p, _f32p, ctypes.c_int64, ctypes.c_int64,
+                                           ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray | None, t=_f64p):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def default_threads() -> int:
+    return max(1, os.cpu_count() or 1)
+
+
+# --------------------------------------------------------------------------
+# flat (channels, samples) cores
+# --------------------------------------------------------------------------
+
+def sosfilt_rows(sos: np.ndarray, x: np.ndarray, zi: np.ndarray | None = None,
+                 chans_per_filter: int | None = None, n_threads: int = 1):
+    """Real FP64 sosfilt over rows of x (B, T).
+
+    sos is (S, 6) or (n_filters, S, 6) with channel c using filter
+    c // chans_per_filter.  zi is (B, S, 2) or None.  Returns (y, zf)."""
+    sos = np.ascontiguousarray(sos, dtype=np.float64)
+    if sos.ndim == 2:
+        sos = sos[None]
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    B, T = x.shape
+    S = sos.shape[1]
+    cpf = B if chans_per_filter is None else chans_per_filter
+    cpf = max(cpf, 1)
+    y = np.empty_like(x)
+    zf = np.empty((B, S, 2), dtype=np.float64)
+    zi_c = None if zi is None else np.ascontiguousarray(zi, dtype=np.float64)
+    lib().oracle_sosfilt_f64(_p(sos), S, cpf, _p(x), _p(y), B, T, _p(zi_c), _p(zf),
+                             n_threads)
+    return y, zf
+
+
+def sosfilt_rows_f32io(sos: np.ndarray, x: np.ndarray, chans_per_filter: int | None = None,
+                       n_threads: int = 1) -> np.ndarray:
+    """Float32 rows in and out, FP64 arithmetic (the CPU baseline form)."""
+    sos = np.ascontiguousarray(sos, dtype=np.float64)
+    if sos.ndim == 2:
+        sos = sos[None]
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    B, T = x.shape
+    cpf = B if chans_per_filter is None else max(chans_per_filter, 1)
+    y = np.empty_like(x)
+    lib().oracle_sosfilt_f32io(_p(sos), sos.shape[1], cpf, _p(x, _f32p), _p(y, _f32p),
+                               B, T, n_threads)
+    return y
+
+
+def lfilter_rows(b: np.ndarray, a: np.ndarray, x: np.ndarray,
+                 zi: np.ndarray | None = None, n_threads: int = 1):
+    """Real FP64 lfilter over rows of x (B, T); zi (B, N).  Returns (y, zf)."""
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    B, T = x.shape
+    order = max(b.size, a.size) - 1
+    if order > 63:
+        raise ValueError("oracle supports filter order <= 63")
+    y = np.empty_like(x)
+    zf = np.empty((B, max(order, 0)), dtype=np.float64)
+    zi_c = None if zi is None else np.ascontiguousarray(zi, dtype=np.float64)
+    lib().oracle_lfilter_f64(_p(b), b.size, _p(a), a.size, _p(x), _p(y), B, T,
+                             _p(zi_c), _p(zf) if order > 0 else None, n_threads)
+    return y, zf
+
+
+def _lfilter_rows_complex(b, a, x, zi):
+    """DF2T with complex coefficients, numpy-vectorised over rows (small
+    inputs only).  Same op order as _lfilter.py:153-166."""
+    b = np.asarray(b, dtype=np.complex128)
+    a = np.asarray(a, dtype=np.complex128)
+    if a[0] != 1.0:
+        b = b / a[0]
+        a = a / a[0]
+    n = max(b.size, a.size)
+    b = np.pad(b, (0, n - b.size))
+    a = np.pad(a, (0, n - a.size))
+    order = n - 1
+    x = np.asarray(x, dtype=np.complex128)
+    B, T = x.shape
+    y = np.empty_like(x)
+    if order == 0:
+        return b[0] * x, np.zeros((B, 0), np.complex128)
+    z = np.zeros((B, order), np.complex128) if zi is None else np.array(zi, np.complex128)
+    for t in range(T):
+        xn = x[:, t]
+        yn = b[0] * xn + z[:, 0]
+        y[:, t] = yn
+        for k in range(order - 1):
+            z[:, k] = b[k + 1] * xn - a[k + 1] * yn + z[:, k + 1]
+        z[:, order - 1] = b[order] * xn - a[order] * yn
+    return y, z
+
+
+# --------------------------------------------------------------------------
+# reference-semantics wrappers
+# --------------------------------------------------------------------------
+
+def _is_complex(*arrs) -> bool:
+    return any(np.iscomplexobj(a) for a in arrs if a is not None)
+
+
+def lfilter(b, a, x, axis: int = -1, zi=None):
+    """lfilter with the reference's semantics (_lfilter.py:12-178)."""
+    b = np.atleast_1d(np.asarray(b))
+    a = np.atleast_1d(np.asarray(a))
+    if b.ndim != 1:
+        raise ValueError("b must be a 1-D tensor")
+    if a.ndim != 1:
+        raise ValueError("a must be a 1-D tensor")
+    x = np.asarray(x)
+    xm = np.moveaxis(x, axis, -1)
+    batch = xm.shape[:-1]
+    T = xm.shape[-1]
+    rows = xm.reshape(-1, T)
+    order = max(b.size, a.size) - 1
+    zi_rows = None
+    if zi is not None:
+        zi_rows = np.asarray(zi).reshape(-1, order) if order > 0 else None
+        if zi_rows is not None and zi_rows.shape[0] == 1 and rows.shape[0] > 1:
+            zi_rows = np.broadcast_to(zi_rows, (rows.shape[0], order))
+    if _is_complex(b, a):
+        y, zf = _lfilter_rows_complex(b, a, rows, zi_rows)
+    elif _is_complex(x, zi):
+        zr = None if zi_rows is None else np.real(zi_rows)
+        zim = None if zi_rows is None else np.imag(zi_rows)
+        yr, zfr = lfilter_rows(np.real(b), np.real(a), np.real(rows), zr)
+        yi, zfi = lfilter_rows(np.real(b), np.real(a), np.imag(rows), zim)
+        y, zf = yr + 1j * yi, zfr + 1j * zfi
+    else:
+        y, zf = lfilter_rows(b, a, rows, zi_rows)
+    y = np.moveaxis(y.reshape(xm.shape), -1, axis)
+    if zi is None:
+        return y
+    if order == 0:
+        return y, np.zeros_like(np.asarray(zi), dtype=y.dtype)
+    return y, zf.reshape(batch + (order,)) if batch else zf.reshape(order)
+
+
+def sosfilt(sos, x, axis: int = -1, zi=None):
+    """sosfilt with the reference's semantics (_sosfilt.py:13-133)."""
+    sos = np.asarray(sos)
+    if sos.ndim != 2 or sos.shape[1] != 6:
+        raise ValueError("sos must be shape (n_sections, 6)")
+    S = sos.shape[0]
+    x = np.asarray(x)
+    xm = np.moveaxis(x, axis, -1)
+    batch = xm.shape[:-1]
+    T = xm.shape[-1]
+    rows = xm.reshape(-1, T)
+    B = rows.shape[0]
+    zi_rows = None
+    if zi is not None:
+        zi = np.asarray(zi)
+        if zi.shape == (S, 2) and len(batch) > 0:
+            zi_rows = np.broadcast_to(zi, (B, S, 2))
+        else:
+            zi_rows = zi.reshape(B, S, 2)
+    if _is_complex(sos):
+        y = rows.astype(np.complex128)
+        zf = np.zeros((B, S, 2), np.complex128)
+        for s in range(S):
+            y, zfs = _lfilter_rows_complex(sos[s, :3], sos[s, 3:], y,
+                                           None if zi_rows is None else zi_rows[:, s, :])
+            zf[:, s, :] = zfs
+    elif _is_complex(x, zi):
+        zr = None if zi_rows is None else np.real(zi_rows)
+        zim = None if zi_rows is None else np.imag(zi_rows)
+        yr, zfr = sosfilt_rows(np.real(sos), np.real(rows), zr)
+        yi, zfi = sosfilt_rows(np.real(sos), np.imag(rows), zim)
+        y, zf = yr + 1j * yi, zfr + 1j * zfi
+    else:
+        y, zf = sosfilt_rows(sos, rows, zi_rows)
+    y = np.moveaxis(y.reshape(xm.shape), -1, axis)
+    if zi is None:
+        return y
+    zf_shape = batch + (S, 2) if (zi.shape != (S, 2) or len(batch) > 0) else (S, 2)
+    return y, zf.reshape(zf_shape)
+
+
+def batched_filter_apply(filters, x, filter_format: str = "sos", broadcast: bool = True,
+                         axis: int = -1):
+    """batched_filter_apply with the reference's semantics (_batched.py:449-566)."""
+    filters = np.asarray(filters)
+    x = np.asarray(x)
+    F = filters.shape[0]
+    if filter_format == "sos":
+        if filters.ndim != 3 or filters.shape[2] != 6:
+            raise ValueError(
+                f"SOS filters must have shape (batch, n_sections, 6), got {filters.shape}")
+        if broadcast:
+            return np.stack([sosfilt(filters[i], x, axis=axis) for i in range(F)])
+        if x.shape[0] != F:
+            raise ValueError(
+                f"When broadcast=False, x batch dim must match filters: {x.shape[0]} != {F}")
+        return np.stack([sosfilt(filters[i], x[i], axis=axis) for i in range(F)])
+    if filter_format == "fir":
+        if filters.ndim != 2:
+            raise ValueError(
+                f"FIR filters must have shape (batch, num_taps), got {filters.shape}")
+        one = np.ones(1)
+        if broadcast:
+            return np.stack([lfilter(filters[i], one, x, axis=axis) for i in range(F)])
+        if x.shape[0] != F:
+            raise ValueError(
+                f"When broadcast=False, x batch dim must match filters: {x.shape[0]} != {F}")
+        return np.stack([lfilter(filters[i], one, x[i], axis=axis) for i in range(F)])
+    raise ValueError(f"filter_format must be 'sos' or 'fir', got '{filter_format}'")
+
+
+def lfilter_zi(b, a) -> np.ndarray:
+    """Steady-state initial state (_lfilter_zi.py:70-128): solve
+    (I - A^T) zi = b[1:] - a[1:] b[0] with A the companion matrix."""
+    b = np.atleast_1d(np.asarray(b, dtype=np.float64))
+    a = np.atleast_1d(np.asarray(a, dtype=np.float64))
+    while a.size > 1 and a[0] == 0.0:
+        a = a[1:]
+    if a[0] != 1.0:
+        b = b / a[0]
+        a = a / a[0]
+    n = max(a.size, b.size)
+    a = np.pad(a, (0, n - a.size))
+    b = np.pad(b, (0, n - b.size))
+    comp = np.zeros((n - 1, n - 1))
+    comp[0, :] = -a[1:]
+    comp[1:, :-1] += np.eye(n - 2) if n > 2 else 0.0
+    m = np.eye(n - 1) - comp.T
+    rhs = b[1:] - a[1:] * b[0]
+    return np.linalg.solve(m, rhs)
+
+
+def sosfilt_zi(sos) -> np.ndarray:
+    """Cascaded steady state (_sosfilt_zi.py:54-86)."""
+    sos = np.asarray(sos, dtype=np.float64)
+    zi = np.empty((sos.shape[0], 2))
+    scale = 1.0
+    for s in range(sos.shape[0]):
+        b, a = sos[s, :3], sos[s, 3:]
+        zi[s] = scale * lfilter_zi(b, a)
+        scale = scale * b.sum() / a.sum()
+    return zi
+
+
+def impulse_state_response(sos_row: np.ndarray, n: int) -> np.ndarray:
+    """Markov parameters of one DF2T biquad: h[k] for k < n (closed form via
+    the state-space matrices, used for brute-force checks)."""
+    b0, b1, b2, a0, a1, a2 = [float(v) for v in sos_row]
+    if a0 != 1.0:
+        b0, b1, b2, a1, a2 = b0 / a0, b1 / a0, b2 / a0, a1 / a0, a2 / a0
+    A = np.array([[-a1, 1.0], [-a2, 0.0]])
+    Bv = np.array([b1 - a1 * b0, b2 - a2 * b0])
+    C = np.array([1.0, 0.0])
+    h = np.empty(n)
+    h[0] = b0
+    P = np.eye(2)
+    for k in range(1, n):
+        h[k] = C @ P @ Bv
+        P = A @ P
+    return h

@@ -1,0 +1,129 @@
+"""Host-side logic on the CPU: planning, error behaviour, and the C ABI
+library's exported symbols (loaded, not called: no GPU here)."""
+
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+import torch
+
+from torchscience_amd import _lib
+from torchscience_amd.signal_processing.filter import sosfilt, lfilter, batched_filter_apply
+from torchscience_amd.signal_processing.filter._plan import plan_lfilter, plan_sosfilt
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_symbols():
+    text = open(os.path.join(ROOT, "include", "tsiir.h")).read()
+    return set(re.findall(r"\b(tsiir_[a-z_0-9]+)\s*\(", text))
+
+
+def test_header_and_binding_agree():
+    assert _header_symbols() == set(_lib.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_lib.LIB_PATH):
+        from torchscience_amd import build
+        build.build()
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    for name in _header_symbols():
+        assert hasattr(L, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    for name in _header_symbols():
+        assert re.search(rf"\bT {name}$", out, re.M), name
+    assert _lib.load().tsiir_version() == 1
+
+
+def test_desc_layout_matches_header():
+    assert ctypes.sizeof(_lib.SosfiltDesc) == 72
+    assert _lib.SosfiltDesc.chans_per_filter.offset == 64
+
+
+def test_workspace_size_query_without_gpu():
+    L = _lib.load()
+    d = _lib.SosfiltDesc(channels=4096, n_samples=65536, n_sections=6, io_dtype=0, coef_dtype=0,
+                         mode=0, flags=0, reserved=0, x_rows=4096, x_row_stride=65536,
+                         y_row_stride=65536, chans_per_filter=4096)
+    n = L.tsiir_sosfilt_workspace_bytes(ctypes.byref(d))
+    # header + one table entry per section + 16 windows x 4096 x 6 x 64 B of status
+    assert n == 256 + 6 * 64 * 8 + 16 * 4096 * 6 * 64
+    d.n_sections = 0
+    assert L.tsiir_sosfilt_workspace_bytes(ctypes.byref(d)) == 0
+    assert L.tsiir_status_string(2) == b"unsupported request"
+
+
+def test_sos_shape_error_matches_reference():
+    with pytest.raises(ValueError, match="shape"):
+        sosfilt(torch.ones(1, 3), torch.randn(10))
+    with pytest.raises(ValueError, match=r"sos must be shape \(n_sections, 6\)"):
+        sosfilt(torch.ones(2, 6, 1), torch.randn(10))
+
+
+def test_lfilter_1d_errors():
+    with pytest.raises(ValueError, match="b must be a 1-D tensor"):
+        lfilter(torch.ones(2, 2), torch.ones(2), torch.randn(10))
+    with pytest.raises(ValueError, match="a must be a 1-D tensor"):
+        lfilter(torch.ones(2), torch.ones(2, 2), torch.randn(10))
+    assert plan_lfilter((5,), (2,)) == 4
+    assert plan_lfilter((), ()) == 0
+
+
+def test_batched_errors():
+    with pytest.raises(ValueError, match=r"SOS filters must have shape \(batch, n_sections, 6\)"):
+        batched_filter_apply(torch.ones(2, 6), torch.randn(10))
+    with pytest.raises(ValueError, match="x batch dim must match filters"):
+        batched_filter_apply(torch.ones(3, 2, 6), torch.randn(2, 10), broadcast=False)
+    with pytest.raises(ValueError, match="filter_format must be 'sos' or 'fir'"):
+        batched_filter_apply(torch.ones(3, 2, 6), torch.randn(10), filter_format="ba")
+
+
+def test_cpu_tensors_fail_loudly():
+    with pytest.raises(RuntimeError, match="HIP"):
+        sosfilt(torch.ones(2, 6, dtype=torch.float64), torch.randn(10, dtype=torch.float64))
+
+
+def test_plan_promotion_and_zi_broadcast():
+    f32, f64 = torch.float32, torch.float64
+    p = plan_sosfilt((4, 6), f64, (3, 100), f32)
+    assert p.out_dtype == f64 and p.compute_dtype == f64 and p.channels == 3
+    p = plan_sosfilt((4, 6), f32, (100,), f32)
+    assert p.out_dtype == f32 and p.batch_shape == () and p.channels == 1
+    p = plan_sosfilt((4, 6), f32, (5, 100, 8), f32, axis=1, zi_shape=(4, 2), zi_dtype=f64)
+    assert p.zi_broadcast and p.zf_shape == (5, 8, 4, 2) and p.zf_dtype == f64
+    assert p.channels == 40 and p.n_samples == 100
+    p = plan_sosfilt((4, 6), f32, (100,), f32, zi_shape=(4, 2), zi_dtype=f32)
+    assert not p.zi_broadcast and p.zf_shape == (4, 2)
+    p = plan_sosfilt((4, 6), f32, (3, 100), torch.complex64)
+    assert p.complex_input and p.compute_dtype == f32
+    p = plan_sosfilt((4, 6), f32, (3, 100), torch.int64)
+    assert p.out_dtype == f32
+    p = plan_sosfilt((4, 6), f32, (3, 100), f32, precision="bf16io")
+    assert p.io_dtype == torch.bfloat16 and p.out_dtype == torch.bfloat16
+    with pytest.raises(RuntimeError, match="zi must have shape"):
+        plan_sosfilt((4, 6), f32, (3, 100), f32, zi_shape=(2, 4, 2), zi_dtype=f32)
+    with pytest.raises(NotImplementedError):
+        plan_sosfilt((4, 6), torch.complex128, (3, 100), f32)
+
+
+def test_design_matches_scipy_magnitude():
+    import numpy as np
+    import scipy.signal as ss
+    from torchscience_amd.signal_processing.filter._design import (butterworth_sos,
+                                                                    chebyshev1_sos)
+    pairs = [(butterworth_sos(6, [0.05, 0.3], "bandpass"),
+              ss.butter(6, [0.05, 0.3], "bandpass", output="sos")),
+             (butterworth_sos(4, 0.125), ss.butter(4, 0.125, output="sos")),
+             (butterworth_sos(8, 0.33), ss.butter(8, 0.33, output="sos")),
+             (chebyshev1_sos(16, 0.01, 1.0), ss.cheby1(16, 1.0, 0.01, output="sos"))]
+    for sos, ref in pairs:
+        _, h = ss.sosfreqz(sos, 4096)
+        _, h2 = ss.sosfreqz(ref, 4096)
+        assert np.max(np.abs(np.abs(h) - np.abs(h2))) / np.max(np.abs(h2)) < 1e-10
+    # the reference's even gain split: every section carries |k|^(1/S)
+    sos = butterworth_sos(4, 0.125)
+    assert np.allclose(sos[:, 0], sos[0, 0]) and sos.shape == (2, 6)

@@ -1,0 +1,165 @@
+"""Pins of the FP64 oracle (CPU).
+
+Each check is one a plausible oracle mistake (dropped term, wrong sign or
+index, swapped coefficient, wrong section order, bad state hand-off) fails:
+scipy golden vectors, the biquad closed form, FIR = convolution, the
+state-space brute force, zf -> zi continuation and the steady state of
+sosfilt_zi, plus planted-mistake checks showing the goldens catch them.
+"""
+
+import numpy as np
+import pytest
+
+from tests.conftest import golden_cases
+
+RTOL = ATOL = 1e-10  # the reference tests' tolerance (test__sosfilt.py:22)
+
+
+def test_sosfilt_matches_scipy_goldens(golden, oracle_mod):
+    cases = golden_cases(golden, "sos")
+    assert len(cases) >= 15
+    for name, c in cases.items():
+        axis = int(c["axis"])
+        if "zi" in c:
+            y, zf = oracle_mod.sosfilt(c["sos"], c["x"], axis=axis, zi=c["zi"])
+            np.testing.assert_allclose(zf, c["zf"], rtol=RTOL, atol=ATOL, err_msg=name)
+        else:
+            y = oracle_mod.sosfilt(c["sos"], c["x"], axis=axis)
+        tol = 1e-9 if name == "butter12_many" else RTOL
+        np.testing.assert_allclose(y, c["y"], rtol=tol, atol=tol, err_msg=name)
+
+
+def test_lfilter_matches_scipy_goldens(golden, oracle_mod):
+    cases = golden_cases(golden, "lf")
+    assert len(cases) >= 10
+    for name, c in cases.items():
+        if "zi" in c:
+            y, zf = oracle_mod.lfilter(c["b"], c["a"], c["x"], zi=c["zi"])
+            np.testing.assert_allclose(zf, c["zf"], rtol=RTOL, atol=ATOL, err_msg=name)
+        else:
+            y = oracle_mod.lfilter(c["b"], c["a"], c["x"])
+        np.testing.assert_allclose(y, c["y"], rtol=RTOL, atol=ATOL, err_msg=name)
+
+
+def test_zi_helpers_match_scipy(golden, oracle_mod):
+    for name in ("butter5", "butter6"):
+        np.testing.assert_allclose(oracle_mod.sosfilt_zi(golden[f"zi/sos/{name}/sos"]),
+                                   golden[f"zi/sos/{name}/zi"], rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(
+        oracle_mod.lfilter_zi(golden["zi/lf/butter5/b"], golden["zi/lf/butter5/a"]),
+        golden["zi/lf/butter5/zi"], rtol=1e-10, atol=1e-12)
+
+
+def test_config1_golden(golden, oracle_mod):
+    import torch
+    g = torch.Generator().manual_seed(0)
+    x = torch.randn(16, 4096, dtype=torch.float64, generator=g).numpy()
+    chk = golden["c1/x_checksum"]
+    np.testing.assert_allclose([x.sum(), (x * x).sum(), x[3, 77], x[15, 4095]], chk,
+                               rtol=0, atol=1e-9)
+    y, zf = oracle_mod.sosfilt_rows(golden["c1/sos"], x, np.zeros((16, 4, 2)))
+    np.testing.assert_allclose(y, golden["c1/y"], rtol=1e-10, atol=1e-12)
+
+
+def test_biquad_impulse_closed_form(oracle_mod):
+    # poles r e^{+-j th}: h[n] = r^n sin((n+1) th) / sin th for b = [1, 0, 0]
+    r, th = 0.93, 0.41
+    sos = np.array([[1.0, 0.0, 0.0, 1.0, -2 * r * np.cos(th), r * r]])
+    x = np.zeros(300)
+    x[0] = 1.0
+    y = oracle_mod.sosfilt(sos, x)
+    n = np.arange(300)
+    np.testing.assert_allclose(y, r ** n * np.sin((n + 1) * th) / np.sin(th), rtol=1e-12,
+                               atol=1e-14)
+
+
+def test_fir_equals_convolution(oracle_mod):
+    rng = np.random.default_rng(1)
+    b = rng.standard_normal(9)
+    x = rng.standard_normal((3, 257))
+    y = oracle_mod.lfilter(b, np.array([1.0]), x)
+    ref = np.stack([np.convolve(row, b)[:257] for row in x])
+    np.testing.assert_allclose(y, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_state_space_brute_force(oracle_mod):
+    # y = Toeplitz(h) x + O zi for one section, h from the Markov parameters
+    rng = np.random.default_rng(2)
+    sos = np.array([[0.3, -0.2, 0.5, 1.0, -1.1, 0.45]])
+    T = 48
+    x = rng.standard_normal(T)
+    zi = rng.standard_normal((1, 2))
+    h = oracle_mod.impulse_state_response(sos[0], T)
+    A = np.array([[1.1, 1.0], [-0.45, 0.0]])
+    O = np.zeros((T, 2))
+    P = np.eye(2)
+    for n in range(T):
+        O[n] = P[0]  # y[n] = C A^n zi
+        P = A @ P
+    Tm = np.zeros((T, T))
+    for i in range(T):
+        Tm[i, : i + 1] = h[: i + 1][::-1]
+    y, zf = oracle_mod.sosfilt(sos, x, zi=zi)
+    np.testing.assert_allclose(y, Tm @ x + O @ zi[0], rtol=1e-11, atol=1e-12)
+
+
+def test_continuation_and_steady_state(oracle_mod):
+    import scipy.signal as ss
+    sos = ss.butter(4, 0.3, output="sos")
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((4, 400))
+    y_full = oracle_mod.sosfilt(sos, x)
+    y1, zf = oracle_mod.sosfilt(sos, x[:, :173], zi=np.zeros((4, 2, 2)))
+    y2, _ = oracle_mod.sosfilt(sos, x[:, 173:], zi=zf)
+    np.testing.assert_allclose(np.concatenate([y1, y2], -1), y_full, rtol=1e-12, atol=1e-12)
+    sos6 = ss.butter(6, 0.2, output="sos")
+    y, _ = oracle_mod.sosfilt(sos6, np.ones(50), zi=oracle_mod.sosfilt_zi(sos6))
+    np.testing.assert_allclose(y, np.ones(50), rtol=1e-10, atol=1e-10)
+
+
+def test_linearity_and_time_invariance(oracle_mod):
+    import scipy.signal as ss
+    sos = ss.cheby1(6, 0.5, 0.3, output="sos")
+    rng = np.random.default_rng(4)
+    x1, x2 = rng.standard_normal((2, 500))
+    a, b = 1.7, -0.4
+    np.testing.assert_allclose(oracle_mod.sosfilt(sos, a * x1 + b * x2),
+                               a * oracle_mod.sosfilt(sos, x1) + b * oracle_mod.sosfilt(sos, x2),
+                               rtol=1e-10, atol=1e-11)
+    xs = np.concatenate([np.zeros(37), x1])
+    np.testing.assert_allclose(oracle_mod.sosfilt(sos, xs)[37:], oracle_mod.sosfilt(sos, x1),
+                               rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("mutation", ["swap_a", "drop_b2", "reverse_sections"])
+def test_planted_mistakes_are_caught(golden, oracle_mod, mutation):
+    """A wrong oracle (planted here, on the CPU) fails the golden check."""
+    c = golden_cases(golden, "sos")["butter5_025"]
+    sos = np.array(c["sos"])
+    if mutation == "swap_a":
+        sos[:, [4, 5]] = sos[:, [5, 4]]
+    elif mutation == "drop_b2":
+        sos[:, 2] = 0.0
+    else:
+        sos = sos[::-1].copy()
+    y = oracle_mod.sosfilt(sos, c["x"])
+    if mutation == "reverse_sections":
+        # the product of section transfer functions commutes in exact
+        # arithmetic; the golden still fixes the rounding order to 1e-10
+        assert np.max(np.abs(y - c["y"])) < 1e-10
+    else:
+        assert not np.allclose(y, c["y"], rtol=RTOL, atol=ATOL)
+
+
+def test_mixed_complex(golden, oracle_mod):
+    import scipy.signal as ss
+    rng = np.random.default_rng(5)
+    sos = ss.butter(3, 0.3, output="sos")
+    x = rng.standard_normal(100) + 1j * rng.standard_normal(100)
+    np.testing.assert_allclose(oracle_mod.sosfilt(sos, x), ss.sosfilt(sos, x), rtol=1e-10,
+                               atol=1e-10)
+    b = np.array([0.5 + 0.1j, 0.5 - 0.1j])
+    a = np.array([1.0, -0.3 + 0.1j])
+    xr = rng.standard_normal(100)
+    np.testing.assert_allclose(oracle_mod.lfilter(b, a, xr), ss.lfilter(b, a, xr), rtol=1e-10,
+                               atol=1e-10)

@@ -1,0 +1,155 @@
+"""ctypes binding of ``libtsiir.so`` (C ABI declared in include/tsiir.h).
+
+Torch is used only for device memory and streams: tensors hand their device
+pointers and the current HIP stream to the C entry points.  There is no
+fallback: if the library is missing the import of the filtering ops raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "_lib", "libtsiir.so")
+
+OK, EINVAL, EUNSUPPORTED, ELAUNCH, EWORKSPACE = range(5)
+F32, F64, BF16 = 0, 1, 2
+MODE_PLAIN, MODE_COMPENSATED = 0, 1
+HAS_ZI, WRITE_ZF, ZI_BROADCAST, REVERSE = 0x1, 0x2, 0x4, 0x8
+
+# Every symbol include/tsiir.h declares.
+EXPORTED = (
+    "tsiir_version",
+    "tsiir_status_string",
+    "tsiir_sosfilt_workspace_bytes",
+    "tsiir_workspace_init",
+    "tsiir_sosfilt_forward",
+    "tsiir_lfilter_forward",
+    "tsiir_sosfilt_backward_workspace_bytes",
+    "tsiir_sosfilt_backward",
+)
+
+
+class SosfiltDesc(ctypes.Structure):
+    _fields_ = [
+        ("channels", ctypes.c_int64),
+        ("n_samples", ctypes.c_int64),
+        ("n_sections", ctypes.c_int32),
+        ("io_dtype", ctypes.c_int32),
+        ("coef_dtype", ctypes.c_int32),
+        ("mode", ctypes.c_int32),
+        ("flags", ctypes.c_uint32),
+        ("reserved", ctypes.c_int32),
+        ("x_rows", ctypes.c_int64),
+        ("x_row_stride", ctypes.c_int64),
+        ("y_row_stride", ctypes.c_int64),
+        ("chans_per_filter", ctypes.c_int64),
+    ]
+
+
+class TsiirError(RuntimeError):
+    """A libtsiir entry point returned a non-OK status."""
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load(path: str | None = None) -> ctypes.CDLL:
+    """Load libtsiir.so (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise ImportError(
+            f"libtsiir.so not found at {p}; build it with `python -m torchscience_amd.build`")
+    L = ctypes.CDLL(p)
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    dp = ctypes.POINTER(SosfiltDesc)
+    L.tsiir_version.restype = ctypes.c_int
+    L.tsiir_status_string.restype = ctypes.c_char_p
+    L.tsiir_status_string.argtypes = [ctypes.c_int]
+    L.tsiir_sosfilt_workspace_bytes.restype = sz
+    L.tsiir_sosfilt_workspace_bytes.argtypes = [dp]
+    L.tsiir_workspace_init.restype = ctypes.c_int
+    L.tsiir_workspace_init.argtypes = [vp, sz, vp]
+    L.tsiir_sosfilt_forward.restype = ctypes.c_int
+    L.tsiir_sosfilt_forward.argtypes = [dp, vp, vp, vp, vp, vp, vp, sz, vp]
+    L.tsiir_lfilter_forward.restype = ctypes.c_int
+    L.tsiir_lfilter_forward.argtypes = [dp, vp, vp, vp, vp, vp, vp]
+    L.tsiir_sosfilt_backward_workspace_bytes.restype = sz
+    L.tsiir_sosfilt_backward_workspace_bytes.argtypes = [dp]
+    L.tsiir_sosfilt_backward.restype = ctypes.c_int
+    L.tsiir_sosfilt_backward.argtypes = [dp, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, vp]
+    if path is None:
+        _lib = L
+    return L
+
+
+def check(status: int, what: str) -> None:
+    if status != OK:
+        msg = load().tsiir_status_string(status).decode()
+        raise TsiirError(f"{what} failed: {msg} (status {status})")
+
+
+def ptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def stream_handle(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def io_code(dtype: torch.dtype) -> int:
+    if dtype == torch.float32:
+        return F32
+    if dtype == torch.float64:
+        return F64
+    if dtype == torch.bfloat16:
+        return BF16
+    raise TypeError(f"unsupported I/O dtype {dtype}")
+
+
+class _Workspaces:
+    """One zero-initialised workspace per (device, stream), grown on demand.
+
+    The kernels keep the workspace header consistent across launches, so a
+    buffer is initialised once, when it is allocated.  Keying by stream keeps
+    concurrent streams from sharing scan status words."""
+
+    def __init__(self):
+        self._bufs: dict[tuple[int, int, str], torch.Tensor] = {}
+
+    def get(self, device: torch.device, nbytes: int, kind: str = "fwd") -> torch.Tensor:
+        stream = stream_handle(device)
+        key = (device.index if device.index is not None else torch.cuda.current_device(),
+               stream, kind)
+        with _lock:
+            buf = self._bufs.get(key)
+            if buf is None or buf.numel() < nbytes:
+                size = max(nbytes, 1 << 20)
+                size = (size + 255) // 256 * 256
+                buf = torch.empty(size, dtype=torch.uint8, device=device)
+                check(load().tsiir_workspace_init(buf.data_ptr(), size, stream),
+                      "tsiir_workspace_init")
+                self._bufs[key] = buf
+            return buf
+
+    def header_error(self, device: torch.device, kind: str = "fwd") -> int:
+        """Error word of the workspace header (non-zero after a look-back
+        wait hit its spin limit).  Synchronises the stream."""
+        stream = stream_handle(device)
+        key = (device.index if device.index is not None else torch.cuda.current_device(),
+               stream, kind)
+        buf = self._bufs.get(key)
+        if buf is None:
+            return 0
+        return int(buf[12:16].view(torch.int32).item())
+
+
+workspaces = _Workspaces()

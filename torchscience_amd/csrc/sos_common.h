@@ -1,0 +1,392 @@
+// sos_common.h - shared device helpers for the gfx950 SOS kernels.
+//
+// Layout constants, the workspace header, coefficient tables and the
+// self-validating scan status words used by the decoupled look-back.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tsiir {
+
+constexpr int kWave = 64;        // CDNA wavefront
+constexpr int kNPow = 13;        // A^(W*2^i), i = 0..12
+constexpr int kTableStride = 64; // doubles per (filter, section) table entry
+constexpr int kStatusWords = 8;  // 4 words aggregate + 4 words inclusive
+constexpr uint32_t kSpinLimit = 1u << 18;
+
+// Workspace header (256 bytes).  `epoch` advances before every launch that
+// publishes scan status words (see launch_flag).
+struct WsHeader {
+  uint32_t reserved0;
+  uint32_t reserved1;
+  uint32_t epoch;
+  uint32_t error;  // set when a look-back wait exceeded kSpinLimit
+  uint32_t pad[60];
+};
+static_assert(sizeof(WsHeader) == 256, "header size");
+
+struct Mat2 {
+  double a, b, c, d;  // [[a, b], [c, d]]
+};
+
+__device__ __forceinline__ Mat2 mat_mul(const Mat2& x, const Mat2& y) {
+  return Mat2{x.a * y.a + x.b * y.c, x.a * y.b + x.b * y.d,
+              x.c * y.a + x.d * y.c, x.c * y.b + x.d * y.d};
+}
+
+__device__ __forceinline__ Mat2 load_mat(const double* t) {
+  return Mat2{t[0], t[1], t[2], t[3]};
+}
+
+// Table entry of one (filter, section):
+//   [0..4]  b0 b1 b2 a1 a2 (normalised by a0)
+//   [8 + 4i .. 8 + 4i + 3]  A^(W * 2^i) for i < kNPow, where
+//   A = [[-a1, 1], [-a2, 0]] is the DF2T state matrix of the section.
+__device__ __forceinline__ const double* table_pow(const double* t, int i) {
+  return t + 8 + 4 * i;
+}
+
+// 64-bit status word = (flag << 32) | 32-bit payload.  A double is stored
+// as two words (low half, high half).  Each word is written and read with a
+// single 64-bit relaxed atomic at agent scope, so a reader that sees the
+// current flag in every word has a consistent value without any fence.
+__device__ __forceinline__ uint64_t atomic_ld64(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_st64(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Lanes 0..3 of the calling wave each write one word of (v0, v1).
+__device__ __forceinline__ void publish_pair(uint64_t* words, double v0, double v1,
+                                             uint32_t flag, int lane) {
+  if (lane < 4) {
+    const double v = (lane < 2) ? v0 : v1;
+    const uint64_t bits = static_cast<uint64_t>(__double_as_longlong(v));
+    const uint32_t half = (lane & 1) ? static_cast<uint32_t>(bits >> 32)
+                                     : static_cast<uint32_t>(bits);
+    atomic_st64(words + lane, (static_cast<uint64_t>(flag) << 32) | half);
+  }
+}
+
+// Reads a pair published by publish_pair; returns false unless all four
+// words carry `flag`.
+__device__ __forceinline__ bool read_pair(const uint64_t* words, uint32_t flag,
+                                          double& v0, double& v1) {
+  const uint64_t w0 = atomic_ld64(words + 0);
+  const uint64_t w1 = atomic_ld64(words + 1);
+  const uint64_t w2 = atomic_ld64(words + 2);
+  const uint64_t w3 = atomic_ld64(words + 3);
+  const bool ok = (static_cast<uint32_t>(w0 >> 32) == flag) &
+                  (static_cast<uint32_t>(w1 >> 32) == flag) &
+                  (static_cast<uint32_t>(w2 >> 32) == flag) &
+                  (static_cast<uint32_t>(w3 >> 32) == flag);
+  const uint64_t b0 = (w0 & 0xffffffffull) | ((w1 & 0xffffffffull) << 32);
+  const uint64_t b1 = (w2 & 0xffffffffull) | ((w3 & 0xffffffffull) << 32);
+  v0 = __longlong_as_double(static_cast<long long>(b0));
+  v1 = __longlong_as_double(static_cast<long long>(b1));
+  return ok;
+}
+
+// ---- wave-level scan of 2x2 affine maps -----------------------------------
+//
+// Lane q holds the zero-state end state e_q of its W samples.  With
+// M = A^W (table power 0), the inclusive scan gives
+//   incl_q = sum_{p <= q} M^(q-p) e_p,
+// the window aggregate incl_63 and the exclusive value incl_(q-1).
+__device__ __forceinline__ void wave_scan(const double* tb, int lane, double e0, double e1,
+                                          double& agg0, double& agg1, double& x0, double& x1) {
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const int d = 1 << i;
+    const double n0 = __shfl_up(e0, d);
+    const double n1 = __shfl_up(e1, d);
+    const Mat2 P = load_mat(table_pow(tb, i));
+    if (lane >= d) {
+      e0 += P.a * n0 + P.b * n1;
+      e1 += P.c * n0 + P.d * n1;
+    }
+  }
+  agg0 = __shfl(e0, kWave - 1);
+  agg1 = __shfl(e1, kWave - 1);
+  x0 = __shfl_up(e0, 1);
+  x1 = __shfl_up(e1, 1);
+  if (lane == 0) {
+    x0 = 0.0;
+    x1 = 0.0;
+  }
+}
+
+// Incoming state of lane q: M^q s_window + exclusive_q (M^q from the binary
+// digits of q and the table powers M^(2^i)).
+__device__ __forceinline__ void lane_incoming(const double* tb, int lane, double sw0, double sw1,
+                                              double x0, double x1, double& s0, double& s1) {
+  s0 = sw0;
+  s1 = sw1;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const Mat2 P = load_mat(table_pow(tb, i));
+    if ((lane >> i) & 1) {
+      const double t0 = P.a * s0 + P.b * s1;
+      const double t1 = P.c * s0 + P.d * s1;
+      s0 = t0;
+      s1 = t1;
+    }
+  }
+  s0 += x0;
+  s1 += x1;
+}
+
+// Decoupled look-back: incoming state of window k of (channel c, section s),
+// the same value in every lane.  Lane l inspects window k-1-l; the nearest
+// window with an inclusive state (or the virtual window -1 holding zi) ends
+// the search, windows in between contribute their aggregates:
+//   in_k = sum_{l < f} Mw^l agg_(k-1-l) + Mw^f incl_(k-1-f),  Mw = A^(64W),
+// combined by a shuffle-down reduction with Mw^(2^i) = table power 6+i.  A
+// wait longer than kSpinLimit polls sets hdr->error and gives up, so every
+// wave terminates.
+__device__ __forceinline__ void lookback(const uint64_t* status, int64_t B, int S, WsHeader* hdr,
+                                         int64_t k, int64_t c, int s, uint32_t flag,
+                                         const double* tb, double zi0, double zi1, double& sw0,
+                                         double& sw1) {
+  const int lane = threadIdx.x;
+  double acc0 = 0.0, acc1 = 0.0;
+  Mat2 P{1.0, 0.0, 0.0, 1.0};
+  int64_t base = k - 1;
+  uint32_t spins = 0;
+  while (true) {
+    const int64_t j = base - lane;
+    int st = 2;
+    double v0 = 0.0, v1 = 0.0;
+    if (j >= 0) {
+      const uint64_t* e = status + ((j * B + c) * S + s) * kStatusWords;
+      if (read_pair(e + 4, flag, v0, v1)) {
+        st = 2;
+      } else if (read_pair(e, flag, v0, v1)) {
+        st = 1;
+      } else {
+        st = 0;
+      }
+    } else if (j == -1) {
+      v0 = zi0;
+      v1 = zi1;
+    }
+    const uint64_t incl = __ballot(st == 2);
+    const uint64_t ready = __ballot(st != 0);
+    const int f = incl ? (__ffsll(static_cast<unsigned long long>(incl)) - 1) : 64;
+    const uint64_t need = (f >= 63) ? ~0ull : ((1ull << (f + 1)) - 1ull);
+    if ((ready & need) != need) {
+      if (++spins > kSpinLimit) {
+        if (lane == 0) atomicOr(&hdr->error, 1u);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+      continue;
+    }
+    if (f == 0) {  // the nearest window is already inclusive (the common case)
+      const double r0 = __shfl(v0, 0);
+      const double r1 = __shfl(v1, 0);
+      acc0 += P.a * r0 + P.b * r1;
+      acc1 += P.c * r0 + P.d * r1;
+      break;
+    }
+    if (lane > f) {
+      v0 = 0.0;
+      v1 = 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const int d = 1 << i;
+      const double n0 = __shfl_down(v0, d);
+      const double n1 = __shfl_down(v1, d);
+      const Mat2 Q = load_mat(table_pow(tb, 6 + i));
+      if (lane + d < kWave) {
+        v0 += Q.a * n0 + Q.b * n1;
+        v1 += Q.c * n0 + Q.d * n1;
+      }
+    }
+    const double r0 = __shfl(v0, 0);
+    const double r1 = __shfl(v1, 0);
+    acc0 += P.a * r0 + P.b * r1;
+    acc1 += P.c * r0 + P.d * r1;
+    if (f < 64) break;
+    P = mat_mul(P, load_mat(table_pow(tb, 12)));
+    base -= kWave;
+  }
+  sw0 = acc0;
+  sw1 = acc1;
+}
+
+// Epoch of the running launch (wave-uniform).  The prep kernel that runs
+// before every launch advances hdr->epoch, so status words written by earlier
+// launches never carry a current flag.  A call that makes several launches
+// reserves one epoch per launch and launch i uses epoch - i.
+__device__ __forceinline__ uint32_t launch_flag(const WsHeader* hdr, uint32_t back) {
+  return __builtin_amdgcn_readfirstlane(
+             __hip_atomic_load(&hdr->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) -
+         back;
+}
+
+// ---- element I/O ---------------------------------------------------------
+
+struct bf16_t {
+  uint16_t bits;
+};
+
+template <typename T>
+struct IOTraits;
+
+template <>
+struct IOTraits<float> {
+  using vec_t = float4;
+  static constexpr int kVec = 4;
+  __device__ static __forceinline__ float to_f(float v) { return v; }
+  __device__ static __forceinline__ float from_f(float v) { return v; }
+};
+
+template <>
+struct IOTraits<double> {
+  using vec_t = double2;
+  static constexpr int kVec = 2;
+  __device__ static __forceinline__ double to_f(double v) { return v; }
+  __device__ static __forceinline__ double from_f(double v) { return v; }
+};
+
+template <>
+struct IOTraits<bf16_t> {
+  using vec_t = uint4;
+  static constexpr int kVec = 8;
+  __device__ static __forceinline__ float to_f(bf16_t v) {
+    return __uint_as_float(static_cast<uint32_t>(v.bits) << 16);
+  }
+  // round to nearest even; NaN stays NaN
+  __device__ static __forceinline__ bf16_t from_f(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return bf16_t{static_cast<uint16_t>((u >> 16) | 0x40)};
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return bf16_t{static_cast<uint16_t>(u >> 16)};
+  }
+};
+
+// Load W consecutive elements into registers (converted to TACC).  `vec`
+// selects 16-byte loads (row start 16-byte aligned and all W in range);
+// otherwise elements at or beyond `m` read as zero.
+template <typename TIO, typename TACC, int W>
+__device__ __forceinline__ void load_run(const TIO* p, bool vec, int m, TACC (&v)[W]) {
+  using Tr = IOTraits<TIO>;
+  if (vec) {
+    const typename Tr::vec_t* q = reinterpret_cast<const typename Tr::vec_t*>(p);
+#pragma unroll
+    for (int i = 0; i < W / Tr::kVec; ++i) {
+      const typename Tr::vec_t r = q[i];
+      const TIO* e = reinterpret_cast<const TIO*>(&r);
+#pragma unroll
+      for (int j = 0; j < Tr::kVec; ++j) v[i * Tr::kVec + j] = static_cast<TACC>(Tr::to_f(e[j]));
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < W; ++j) v[j] = (j < m) ? static_cast<TACC>(Tr::to_f(p[j])) : TACC(0);
+  }
+}
+
+// ---- LDS-staged window I/O --------------------------------------------------
+//
+// A wave's window is 64*W contiguous elements; lane q owns [q*W, (q+1)*W).
+// Global memory is read and written with fully coalesced 16-byte accesses
+// (instruction i, lane l -> vector i*64 + l) and transposed through a
+// per-wave LDS buffer of 32 padded rows, in two rounds of 32 lanes.
+template <typename TIO, int W>
+struct LdsStage {
+  static constexpr int kVec = IOTraits<TIO>::kVec;      // elements per 16 bytes
+  static constexpr int kNV = W / kVec;                  // 16-byte vectors per lane
+  static constexpr int kRow = W + kVec;                 // padded row (elements)
+  static constexpr int kElems = 32 * kRow;              // buffer size (elements)
+};
+
+template <typename TIO, typename TACC, int W>
+__device__ __forceinline__ void load_window_lds(const TIO* win, TIO* lds, TACC (&v)[W]) {
+  using L = LdsStage<TIO, W>;
+  using vec_t = typename IOTraits<TIO>::vec_t;
+  const int lane = threadIdx.x;
+  const vec_t* g = reinterpret_cast<const vec_t*>(win);
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    vec_t raw[L::kNV / 2];
+#pragma unroll
+    for (int i = 0; i < L::kNV / 2; ++i) raw[i] = g[(r * L::kNV / 2 + i) * kWave + lane];
+#pragma unroll
+    for (int i = 0; i < L::kNV / 2; ++i) {
+      const int vi = (r * L::kNV / 2 + i) * kWave + lane;  // vector index in the window
+      const int q = vi / L::kNV - 32 * r;                    // row within this round
+      const int p = (vi % L::kNV) * L::kVec;                 // element column
+      *reinterpret_cast<vec_t*>(lds + q * L::kRow + p) = raw[i];
+    }
+    __syncthreads();
+    if ((lane >> 5) == r) {
+      const TIO* row = lds + (lane & 31) * L::kRow;
+#pragma unroll
+      for (int i = 0; i < L::kNV; ++i) {
+        const vec_t rv = *reinterpret_cast<const vec_t*>(row + i * L::kVec);
+        const TIO* e = reinterpret_cast<const TIO*>(&rv);
+#pragma unroll
+        for (int j = 0; j < L::kVec; ++j)
+          v[i * L::kVec + j] = static_cast<TACC>(IOTraits<TIO>::to_f(e[j]));
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename TIO, typename TACC, int W>
+__device__ __forceinline__ void store_window_lds(TIO* win, TIO* lds, const TACC (&v)[W]) {
+  using L = LdsStage<TIO, W>;
+  using vec_t = typename IOTraits<TIO>::vec_t;
+  const int lane = threadIdx.x;
+  vec_t* g = reinterpret_cast<vec_t*>(win);
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    if ((lane >> 5) == r) {
+      TIO* row = lds + (lane & 31) * L::kRow;
+#pragma unroll
+      for (int i = 0; i < L::kNV; ++i) {
+        vec_t rv;
+        TIO* e = reinterpret_cast<TIO*>(&rv);
+#pragma unroll
+        for (int j = 0; j < L::kVec; ++j) e[j] = IOTraits<TIO>::from_f(v[i * L::kVec + j]);
+        *reinterpret_cast<vec_t*>(row + i * L::kVec) = rv;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = r * L::kNV / 2; i < (r + 1) * L::kNV / 2; ++i) {
+      const int vi = i * kWave + lane;
+      const int q = vi / L::kNV - 32 * r;
+      const int p = (vi % L::kNV) * L::kVec;
+      g[vi] = *reinterpret_cast<const vec_t*>(lds + q * L::kRow + p);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename TIO, typename TACC, int W>
+__device__ __forceinline__ void store_run(TIO* p, bool vec, int m, const TACC (&v)[W]) {
+  using Tr = IOTraits<TIO>;
+  if (vec) {
+    typename Tr::vec_t* q = reinterpret_cast<typename Tr::vec_t*>(p);
+#pragma unroll
+    for (int i = 0; i < W / Tr::kVec; ++i) {
+      typename Tr::vec_t r;
+      TIO* e = reinterpret_cast<TIO*>(&r);
+#pragma unroll
+      for (int j = 0; j < Tr::kVec; ++j) e[j] = Tr::from_f(v[i * Tr::kVec + j]);
+      q[i] = r;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+      if (j < m) p[j] = Tr::from_f(v[j]);
+  }
+}
+
+}  // namespace tsiir

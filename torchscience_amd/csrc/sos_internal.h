@@ -1,0 +1,39 @@
+// sos_internal.h - host-side entry points shared between the kernel files
+// of libtsiir (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "tsiir.h"
+
+namespace tsiir {
+
+size_t align_up(size_t v, size_t a);
+
+struct FwdGeometry {
+  int W;               // samples per lane
+  int64_t n_windows;   // windows of 64*W samples per channel
+  int64_t n_filters;
+  size_t table_bytes;
+  size_t status_bytes;
+  size_t total;        // workspace bytes
+};
+
+bool fwd_geometry(const tsiir_sosfilt_desc* d, FwdGeometry* g);
+
+// Coefficient tables (see sos_common.h) for n_rows (filter, section) rows.
+// Also advances the workspace epoch by n_epochs (one per following launch
+// that publishes scan status words).
+struct WsHeader;
+void launch_prep(const void* sos, int coef_dtype, int64_t n_rows, int log2w, bool adjoint,
+                 double* tables, WsHeader* hdr, uint32_t n_epochs, hipStream_t stream);
+
+// tsiir_sosfilt_forward, optionally also writing the outputs of sections
+// 0..S-2 to `saved` ((S-1, channels, T) in the I/O type).
+tsiir_status forward_impl(const tsiir_sosfilt_desc* d, const void* x, void* y, const void* sos,
+                          const void* zi, void* zf, void* workspace, size_t workspace_bytes,
+                          void* stream, void* saved);
+
+}  // namespace tsiir

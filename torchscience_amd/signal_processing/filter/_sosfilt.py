@@ -1,0 +1,102 @@
+"""sosfilt on MI355X: drop-in for torchscience.signal_processing.filter.sosfilt.
+
+Reference: src/torchscience/signal_processing/filter/_sosfilt.py:13-133.
+Same signature, return arity ((y) or (y, zf) iff zi is given), ValueError
+text, zi broadcasting, dtype promotion and autograd reach (x, sos, zi).
+The cascade runs in one kernel pass per call instead of S x T Python
+iterations; gradients come from the adjoint kernels.
+"""
+
+from __future__ import annotations
+
+from typing import Optional, Union
+
+import torch
+from torch import Tensor
+
+from ._ops import SosfiltRows, sosfilt_rows_forward
+from ._plan import plan_sosfilt
+
+
+def _to_rows(x: Tensor, ax: int, T: int) -> Tensor:
+    return torch.movedim(x, ax, -1).reshape(-1, T)
+
+
+def sosfilt(
+    sos: Tensor,
+    x: Tensor,
+    axis: int = -1,
+    zi: Optional[Tensor] = None,
+    *,
+    precision: Optional[str] = None,
+) -> Union[Tensor, tuple[Tensor, Tensor]]:
+    """Filter x along `axis` with cascaded second-order sections.
+
+    sos is (n_sections, 6) with rows [b0, b1, b2, a0, a1, a2]; each row is
+    normalised by a0 when a0 != 1.  zi, if given, is (..., n_sections, 2)
+    with x's batch dims, or (n_sections, 2) for every channel; then
+    (y, zf) is returned, zf with zi's dtype.
+
+    ``precision`` (keyword-only, not in the reference) selects the kernel
+    form: None follows the reference's dtype promotion; "fp32" / "fp64"
+    force the compute type; "bf16io" reads and writes bfloat16 with float32
+    arithmetic and returns bfloat16.
+    """
+    plan = plan_sosfilt(sos.shape, sos.dtype, x.shape, x.dtype, axis,
+                        None if zi is None else zi.shape,
+                        None if zi is None else zi.dtype, precision)
+    S, T, B = plan.n_sections, plan.n_samples, plan.channels
+    comp = plan.compute_dtype
+    dev = x.device
+    if S == 0:
+        # no sections: y = x.clone() and an (empty) zf, as _sosfilt.py:96,114
+        y = x.clone()
+        if zi is None:
+            return y
+        return y, torch.zeros(plan.zf_shape, dtype=zi.dtype, device=zi.device)
+    sos_c = sos.to(device=dev, dtype=comp).reshape(1, S, 6).contiguous()
+
+    if plan.complex_input:
+        # Real coefficients act on the real and imaginary parts separately.
+        xc = _to_rows(x.to(plan.out_dtype), plan.axis, T)
+        xr = torch.view_as_real(xc).permute(0, 2, 1).reshape(2 * B, T)
+        zr = None
+        if zi is not None:
+            zc = zi.to(device=dev, dtype=plan.out_dtype)
+            zc = zc.expand(plan.zf_shape) if plan.zi_broadcast else zc
+            zr = torch.view_as_real(zc.reshape(B, S, 2)).permute(0, 3, 1, 2).reshape(2 * B, S, 2)
+            zr = zr.to(comp).contiguous()
+        y2, zf2 = _run(sos_c, xr.to(comp).contiguous(), zr, 2 * B, comp, comp, zi is not None,
+                       False)
+        y = torch.view_as_complex(y2.reshape(B, 2, T).permute(0, 2, 1).contiguous())
+        y = torch.movedim(y.reshape(plan.batch_shape + (T,)), -1, plan.axis)
+        if zi is None:
+            return y
+        zf = torch.view_as_complex(zf2.reshape(B, 2, S, 2).permute(0, 2, 3, 1).contiguous())
+        return y, zf.reshape(plan.zf_shape).to(zi.dtype)
+
+    xr = _to_rows(x, plan.axis, T).to(plan.io_dtype).contiguous()
+    zr = None
+    if zi is not None:
+        zr = zi.to(device=dev, dtype=comp)
+        zr = zr.reshape(1, S, 2) if plan.zi_broadcast else zr.reshape(B, S, 2)
+        zr = zr.contiguous()
+    y, zf = _run(sos_c, xr, zr, B, comp, plan.io_dtype, zi is not None, plan.zi_broadcast)
+    y = torch.movedim(y.reshape(plan.batch_shape + (T,)), -1, plan.axis)
+    if y.dtype != plan.out_dtype:
+        y = y.to(plan.out_dtype)
+    if zi is None:
+        return y
+    return y, zf.reshape(plan.zf_shape).to(zi.dtype)
+
+
+def _run(sos_c, xr, zr, B, comp, io, want_zf, zi_b):
+    needs_grad = torch.is_grad_enabled() and (
+        sos_c.requires_grad or xr.requires_grad or (zr is not None and zr.requires_grad))
+    if needs_grad and io == comp:
+        y, zf = SosfiltRows.apply(xr, sos_c, zr, B, B, want_zf, zi_b)
+        return y, zf
+    if needs_grad:
+        raise NotImplementedError("gradients are not available for precision='bf16io'")
+    return sosfilt_rows_forward(sos_c, xr, zr, channels=B, chans_per_filter=B,
+                                want_zf=want_zf, zi_broadcast=zi_b, io_dtype=io)
