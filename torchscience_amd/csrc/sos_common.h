@@ -10,8 +10,8 @@
 namespace tsiir {
 
 constexpr int kWave = 64;        // CDNA wavefront
-constexpr int kNPow = 13;        // A^(W*2^i), i = 0..12
-constexpr int kTableStride = 64; // doubles per (filter, section) table entry
+constexpr int kNPow = 15;        // A^(W*2^i), i = 0..14
+constexpr int kTableStride = 72; // doubles per (filter, section) table entry
 constexpr int kStatusWords = 8;  // 4 words aggregate + 4 words inclusive
 constexpr uint32_t kSpinLimit = 1u << 18;
 
@@ -142,15 +142,16 @@ __device__ __forceinline__ void lane_incoming(const double* tb, int lane, double
 // the same value in every lane.  Lane l inspects window k-1-l; the nearest
 // window with an inclusive state (or the virtual window -1 holding zi) ends
 // the search, windows in between contribute their aggregates:
-//   in_k = sum_{l < f} Mw^l agg_(k-1-l) + Mw^f incl_(k-1-f),  Mw = A^(64W),
-// combined by a shuffle-down reduction with Mw^(2^i) = table power 6+i.  A
-// wait longer than kSpinLimit polls sets hdr->error and gives up, so every
-// wave terminates.
+//   in_k = sum_{l < f} Mw^l agg_(k-1-l) + Mw^f incl_(k-1-f),
+// where Mw = A^(window length) is table power `pbase`, combined by a
+// shuffle-down reduction with Mw^(2^i) = table power pbase+i.  A wait longer
+// than kSpinLimit polls sets hdr->error and gives up, so every wave
+// terminates.
 __device__ __forceinline__ void lookback(const uint64_t* status, int64_t B, int S, WsHeader* hdr,
                                          int64_t k, int64_t c, int s, uint32_t flag,
-                                         const double* tb, double zi0, double zi1, double& sw0,
-                                         double& sw1) {
-  const int lane = threadIdx.x;
+                                         const double* tb, int pbase, double zi0, double zi1,
+                                         double& sw0, double& sw1) {
+  const int lane = threadIdx.x & (kWave - 1);
   double acc0 = 0.0, acc1 = 0.0;
   Mat2 P{1.0, 0.0, 0.0, 1.0};
   int64_t base = k - 1;
@@ -200,7 +201,7 @@ __device__ __forceinline__ void lookback(const uint64_t* status, int64_t B, int 
       const int d = 1 << i;
       const double n0 = __shfl_down(v0, d);
       const double n1 = __shfl_down(v1, d);
-      const Mat2 Q = load_mat(table_pow(tb, 6 + i));
+      const Mat2 Q = load_mat(table_pow(tb, pbase + i));
       if (lane + d < kWave) {
         v0 += Q.a * n0 + Q.b * n1;
         v1 += Q.c * n0 + Q.d * n1;
@@ -211,7 +212,7 @@ __device__ __forceinline__ void lookback(const uint64_t* status, int64_t B, int 
     acc0 += P.a * r0 + P.b * r1;
     acc1 += P.c * r0 + P.d * r1;
     if (f < 64) break;
-    P = mat_mul(P, load_mat(table_pow(tb, 12)));
+    P = mat_mul(P, load_mat(table_pow(tb, pbase + 6)));
     base -= kWave;
   }
   sw0 = acc0;
@@ -308,7 +309,7 @@ template <typename TIO, typename TACC, int W>
 __device__ __forceinline__ void load_window_lds(const TIO* win, TIO* lds, TACC (&v)[W]) {
   using L = LdsStage<TIO, W>;
   using vec_t = typename IOTraits<TIO>::vec_t;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
   const vec_t* g = reinterpret_cast<const vec_t*>(win);
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
@@ -342,7 +343,7 @@ template <typename TIO, typename TACC, int W>
 __device__ __forceinline__ void store_window_lds(TIO* win, TIO* lds, const TACC (&v)[W]) {
   using L = LdsStage<TIO, W>;
   using vec_t = typename IOTraits<TIO>::vec_t;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
   vec_t* g = reinterpret_cast<vec_t*>(win);
 #pragma unroll
   for (int r = 0; r < 2; ++r) {

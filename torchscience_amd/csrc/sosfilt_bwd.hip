@@ -144,18 +144,18 @@ __global__ __launch_bounds__(64) void sos_adjoint_kernel(AdjArgs a, int64_t k_ta
 
   double agg0, agg1, x0, x1;
   wave_scan(tb, lane, static_cast<double>(p1), static_cast<double>(p2), agg0, agg1, x0, x1);
-  double sw0 = 0.0, sw1 = 0.0;
-  if (k == 0) {
-    if (a.dzf != nullptr) {
-      const T* dp = static_cast<const T*>(a.dzf) + (c * a.S + a.s) * 2;
-      sw0 = static_cast<double>(dp[0]);
-      sw1 = static_cast<double>(dp[1]);
-    }
+  // dzf is the inclusive state of the virtual window -1 (after the end)
+  double in0 = 0.0, in1 = 0.0;
+  if (a.dzf != nullptr) {
+    const T* dp = static_cast<const T*>(a.dzf) + (c * a.S + a.s) * 2;
+    in0 = static_cast<double>(dp[0]);
+    in1 = static_cast<double>(dp[1]);
   }
+  double sw0 = in0, sw1 = in1;
   if (!TAIL) {
     uint64_t* st = a.status + (k * a.B + c) * kStatusWords;
     publish_pair(st, agg0, agg1, flag, lane);
-    if (k > 0) lookback(a.status, a.B, 1, a.hdr, k, c, 0, flag, tb, 0.0, 0.0, sw0, sw1);
+    if (k > 0) lookback(a.status, a.B, 1, a.hdr, k, c, 0, flag, tb, 6, in0, in1, sw0, sw1);
     const Mat2 Mw = load_mat(table_pow(tb, 6));
     publish_pair(st + 4, Mw.a * sw0 + Mw.b * sw1 + agg0, Mw.c * sw0 + Mw.d * sw1 + agg1, flag,
                  lane);

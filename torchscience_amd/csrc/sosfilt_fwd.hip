@@ -32,6 +32,20 @@
 
 namespace tsiir {
 
+// Window shape policy.  Many channels: one wave per window and chained
+// windows (a window's predecessor was dispatched a full window-layer earlier
+// and is normally complete).  Few channels (long signals): four waves per
+// window and the decoupled look-back, which keeps the number of windows in
+// flight per channel, and so the look-back depth, small.
+struct FwdPolicy {
+  int nw;
+  bool chained;
+};
+static inline FwdPolicy fwd_policy(const tsiir_sosfilt_desc* d) {
+  if (d->channels >= 1024 && d->n_sections <= 64) return FwdPolicy{1, true};
+  return FwdPolicy{4, false};
+}
+
 struct FwdArgs {
   const void* x;
   void* y;
@@ -58,94 +72,251 @@ struct Coef {
         a2(static_cast<TACC>(tb[4])) {}
 };
 
-// Full windows.  Window I/O is LDS-staged (coalesced) when LDS is set and the
-// rows are 16-byte aligned, otherwise per-lane 16-byte accesses.
-template <typename TIO, typename TACC, int W, bool LDS>
-__global__ __launch_bounds__(64) void sos_fwd_kernel(FwdArgs a,
-                                                     const double* __restrict__ tables) {
-  __shared__ __align__(16) TIO lds[LDS ? LdsStage<TIO, W>::kElems : 8];
-  const int lane = threadIdx.x;
-  // Window-major order: workgroup i = k * B + c.  Workgroups are dispatched in
-  // index order, so a wave only waits on windows dispatched before it; the
-  // bounded spin in lookback() ends every wait regardless.
+// One workgroup = NW waves = one window of NW*64*W samples of one channel;
+// wave w owns the sub-window [w*64*W, (w+1)*64*W) and lane q of it the W
+// samples [q*W, (q+1)*W) of that sub-window.
+//
+// TAIL = false: the full windows of the signal, workgroup i = k * B + c
+// (window-major), decoupled look-back over earlier windows.  Workgroups are
+// dispatched in index order, so a look-back only waits on windows dispatched
+// before it; the bounded spin in lookback() ends every wait regardless.
+// TAIL = true: the partial last window of every channel (workgroup i =
+// channel i), launched after the full windows; its incoming state is the
+// inclusive state published for window k-1, samples at or after T are
+// masked and the lane states freeze after the last sample so zf is exact.
+//
+// CHAINED (NW == 1, full windows): instead of the decoupled look-back, the
+// wave takes the inclusive states of window k-1 (normally complete: with many
+// channels, window k-1 of a channel was dispatched one window-layer earlier),
+// reading every section's state in one round trip up front.  The window's
+// incoming state is then folded into lane 0 before the wave scan, so the scan
+// yields every lane's incoming state directly and only the inclusive state
+// is published.
+template <typename TIO, typename TACC, typename TR, int W, int NW, bool LDS, bool TAIL,
+          bool CHAINED>
+__global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_tail,
+                                                          const double* __restrict__ tables) {
+  constexpr int kLog2NW = NW == 1 ? 0 : NW == 2 ? 1 : NW == 4 ? 2 : 3;
+  static_assert((1 << kLog2NW) == NW, "NW must be 1, 2, 4 or 8");
+  static_assert(!CHAINED || (NW == 1 && !TAIL), "chained windows are single-wave");
+  __shared__ __align__(16) TIO lds[(LDS && !TAIL) ? NW * LdsStage<TIO, W>::kElems : 8];
+  __shared__ double wagg[NW][2];  // wave aggregates of the current section
+  __shared__ double wsin[2];      // incoming state of the window
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x >> 6;
   const uint32_t flag = launch_flag(a.hdr, 0);
-  const int64_t k = static_cast<int64_t>(blockIdx.x) / a.B;
-  const int64_t c = static_cast<int64_t>(blockIdx.x) % a.B;
-  const int64_t win = static_cast<int64_t>(kWave) * W;
-  const int64_t t0 = k * win + static_cast<int64_t>(lane) * W;
+  const int64_t k = TAIL ? k_tail : static_cast<int64_t>(blockIdx.x) / a.B;
+  const int64_t c = TAIL ? static_cast<int64_t>(blockIdx.x)
+                         : static_cast<int64_t>(blockIdx.x) % a.B;
+  const int64_t wwin = static_cast<int64_t>(kWave) * W;
+  const int64_t t0w = k * NW * wwin + wv * wwin;  // first sample of this wave
+  const int64_t t0 = t0w + static_cast<int64_t>(lane) * W;
+  int m = W;
+  if (TAIL) {
+    const int64_t rem = a.T - t0;
+    m = rem <= 0 ? 0 : (rem >= W ? W : static_cast<int>(rem));
+  }
 
   const TIO* xr = static_cast<const TIO*>(a.x) + (c % a.x_rows) * a.x_stride;
   TIO* yr = static_cast<TIO*>(a.y) + c * a.y_stride;
-  const bool xvec = (reinterpret_cast<uintptr_t>(xr) & 15u) == 0;
-  const bool yvec = (reinterpret_cast<uintptr_t>(yr) & 15u) == 0;
+  const bool xvec = !TAIL && (reinterpret_cast<uintptr_t>(xr) & 15u) == 0;
+  const bool yvec = !TAIL && (reinterpret_cast<uintptr_t>(yr) & 15u) == 0;
+  TIO* my_lds = lds + ((LDS && !TAIL) ? wv * LdsStage<TIO, W>::kElems : 0);
 
   TACC v[W];
-  if (LDS && xvec)
-    load_window_lds<TIO, TACC, W>(xr + k * win, lds, v);
+  if (LDS && !TAIL && xvec)
+    load_window_lds<TIO, TACC, W>(xr + t0w, my_lds, v);
   else
-    load_run<TIO, TACC, W>(xr + t0, xvec, W, v);
+    load_run<TIO, TACC, W>(xr + t0, xvec, m, v);
 
   const double* tf = tables + (c / a.cpf) * static_cast<int64_t>(a.S) * kTableStride;
   const bool has_zi = (a.flags & TSIIR_HAS_ZI) != 0;
-  // zf comes from here only when the signal ends exactly on this window
-  const bool write_zf = (a.flags & TSIIR_WRITE_ZF) != 0 && (t0 + W == a.T);
+  const bool write_zf = (a.flags & TSIIR_WRITE_ZF) != 0 &&
+                        (TAIL ? (t0 < a.T && t0 + W >= a.T) : (t0 + W == a.T));
   const int64_t zi_row = (a.flags & TSIIR_ZI_BROADCAST) ? 0 : c;
+
+  // CHAINED: lane l < S fetches window k-1's inclusive state of section l
+  double pv0 = 0.0, pv1 = 0.0;
+  bool pok = true;
+  if (CHAINED && k > 0 && lane < a.S) {
+    const uint64_t* st = a.status + (((k - 1) * a.B + c) * a.S + lane) * kStatusWords;
+    pok = read_pair(st + 4, flag, pv0, pv1);
+  }
 
   // The zero-input response that corrects section s is applied inside the
   // pass of section s+1 (one sweep over the registers per section).
-  TACC pw0 = 0, pw1 = 0, pa1 = 0, pa2 = 0;
+  TR pw0 = 0, pw1 = 0, pa1 = 0, pa2 = 0;
   for (int s = 0; s < a.S; ++s) {
     const double* tb = tf + static_cast<int64_t>(s) * kTableStride;
-    const Coef<TACC> q(tb);
+    const Coef<TR> q(tb);
 
     // 1. correction of section s-1, then the zero-state DF2T pass of section s
     //    (the update order of _lfilter.py:158-166, with fused multiply-adds)
-    TACC z0 = 0, z1 = 0;
+    TR z0 = 0, z1 = 0;
 #pragma unroll
     for (int j = 0; j < W; ++j) {
-      const TACC cj = pw0;
-      pw0 = pw1 - pa1 * cj;
-      pw1 = -pa2 * cj;
-      const TACC u = v[j] + cj;
-      const TACC yv = q.b0 * u + z0;
-      z0 = q.b1 * u + (z1 - q.a1 * yv);
-      z1 = q.b2 * u + (-q.a2 * yv);
-      v[j] = yv;
+      if (!TAIL || j < m) {
+        const TR cj = pw0;
+        pw0 = pw1 - pa1 * cj;
+        pw1 = -pa2 * cj;
+        const TR u = static_cast<TR>(v[j]) + cj;
+        const TR yv = q.b0 * u + z0;
+        z0 = q.b1 * u + (z1 - q.a1 * yv);
+        z1 = q.b2 * u + (-q.a2 * yv);
+        v[j] = static_cast<TACC>(yv);
+      }
     }
 
-    // 2. wave scan of the lanes' end states
+    double s0, s1;  // the lane's true incoming state
+    if (CHAINED) {
+      // window incoming state: zi (window 0) or window k-1's inclusive state
+      double sw0 = 0.0, sw1 = 0.0;
+      if (k == 0) {
+        if (has_zi) {
+          const TACC* zp = static_cast<const TACC*>(a.zi) + (zi_row * a.S + s) * 2;
+          sw0 = static_cast<double>(zp[0]);
+          sw1 = static_cast<double>(zp[1]);
+        }
+      } else {
+        sw0 = __shfl(pv0, s);
+        sw1 = __shfl(pv1, s);
+        if (!__shfl(static_cast<int>(pok), s)) {
+          const uint64_t* st = a.status + (((k - 1) * a.B + c) * a.S + s) * kStatusWords;
+          uint32_t spins = 0;
+          while (!read_pair(st + 4, flag, sw0, sw1)) {
+            if (++spins > kSpinLimit) {
+              if (lane == 0) atomicOr(&a.hdr->error, 1u);
+              break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+          }
+        }
+      }
+      // lane 0 carries the window's incoming state: e_0 += M sw
+      double e0 = static_cast<double>(z0), e1 = static_cast<double>(z1);
+      if (lane == 0) {
+        const Mat2 M = load_mat(table_pow(tb, 0));
+        e0 += M.a * sw0 + M.b * sw1;
+        e1 += M.c * sw0 + M.d * sw1;
+      }
+      double incl0, incl1;
+      wave_scan(tb, lane, e0, e1, incl0, incl1, s0, s1);
+      if (lane == 0) {
+        s0 = sw0;
+        s1 = sw1;
+      }
+      uint64_t* st = a.status + ((k * a.B + c) * a.S + s) * kStatusWords;
+      publish_pair(st + 4, incl0, incl1, flag, lane);
+    } else {
+    // 2. wave scan of the lanes' end states, then the waves of the window
     double agg0, agg1, x0, x1;
     wave_scan(tb, lane, static_cast<double>(z0), static_cast<double>(z1), agg0, agg1, x0, x1);
+    if (NW > 1) {
+      if (lane == 0) {
+        wagg[wv][0] = agg0;
+        wagg[wv][1] = agg1;
+      }
+      __syncthreads();
+    }
+    const Mat2 Mw = load_mat(table_pow(tb, 6));  // A^(64W): one wave's sub-window
+    double e0 = 0.0, e1 = 0.0, g0 = agg0, g1 = agg1;
+    if (NW > 1) {
+      g0 = 0.0;
+      g1 = 0.0;
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        if (j == wv) {
+          e0 = g0;
+          e1 = g1;
+        }
+        const double n0 = Mw.a * g0 + Mw.b * g1 + wagg[j][0];
+        const double n1 = Mw.c * g0 + Mw.d * g1 + wagg[j][1];
+        g0 = n0;
+        g1 = n1;
+      }
+    }
 
-    // 3. publish aggregate, look back, publish inclusive
-    uint64_t* st = a.status + ((k * a.B + c) * a.S + s) * kStatusWords;
-    publish_pair(st, agg0, agg1, flag, lane);
-    double sw0 = 0.0, sw1 = 0.0;
-    if (k == 0) {
+    // 3. window incoming state: zi for window 0, else the look-back
+    if (wv == 0) {
+      double zi0 = 0.0, zi1 = 0.0;
       if (has_zi) {
         const TACC* zp = static_cast<const TACC*>(a.zi) + (zi_row * a.S + s) * 2;
-        sw0 = static_cast<double>(zp[0]);
-        sw1 = static_cast<double>(zp[1]);
+        zi0 = static_cast<double>(zp[0]);
+        zi1 = static_cast<double>(zp[1]);
       }
-    } else {
-      lookback(a.status, a.B, a.S, a.hdr, k, c, s, flag, tb, 0.0, 0.0, sw0, sw1);
+      double sw0 = zi0, sw1 = zi1;
+      if (!TAIL) {
+        uint64_t* st = a.status + ((k * a.B + c) * a.S + s) * kStatusWords;
+        publish_pair(st, g0, g1, flag, lane);
+        if (k > 0)
+          lookback(a.status, a.B, a.S, a.hdr, k, c, s, flag, tb, 6 + kLog2NW, zi0, zi1, sw0,
+                   sw1);
+        const Mat2 Mwin = load_mat(table_pow(tb, 6 + kLog2NW));
+        publish_pair(st + 4, Mwin.a * sw0 + Mwin.b * sw1 + g0, Mwin.c * sw0 + Mwin.d * sw1 + g1,
+                     flag, lane);
+      } else if (k > 0) {
+        const uint64_t* st = a.status + (((k - 1) * a.B + c) * a.S + s) * kStatusWords;
+        if (!read_pair(st + 4, flag, sw0, sw1) && lane == 0) atomicOr(&a.hdr->error, 2u);
+      }
+      if (NW > 1 && lane == 0) {
+        wsin[0] = sw0;
+        wsin[1] = sw1;
+      }
+      if (NW == 1) {
+        e0 = sw0;
+        e1 = sw1;
+      }
     }
-    {
-      const Mat2 Mw = load_mat(table_pow(tb, 6));
-      publish_pair(st + 4, Mw.a * sw0 + Mw.b * sw1 + agg0, Mw.c * sw0 + Mw.d * sw1 + agg1,
-                   flag, lane);
+    double sv0, sv1;
+    if (NW > 1) {
+      __syncthreads();
+      // wave incoming state: Mw^wv s_window + exclusive offset of the wave
+      double t0v = wsin[0], t1v = wsin[1];
+#pragma unroll
+      for (int i = 0; i < kLog2NW; ++i) {
+        const Mat2 P = load_mat(table_pow(tb, 6 + i));
+        if ((wv >> i) & 1) {
+          const double u0 = P.a * t0v + P.b * t1v;
+          const double u1 = P.c * t0v + P.d * t1v;
+          t0v = u0;
+          t1v = u1;
+        }
+      }
+      sv0 = t0v + e0;
+      sv1 = t1v + e1;
+    } else {
+      sv0 = e0;
+      sv1 = e1;
+    }
+    lane_incoming(tb, lane, sv0, sv1, x0, x1, s0, s1);
     }
 
-    // 4. the lane's true incoming state; its zero-input response is pending
-    double s0, s1;
-    lane_incoming(tb, lane, sw0, sw1, x0, x1, s0, s1);
-    pw0 = static_cast<TACC>(s0);
-    pw1 = static_cast<TACC>(s1);
+    // 4. the zero-input response from the lane's incoming state is pending
+    pw0 = static_cast<TR>(s0);
+    pw1 = static_cast<TR>(s1);
     pa1 = q.a1;
     pa2 = q.a2;
-    if (write_zf) {
-      // state after the window's last sample: zero-state part + A^W s_q,
-      // A^W s_q = M s_q with M = table power 0
+    if (TAIL) {
+      // apply now, freezing the homogeneous state after the last sample
+      TR w0 = pw0, w1 = pw1;
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        if (j < m) {
+          const TR cj = w0;
+          v[j] = static_cast<TACC>(static_cast<TR>(v[j]) + cj);
+          w0 = w1 - q.a1 * cj;
+          w1 = -q.a2 * cj;
+        }
+      }
+      pw0 = 0;
+      pw1 = 0;
+      if (write_zf) {
+        TACC* zp = static_cast<TACC*>(a.zf) + (c * a.S + s) * 2;
+        zp[0] = static_cast<TACC>(w0 + z0);
+        zp[1] = static_cast<TACC>(w1 + z1);
+      }
+    } else if (write_zf) {
+      // state after the window's last sample: zero-state part + A^W s_q
       const Mat2 M = load_mat(table_pow(tb, 0));
       TACC* zp = static_cast<TACC*>(a.zf) + (c * a.S + s) * 2;
       zp[0] = static_cast<TACC>(static_cast<double>(z0) + M.a * s0 + M.b * s1);
@@ -154,113 +325,40 @@ __global__ __launch_bounds__(64) void sos_fwd_kernel(FwdArgs a,
     if (a.saved != nullptr && s + 1 < a.S) {
       // make the registers exact now (the pending correction becomes zero)
       // and keep section s's output for the backward pass
+      if (!TAIL) {
 #pragma unroll
-      for (int j = 0; j < W; ++j) {
-        const TACC cj = pw0;
-        pw0 = pw1 - pa1 * cj;
-        pw1 = -pa2 * cj;
-        v[j] += cj;
+        for (int j = 0; j < W; ++j) {
+          const TR cj = pw0;
+          pw0 = pw1 - pa1 * cj;
+          pw1 = -pa2 * cj;
+          v[j] = static_cast<TACC>(static_cast<TR>(v[j]) + cj);
+        }
+        pw0 = 0;
+        pw1 = 0;
       }
-      pw0 = 0;
-      pw1 = 0;
       TIO* sr = static_cast<TIO*>(a.saved) + (static_cast<int64_t>(s) * a.B + c) * a.T;
-      const bool svec = (reinterpret_cast<uintptr_t>(sr) & 15u) == 0;
-      if (LDS && svec)
-        store_window_lds<TIO, TACC, W>(sr + k * win, lds, v);
+      const bool svec = !TAIL && (reinterpret_cast<uintptr_t>(sr) & 15u) == 0;
+      if (LDS && !TAIL && svec)
+        store_window_lds<TIO, TACC, W>(sr + t0w, my_lds, v);
       else
-        store_run<TIO, TACC, W>(sr + t0, svec, W, v);
+        store_run<TIO, TACC, W>(sr + t0, svec, m, v);
     }
+    if (NW > 1) __syncthreads();  // wagg / wsin are reused by the next section
   }
-  {
+  if (!TAIL) {
 #pragma unroll
     for (int j = 0; j < W; ++j) {
-      const TACC cj = pw0;
+      const TR cj = pw0;
       pw0 = pw1 - pa1 * cj;
       pw1 = -pa2 * cj;
-      v[j] += cj;
+      v[j] = static_cast<TACC>(static_cast<TR>(v[j]) + cj);
     }
   }
 
-  if (LDS && yvec)
-    store_window_lds<TIO, TACC, W>(yr + k * win, lds, v);
+  if (LDS && !TAIL && yvec)
+    store_window_lds<TIO, TACC, W>(yr + t0w, my_lds, v);
   else
-    store_run<TIO, TACC, W>(yr + t0, yvec, W, v);
-}
-
-// The partial last window of every channel (launched after the full-window
-// kernel, one wave per channel).  Its incoming state is the inclusive state
-// the full-window kernel published for window k-1 (or zi when k == 0); lane
-// states freeze after the last real sample so zf is exact.
-template <typename TIO, typename TACC, int W>
-__global__ __launch_bounds__(64) void sos_fwd_tail_kernel(FwdArgs a, int64_t k,
-                                                          const double* __restrict__ tables) {
-  const int lane = threadIdx.x;
-  const int64_t c = blockIdx.x;
-  const uint32_t flag = launch_flag(a.hdr, 0);
-  const int64_t win = static_cast<int64_t>(kWave) * W;
-  const int64_t t0 = k * win + static_cast<int64_t>(lane) * W;
-  const int64_t rem = a.T - t0;
-  const int m = rem <= 0 ? 0 : (rem >= W ? W : static_cast<int>(rem));
-  const TIO* xr = static_cast<const TIO*>(a.x) + (c % a.x_rows) * a.x_stride;
-  TIO* yr = static_cast<TIO*>(a.y) + c * a.y_stride;
-  TACC v[W];
-  load_run<TIO, TACC, W>(xr + t0, false, m, v);
-  const double* tf = tables + (c / a.cpf) * static_cast<int64_t>(a.S) * kTableStride;
-  const bool has_zi = (a.flags & TSIIR_HAS_ZI) != 0;
-  const bool write_zf = (a.flags & TSIIR_WRITE_ZF) != 0 && (t0 < a.T) && (t0 + W >= a.T);
-  const int64_t zi_row = (a.flags & TSIIR_ZI_BROADCAST) ? 0 : c;
-  for (int s = 0; s < a.S; ++s) {
-    const double* tb = tf + static_cast<int64_t>(s) * kTableStride;
-    const Coef<TACC> q(tb);
-    TACC z0 = 0, z1 = 0;
-#pragma unroll
-    for (int j = 0; j < W; ++j) {
-      if (j < m) {
-        const TACC u = v[j];
-        const TACC yv = q.b0 * u + z0;
-        z0 = (q.b1 * u - q.a1 * yv) + z1;
-        z1 = q.b2 * u - q.a2 * yv;
-        v[j] = yv;
-      }
-    }
-    double agg0, agg1, x0, x1;
-    wave_scan(tb, lane, static_cast<double>(z0), static_cast<double>(z1), agg0, agg1, x0, x1);
-    double sw0 = 0.0, sw1 = 0.0;
-    if (k == 0) {
-      if (has_zi) {
-        const TACC* zp = static_cast<const TACC*>(a.zi) + (zi_row * a.S + s) * 2;
-        sw0 = static_cast<double>(zp[0]);
-        sw1 = static_cast<double>(zp[1]);
-      }
-    } else {
-      const uint64_t* st = a.status + (((k - 1) * a.B + c) * a.S + s) * kStatusWords;
-      if (!read_pair(st + 4, flag, sw0, sw1)) {
-        if (lane == 0) atomicOr(&a.hdr->error, 2u);
-      }
-    }
-    double s0, s1;
-    lane_incoming(tb, lane, sw0, sw1, x0, x1, s0, s1);
-    TACC w0 = static_cast<TACC>(s0), w1 = static_cast<TACC>(s1);
-#pragma unroll
-    for (int j = 0; j < W; ++j) {
-      if (j < m) {
-        const TACC cj = w0;
-        v[j] += cj;
-        w0 = w1 - q.a1 * cj;
-        w1 = -q.a2 * cj;
-      }
-    }
-    if (write_zf) {
-      TACC* zp = static_cast<TACC*>(a.zf) + (c * a.S + s) * 2;
-      zp[0] = w0 + z0;
-      zp[1] = w1 + z1;
-    }
-    if (a.saved != nullptr && s + 1 < a.S) {
-      TIO* sr = static_cast<TIO*>(a.saved) + (static_cast<int64_t>(s) * a.B + c) * a.T;
-      store_run<TIO, TACC, W>(sr + t0, false, m, v);
-    }
-  }
-  store_run<TIO, TACC, W>(yr + t0, false, m, v);
+    store_run<TIO, TACC, W>(yr + t0, yvec, m, v);
 }
 
 // Per (filter, section): normalised coefficients and P^(W*2^i), i < kNPow,
@@ -338,14 +436,18 @@ __global__ void fill_u32_kernel(uint32_t* p, size_t n) {
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-static inline int samples_per_lane(int io_dtype) { return io_dtype == TSIIR_F64 ? 32 : 64; }
+// Samples per lane: 64 for FP32 recurrences, 32 when the recurrence (and so
+// the register footprint of its temporaries) is FP64.
+static inline int samples_per_lane(const tsiir_sosfilt_desc* d) {
+  return (d->io_dtype == TSIIR_F64 || d->mode == TSIIR_MODE_COMPENSATED) ? 32 : 64;
+}
 
 
 bool fwd_geometry(const tsiir_sosfilt_desc* d, FwdGeometry* g) {
   if (!d || d->channels < 0 || d->n_samples < 0 || d->n_sections < 1) return false;
   if (d->chans_per_filter < 1 || d->x_rows < 1) return false;
-  g->W = samples_per_lane(d->io_dtype);
-  const int64_t win = static_cast<int64_t>(kWave) * g->W;
+  g->W = samples_per_lane(d);
+  const int64_t win = static_cast<int64_t>(kWave) * g->W * fwd_policy(d).nw;
   g->n_windows = (d->n_samples + win - 1) / win;
   g->n_filters = d->channels == 0 ? 0 : (d->channels + d->chans_per_filter - 1) / d->chans_per_filter;
   g->table_bytes = align_up(static_cast<size_t>(g->n_filters) * d->n_sections * kTableStride * 8, 256);
@@ -365,7 +467,7 @@ static bool fwd_lds_io() {
   return lds;
 }
 
-template <typename TIO, typename TACC, int W>
+template <typename TIO, typename TACC, typename TR, int W>
 static tsiir_status launch_fwd(const tsiir_sosfilt_desc* d, const FwdGeometry& g, const void* x,
                                void* y, const void* sos, const void* zi, void* zf, char* ws,
                                hipStream_t stream, void* saved = nullptr) {
@@ -393,21 +495,41 @@ static tsiir_status launch_fwd(const tsiir_sosfilt_desc* d, const FwdGeometry& g
   a.S = d->n_sections;
   a.flags = d->flags;
   a.saved = saved;
-  const int64_t win = static_cast<int64_t>(kWave) * W;
+  const FwdPolicy pol = fwd_policy(d);
+  const int64_t win = static_cast<int64_t>(kWave) * W * pol.nw;
   const int64_t full = d->n_samples / win;
-  const int64_t waves = full * d->channels;
-  if (waves > 0) {
-    if (fwd_lds_io())
-      hipLaunchKernelGGL((sos_fwd_kernel<TIO, TACC, W, true>), dim3(static_cast<unsigned>(waves)),
-                         dim3(kWave), 0, stream, a, a.tables);
-    else
-      hipLaunchKernelGGL((sos_fwd_kernel<TIO, TACC, W, false>),
-                         dim3(static_cast<unsigned>(waves)), dim3(kWave), 0, stream, a, a.tables);
+  const int64_t groups = full * d->channels;
+  const unsigned grid = static_cast<unsigned>(groups);
+  const bool lds = fwd_lds_io();
+  if (pol.nw == 1) {
+    const dim3 block(kWave);
+    if (groups > 0) {
+      if (lds)
+        hipLaunchKernelGGL((sos_fwd_kernel<TIO, TACC, TR, W, 1, true, false, true>), dim3(grid),
+                           block, 0, stream, a, int64_t(0), a.tables);
+      else
+        hipLaunchKernelGGL((sos_fwd_kernel<TIO, TACC, TR, W, 1, false, false, true>),
+                           dim3(grid), block, 0, stream, a, int64_t(0), a.tables);
+    }
+    if (full * win < d->n_samples)
+      hipLaunchKernelGGL((sos_fwd_kernel<TIO, TACC, TR, W, 1, false, true, false>),
+                         dim3(static_cast<unsigned>(d->channels)), block, 0, stream, a, full,
+                         a.tables);
+  } else {
+    const dim3 block(kWave * 4);
+    if (groups > 0) {
+      if (lds)
+        hipLaunchKernelGGL((sos_fwd_kernel<TIO, TACC, TR, W, 4, true, false, false>),
+                           dim3(grid), block, 0, stream, a, int64_t(0), a.tables);
+      else
+        hipLaunchKernelGGL((sos_fwd_kernel<TIO, TACC, TR, W, 4, false, false, false>),
+                           dim3(grid), block, 0, stream, a, int64_t(0), a.tables);
+    }
+    if (full * win < d->n_samples)
+      hipLaunchKernelGGL((sos_fwd_kernel<TIO, TACC, TR, W, 4, false, true, false>),
+                         dim3(static_cast<unsigned>(d->channels)), block, 0, stream, a, full,
+                         a.tables);
   }
-  if (full * win < d->n_samples)
-    hipLaunchKernelGGL((sos_fwd_tail_kernel<TIO, TACC, W>),
-                       dim3(static_cast<unsigned>(d->channels)), dim3(kWave), 0, stream, a, full,
-                       a.tables);
   return hipGetLastError() == hipSuccess ? TSIIR_OK : TSIIR_ELAUNCH;
 }
 
@@ -419,7 +541,7 @@ tsiir_status forward_impl(const tsiir_sosfilt_desc* d, const void* x, void* y, c
   if (d->io_dtype != TSIIR_F32 && d->io_dtype != TSIIR_F64 && d->io_dtype != TSIIR_BF16)
     return TSIIR_EINVAL;
   if (d->coef_dtype != TSIIR_F32 && d->coef_dtype != TSIIR_F64) return TSIIR_EINVAL;
-  if (d->mode != TSIIR_MODE_PLAIN) return TSIIR_EUNSUPPORTED;
+  if (d->mode != TSIIR_MODE_PLAIN && d->mode != TSIIR_MODE_COMPENSATED) return TSIIR_EINVAL;
   if (d->flags & TSIIR_REVERSE) return TSIIR_EUNSUPPORTED;
   const bool has_zi = (d->flags & TSIIR_HAS_ZI) != 0;
   const bool write_zf = (d->flags & TSIIR_WRITE_ZF) != 0;
@@ -449,13 +571,16 @@ tsiir_status forward_impl(const tsiir_sosfilt_desc* d, const void* x, void* y, c
   if (!workspace || workspace_bytes < g.total) return TSIIR_EWORKSPACE;
   if (g.n_windows * d->channels > 0x7fffffffLL) return TSIIR_EUNSUPPORTED;
   char* ws = static_cast<char*>(workspace);
+  const bool comp = d->mode == TSIIR_MODE_COMPENSATED;
   switch (d->io_dtype) {
     case TSIIR_F32:
-      return launch_fwd<float, float, 64>(d, g, x, y, sos, zi, zf, ws, st, saved);
+      return comp ? launch_fwd<float, float, double, 32>(d, g, x, y, sos, zi, zf, ws, st, saved)
+                  : launch_fwd<float, float, float, 64>(d, g, x, y, sos, zi, zf, ws, st, saved);
     case TSIIR_BF16:
-      return launch_fwd<bf16_t, float, 64>(d, g, x, y, sos, zi, zf, ws, st, saved);
+      return comp ? launch_fwd<bf16_t, float, double, 32>(d, g, x, y, sos, zi, zf, ws, st, saved)
+                  : launch_fwd<bf16_t, float, float, 64>(d, g, x, y, sos, zi, zf, ws, st, saved);
     default:
-      return launch_fwd<double, double, 32>(d, g, x, y, sos, zi, zf, ws, st, saved);
+      return launch_fwd<double, double, double, 32>(d, g, x, y, sos, zi, zf, ws, st, saved);
   }
 }
 

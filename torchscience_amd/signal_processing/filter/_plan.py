@@ -36,6 +36,7 @@ class SosPlan:
     zi_broadcast: bool
     zf_shape: tuple[int, ...] | None
     zf_dtype: torch.dtype | None
+    compensated: bool = False      # FP32 storage, FP64 recurrence updates
 
 
 _REAL_FLOATS = (torch.float16, torch.bfloat16, torch.float32, torch.float64)
@@ -76,13 +77,14 @@ def plan_sosfilt(sos_shape, sos_dtype: torch.dtype, x_shape, x_dtype: torch.dtyp
     compute = _compute_dtype(out)
     io = compute
     if precision is not None:
-        if precision not in ("fp32", "fp64", "bf16io"):
-            raise ValueError(f"precision must be 'fp32', 'fp64' or 'bf16io', got {precision!r}")
+        if precision not in ("fp32", "fp32c", "fp64", "bf16io"):
+            raise ValueError(
+                f"precision must be 'fp32', 'fp32c', 'fp64' or 'bf16io', got {precision!r}")
         if complex_in:
             raise NotImplementedError("precision= is for real signals")
         if precision == "fp64":
             compute, io = torch.float64, torch.float64
-        elif precision == "fp32":
+        elif precision in ("fp32", "fp32c"):
             compute, io = torch.float32, torch.float32
         else:
             compute, io, out = torch.float32, torch.bfloat16, torch.bfloat16
@@ -99,7 +101,7 @@ def plan_sosfilt(sos_shape, sos_dtype: torch.dtype, x_shape, x_dtype: torch.dtyp
                     f"zi must have shape {batch + (S, 2)} (or ({S}, 2)), got {zi_shape}")
             zf_shape = zi_shape
     return SosPlan(S, ax, batch, T, B, out, compute, io, complex_in, zi_b, zf_shape,
-                   zi_dtype if zi_shape is not None else None)
+                   zi_dtype if zi_shape is not None else None, precision == "fp32c")
 
 
 def plan_lfilter(b_shape, a_shape):

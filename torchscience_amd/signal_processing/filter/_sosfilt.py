@@ -14,12 +14,13 @@ from typing import Optional, Union
 import torch
 from torch import Tensor
 
+from ... import _lib
 from ._ops import SosfiltRows, sosfilt_rows_forward
 from ._plan import plan_sosfilt
 
 
-def _to_rows(x: Tensor, ax: int, T: int) -> Tensor:
-    return torch.movedim(x, ax, -1).reshape(-1, T)
+def _to_rows(x: Tensor, ax: int, B: int, T: int) -> Tensor:
+    return torch.movedim(x, ax, -1).reshape(B, T)
 
 
 def sosfilt(
@@ -39,8 +40,10 @@ def sosfilt(
 
     ``precision`` (keyword-only, not in the reference) selects the kernel
     form: None follows the reference's dtype promotion; "fp32" / "fp64"
-    force the compute type; "bf16io" reads and writes bfloat16 with float32
-    arithmetic and returns bfloat16.
+    force the compute type; "fp32c" keeps float32 I/O and inter-section
+    storage but runs every recurrence update in float64 (the compensated
+    form for poles near the unit circle); "bf16io" reads and writes bfloat16
+    with float32 arithmetic and returns bfloat16.
     """
     plan = plan_sosfilt(sos.shape, sos.dtype, x.shape, x.dtype, axis,
                         None if zi is None else zi.shape,
@@ -58,7 +61,7 @@ def sosfilt(
 
     if plan.complex_input:
         # Real coefficients act on the real and imaginary parts separately.
-        xc = _to_rows(x.to(plan.out_dtype), plan.axis, T)
+        xc = _to_rows(x.to(plan.out_dtype), plan.axis, B, T)
         xr = torch.view_as_real(xc).permute(0, 2, 1).reshape(2 * B, T)
         zr = None
         if zi is not None:
@@ -75,13 +78,14 @@ def sosfilt(
         zf = torch.view_as_complex(zf2.reshape(B, 2, S, 2).permute(0, 2, 3, 1).contiguous())
         return y, zf.reshape(plan.zf_shape).to(zi.dtype)
 
-    xr = _to_rows(x, plan.axis, T).to(plan.io_dtype).contiguous()
+    xr = _to_rows(x, plan.axis, B, T).to(plan.io_dtype).contiguous()
     zr = None
     if zi is not None:
         zr = zi.to(device=dev, dtype=comp)
         zr = zr.reshape(1, S, 2) if plan.zi_broadcast else zr.reshape(B, S, 2)
         zr = zr.contiguous()
-    y, zf = _run(sos_c, xr, zr, B, comp, plan.io_dtype, zi is not None, plan.zi_broadcast)
+    y, zf = _run(sos_c, xr, zr, B, comp, plan.io_dtype, zi is not None, plan.zi_broadcast,
+                 plan.compensated)
     y = torch.movedim(y.reshape(plan.batch_shape + (T,)), -1, plan.axis)
     if y.dtype != plan.out_dtype:
         y = y.to(plan.out_dtype)
@@ -90,13 +94,15 @@ def sosfilt(
     return y, zf.reshape(plan.zf_shape).to(zi.dtype)
 
 
-def _run(sos_c, xr, zr, B, comp, io, want_zf, zi_b):
+def _run(sos_c, xr, zr, B, comp, io, want_zf, zi_b, compensated=False):
     needs_grad = torch.is_grad_enabled() and (
         sos_c.requires_grad or xr.requires_grad or (zr is not None and zr.requires_grad))
-    if needs_grad and io == comp:
+    if needs_grad and io == comp and not compensated:
         y, zf = SosfiltRows.apply(xr, sos_c, zr, B, B, want_zf, zi_b)
         return y, zf
     if needs_grad:
-        raise NotImplementedError("gradients are not available for precision='bf16io'")
+        raise NotImplementedError(
+            "gradients are available for the default, 'fp32' and 'fp64' precisions")
     return sosfilt_rows_forward(sos_c, xr, zr, channels=B, chans_per_filter=B,
-                                want_zf=want_zf, zi_broadcast=zi_b, io_dtype=io)
+                                want_zf=want_zf, zi_broadcast=zi_b, io_dtype=io,
+                                mode=_lib.MODE_COMPENSATED if compensated else _lib.MODE_PLAIN)
