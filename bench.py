@@ -33,8 +33,10 @@ if ROOT not in sys.path:
 METRIC = ("G sample·sections/s (1 GPU) vs plain; serving p50/p99 latency at "
           "≤1e-5 rel err")
 HBM_PEAK_GBS = 8000.0
-# traffic per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE (profiles/), by workload
-TRAFFIC_BYTES = {}
+# HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs scaled by the same counters over a 1 GiB copy
+# (profiles/sos_fwd_c2a_pmc.txt)
+TRAFFIC_BYTES = {"c2a": 2185012224}
 
 
 def parse():
@@ -185,6 +187,22 @@ def main():
     from torchscience_amd import _lib
     assert _lib.workspaces.header_error(dev) == 0, "look-back spin limit hit"
 
+    # achievable streaming bandwidth on this device: a 1 GiB device copy
+    del step
+    src = torch.empty(1 << 28, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    c0 = torch.cuda.Event(enable_timing=True)
+    c1 = torch.cuda.Event(enable_timing=True)
+    c0.record(stream)
+    for _ in range(5):
+        dst.copy_(src)
+    c1.record(stream)
+    torch.cuda.synchronize()
+    copy_gbs = 2 * src.numel() * 4 / (c0.elapsed_time(c1) / 5 * 1e-3) / 1e9
+    del src, dst
+
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -205,7 +223,9 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": TRAFFIC_BYTES.get(args.workload),
                      "event_ms_per_step": round(ev_ms, 4),
-                     "algorithmic_bytes": algo_bytes},
+                     "algorithmic_bytes": algo_bytes,
+                     "measured_copy_GBs": round(copy_gbs, 1),
+                     "frac_of_measured_copy": round(achieved / copy_gbs, 4)},
     }
     if not args.no_cpu_baseline and args.workload != "c3":
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
