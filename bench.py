@@ -44,7 +44,11 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="c2a", choices=["c2a", "c2b", "c2c", "c5", "c5bf16", "c3"])
+    p.add_argument("--workload", default="c2a",
+                   choices=["c2a", "c2b", "c2c", "c5", "c5c", "c5bf16", "c3", "c4"])
+    p.add_argument("--requests", type=int, default=10000)
+    p.add_argument("--streams", type=int, default=2000)
+    p.add_argument("--rate", type=float, default=20000.0, help="c4 Poisson arrivals per second")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     return p.parse_args()
@@ -77,12 +81,14 @@ def make_workload(name, dev):
         sos = butterworth_sos(6, [0.05, 0.3], "bandpass")
         desc = "long-sequence sosfilt fp32: x[8,2^24] N(0,1), 6-section bandpass [0.05,0.3]"
         cpf = B
-    elif name in ("c5", "c5bf16"):
+    elif name in ("c5", "c5c", "c5bf16"):
         B, T = 2048, 1 << 20
         g.manual_seed(5)
         sos = chebyshev1_sos(16, 0.01, 1.0)
+        form = {"c5": "fp32 (FP64 carries)", "c5c": "fp32c (FP64 recurrence updates)",
+                "c5bf16": "bf16 I/O fp32 arithmetic"}[name]
         desc = ("precision stress sosfilt: x[2048,2^20], order-16 Chebyshev I 1 dB cutoff 0.01 "
-                "(8 sections), " + ("bf16 I/O fp32 arithmetic" if name == "c5bf16" else "fp32"))
+                "(8 sections), " + form)
         cpf = B
     else:  # c3
         B, T = 1024, 16384
@@ -125,6 +131,136 @@ def cpu_baseline(w, budget_s):
                       f"fp32 I/O, OpenMP {threads} threads, {dt:.1f} s"}
 
 
+def c5_accuracy(y, x, sos_np, coef_dtype):
+    """Normwise errors of the first 64 channels against the FP64 oracle, on the
+    coefficient values the kernel consumed (rounded to coef_dtype) and on the
+    FP64-designed coefficients."""
+    import numpy as np
+    from oracle import oracle
+    xs = x[:64].float().cpu().numpy().astype(np.float64)
+    ys = y.numpy().astype(np.float64)
+    res = {}
+    for key, coef in (("vs_fp64_oracle_same_coefficients", sos_np.astype(coef_dtype)),
+                      ("vs_fp64_designed_coefficients", sos_np)):
+        ref, _ = oracle.sosfilt_rows(coef.astype(np.float64), xs,
+                                     n_threads=min(16, os.cpu_count() or 1))
+        d = ys - ref
+        res[key] = {
+            "max_rel": float(np.max(np.max(np.abs(d), axis=1) / np.max(np.abs(ref), axis=1))),
+            "rms_rel": float(np.sqrt(np.sum(d * d) / np.sum(ref * ref))),
+        }
+    res["channels_checked"] = 64
+    return res
+
+
+def serving_bench(args, dev):
+    """Serving workload: Poisson arrivals over persistent streams; the engine
+    (continuous batching, device-resident state) against per-request plain
+    sosfilt calls on the same arrival schedule."""
+    import numpy as np
+    import torch
+    from oracle import oracle
+    from torchscience_amd.serving import HipExecutor, Request, ServingEngine, StreamTable, run_plain
+    from torchscience_amd.signal_processing.filter._design import butterworth_sos
+
+    rng = np.random.default_rng(4)
+    n_req, n_str = args.requests, args.streams
+    sos, chans = {}, {}
+    for sid in range(n_str):
+        sec = int(rng.integers(1, 9))
+        sos[sid] = butterworth_sos(2 * sec, float(rng.uniform(0.02, 0.45)))
+        chans[sid] = int(rng.integers(1, 65))
+    sids = rng.integers(0, n_str, n_req)
+    lens = np.exp(rng.uniform(np.log(256), np.log(65536), n_req)).astype(np.int64)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, n_req))
+    sizes = np.array([chans[int(s)] * int(T) for s, T in zip(sids, lens)], dtype=np.int64)
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    total = int(offs[-1])
+    g = torch.Generator(device=dev)
+    g.manual_seed(4)
+    xbuf = torch.randn(total, device=dev, generator=g)
+
+    def make_requests(at_once=False):
+        out = []
+        for i in range(n_req):
+            C, T = chans[int(sids[i])], int(lens[i])
+            x = xbuf[offs[i]:offs[i + 1]].view(C, T)
+            out.append(Request(i, int(sids[i]), x, 0.0 if at_once else float(arrivals[i])))
+        return out
+
+    table = StreamTable(dev)
+    for sid in range(n_str):
+        table.register(sid, sos[sid], chans[sid])
+    ybuf = torch.empty_like(xbuf)
+    reqs = make_requests()
+    for i, r in enumerate(reqs):
+        r.y = ybuf[offs[i]:offs[i + 1]].view(r.channels, r.samples)
+    def engine():
+        return ServingEngine(table, HipExecutor(dev, 4, precision="fp32c"),
+                             hbm_budget_bytes=32 << 30, max_batch_channels=8192, max_inflight=4)
+    st = engine().run(reqs)
+    torch.cuda.synchronize()
+
+    # accuracy on a sample of streams: concatenated output vs one-shot FP64
+    check = rng.choice(n_str, 16, replace=False)
+    worst = 0.0
+    for sid in check:
+        rs = [r for r in reqs if r.stream_id == int(sid)]
+        if not rs:
+            continue
+        xc = np.concatenate([r.x.cpu().numpy() for r in rs], axis=1).astype(np.float64)
+        yc = np.concatenate([r.y.cpu().numpy() for r in rs], axis=1).astype(np.float64)
+        ref, _ = oracle.sosfilt_rows(sos[int(sid)], xc,
+                                     n_threads=min(16, os.cpu_count() or 1))
+        err = np.max(np.abs(yc - ref), axis=1) / np.max(np.abs(ref), axis=1)
+        worst = max(worst, float(err.max()))
+    del ybuf
+
+    plain_reqs = make_requests()
+    pst = run_plain(plain_reqs, sos, chans, dev, precision="fp32c")
+    del plain_reqs
+    # capacity: every request queued at t=0 (states continue from the runs above)
+    cap = engine().run(make_requests(at_once=True))
+    torch.cuda.synchronize()
+    pcap = run_plain(make_requests(at_once=True), sos, chans, dev, precision="fp32c")
+    return {
+        "metric": METRIC,
+        "value": round(st.gss, 2),
+        "unit": "G sample·sections/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": round(st.wall_s * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: N(0,1) chunks from a seeded device generator, resident in HBM",
+        "config": {"workload": f"serving: {n_req} requests, Poisson {args.rate:.0f}/s over "
+                               f"{n_str} streams, channels U{{1..64}}, length log-U[256,65536], "
+                               "1-8 Butterworth lowpass sections, zi resident on device"},
+        "serving": {"p50_ms": round(st.p50 * 1e3, 3), "p99_ms": round(st.p99 * 1e3, 3),
+                    "wall_s": round(st.wall_s, 4), "batches": st.batches,
+                    "mean_requests_per_batch": round(st.mean_batch_requests, 2),
+                    "samples": st.samples, "sample_sections": st.sample_sections,
+                    "max_rel_err_16_streams_vs_fp64_oneshot": worst,
+                    "plain": {"Gss": round(pst.gss, 3), "p50_ms": round(pst.p50 * 1e3, 3),
+                              "p99_ms": round(pst.p99 * 1e3, 3), "wall_s": round(pst.wall_s, 4)},
+                    "throughput_vs_plain": round(st.gss / pst.gss, 2),
+                    "capacity_all_at_t0": {
+                        "engine": {"Gss": round(cap.gss, 3), "p50_ms": round(cap.p50 * 1e3, 3),
+                                   "p99_ms": round(cap.p99 * 1e3, 3),
+                                   "wall_s": round(cap.wall_s, 4), "batches": cap.batches},
+                        "plain": {"Gss": round(pcap.gss, 3), "p50_ms": round(pcap.p50 * 1e3, 3),
+                                  "p99_ms": round(pcap.p99 * 1e3, 3),
+                                  "wall_s": round(pcap.wall_s, 4)},
+                        "engine_vs_plain": round(cap.gss / pcap.gss, 2)},
+                    "precision": "fp32c (FP32 data, FP64 recurrence updates) in both paths"},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -136,12 +272,17 @@ def main():
     from torchscience_amd.signal_processing.filter import sosfilt, batched_filter_apply
 
     dev = torch.device("cuda:0")
+    if args.workload == "c4":
+        print(json.dumps(serving_bench(args, dev)), flush=True)
+        return
     w = make_workload(args.workload, dev)
     B, T, sos_np = w["B"], w["T"], w["sos"]
     S = sos_np.shape[-2]
     x = w["x"]
     per_channel = sos_np.ndim == 3
-    sos = torch.tensor(sos_np, dtype=torch.float32, device=dev)
+    # the compensated form takes FP64 coefficients; the others FP32
+    sos = torch.tensor(sos_np, dtype=torch.float64 if args.workload == "c5c" else torch.float32,
+                       device=dev)
     io_bytes = 2 if x.dtype == torch.bfloat16 else 4
 
     if args.workload == "c3":
@@ -157,16 +298,18 @@ def main():
     elif per_channel:
         def step():
             return batched_filter_apply(sos, x, broadcast=False)
-        algo_bytes = B * T * io_bytes * 2 + sos.numel() * 4
+        algo_bytes = B * T * io_bytes * 2 + sos.numel() * sos.element_size()
     else:
-        prec = "bf16io" if x.dtype == torch.bfloat16 else None
+        prec = "bf16io" if x.dtype == torch.bfloat16 else ("fp32c" if args.workload == "c5c"
+                                                            else None)
 
         def step():
             return sosfilt(sos, x, precision=prec)
-        algo_bytes = B * T * io_bytes * 2 + sos.numel() * 4
+        algo_bytes = B * T * io_bytes * 2 + sos.numel() * sos.element_size()
 
+    step_out = None
     for _ in range(args.warmup):
-        step()
+        step_out = step()
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream(dev)
     ev0 = torch.cuda.Event(enable_timing=True)
@@ -175,7 +318,7 @@ def main():
     t0 = time.perf_counter()
     ev0.record(stream)
     for _ in range(args.steps):
-        step()
+        step_out = step()
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -188,6 +331,7 @@ def main():
     assert _lib.workspaces.header_error(dev) == 0, "look-back spin limit hit"
 
     # achievable streaming bandwidth on this device: a 1 GiB device copy
+    step_out = step_out[:64].float().cpu() if args.workload.startswith("c5") else None
     del step
     src = torch.empty(1 << 28, device=dev)
     dst = torch.empty_like(src)
@@ -227,6 +371,9 @@ def main():
                      "measured_copy_GBs": round(copy_gbs, 1),
                      "frac_of_measured_copy": round(achieved / copy_gbs, 4)},
     }
+    if args.workload.startswith("c5"):
+        out["accuracy"] = c5_accuracy(step_out, x, sos_np,
+                                      np.float64 if sos.dtype == torch.float64 else np.float32)
     if not args.no_cpu_baseline and args.workload != "c3":
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
     else:

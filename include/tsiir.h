@@ -60,8 +60,8 @@ typedef enum tsiir_dtype {
 
 /* Arithmetic form of the recurrence (the compute type follows io_dtype). */
 typedef enum tsiir_mode {
-  TSIIR_MODE_PLAIN = 0,        /* DF2T in the compute type                 */
-  TSIIR_MODE_COMPENSATED = 1   /* FP32 DF2T with error-free state updates  */
+  TSIIR_MODE_PLAIN = 0,        /* DF2T in the compute type                   */
+  TSIIR_MODE_COMPENSATED = 1   /* FP32/BF16 storage, FP64 recurrence updates */
 } tsiir_mode;
 
 /* flags */
@@ -129,8 +129,9 @@ tsiir_status tsiir_lfilter_forward(const tsiir_sosfilt_desc* desc, const void* c
  * (channels, n_sections, 2).  Replaces the autograd tape the reference
  * records through _lfilter.py:153-166 (gradients reach x, b0..b2, a1, a2 and
  * zi; the a0 column receives 0 when a0 == 1, as in the reference).
- * dsos is accumulated per channel in `dsos_partial` workspace and reduced;
- * dzi / dzf / dsos may be NULL when not wanted. */
+ * Requires x_rows == channels.  Coefficient sums are accumulated per window
+ * and channel in the workspace and reduced per (filter, section).  dzf,
+ * dsos and dzi may be NULL when not given / not wanted. */
 size_t tsiir_sosfilt_backward_workspace_bytes(const tsiir_sosfilt_desc* desc);
 tsiir_status tsiir_sosfilt_backward(const tsiir_sosfilt_desc* desc,
                                     const void* x, const void* sos,
@@ -138,6 +139,42 @@ tsiir_status tsiir_sosfilt_backward(const tsiir_sosfilt_desc* desc,
                                     const void* dzf, void* dx, void* dsos,
                                     void* dzi, void* workspace,
                                     size_t workspace_bytes, void* stream);
+
+/* ---- serving: ragged batches of streaming chunks --------------------------
+ * The reference has no serving layer; these entry points serve the streaming
+ * use of sosfilt with zi/zf (_sosfilt.py:98-133): a chunk of a persistent
+ * stream is filtered from the stream's carried state, and zf becomes the
+ * state for the next chunk.  One launch filters a whole batch of chunks of
+ * different streams, channel counts, lengths and section counts. */
+
+/* One channel of one chunk.  x / y: n_samples FP32 samples (device).  table:
+ * the stream filter's coefficient table (tsiir_serve_prepare_filter).
+ * state: (n_sections, 2) FP64 state of this channel, read as zi and
+ * overwritten with zf.  n_sections in 1..32. */
+typedef struct tsiir_serve_item {
+  const void* x;
+  void* y;
+  const double* table;
+  double* state;
+  int64_t n_samples;
+  int32_t n_sections;
+  int32_t reserved;
+} tsiir_serve_item;
+
+/* Bytes of one filter's coefficient table (0 if n_sections is out of range). */
+size_t tsiir_serve_table_bytes(int32_t n_sections);
+
+/* Build a filter's table on the device from its (n_sections, 6) sos
+ * (coef_dtype F32 or F64); rows are normalised by a0 when a0 != 1. */
+tsiir_status tsiir_serve_prepare_filter(const void* sos, int32_t coef_dtype, int32_t n_sections,
+                                        double* table, void* stream);
+
+/* Filter every item of a batch (items: device array of n_items).  Items must
+ * not share a state slot (one chunk per stream and channel per batch); an
+ * item's result does not depend on the other items.  mode: TSIIR_MODE_PLAIN
+ * (FP32 recurrence) or TSIIR_MODE_COMPENSATED (FP32 storage, FP64 updates). */
+tsiir_status tsiir_serve_batch(const tsiir_serve_item* items, int32_t n_items, int32_t mode,
+                               void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,95 @@
+"""Serving engine on the GPU: streamed results against one-shot FP64
+filtering of each concatenated stream (normwise per channel: <= 1e-5 for the
+compensated form, FP32 data with FP64 updates, against the FP64 coefficients;
+<= 1e-4 for the FP32 recurrence against the float32-rounded coefficients it
+uses), and each request's result identical (bitwise) alone and in a mixed
+batch."""
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch.device("cuda:0")
+
+
+def _streams(rng, n_streams, max_ch=16):
+    from torchscience_amd.signal_processing.filter._design import butterworth_sos
+    sos, chans = {}, {}
+    for s in range(n_streams):
+        sec = int(rng.integers(1, 9))
+        sos[s] = butterworth_sos(2 * sec, float(rng.uniform(0.02, 0.45)))
+        chans[s] = int(rng.integers(1, max_ch + 1))
+    return sos, chans
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32c", 1e-5), ("fp32", 1e-4)])
+def test_serving_matches_oneshot(oracle_mod, precision, tol):
+    from torchscience_amd.serving import HipExecutor, Request, ServingEngine, StreamTable
+    dev = _dev()
+    rng = np.random.default_rng(11)
+    sos, chans = _streams(rng, 24)
+    table = StreamTable(dev)
+    for s in sos:
+        table.register(s, sos[s], chans[s])
+    reqs = []
+    t = np.cumsum(rng.exponential(1 / 5000.0, 160))
+    for i in range(160):
+        sid = int(rng.integers(0, 24))
+        T = int(np.exp(rng.uniform(np.log(40), np.log(9000))))
+        x = torch.from_numpy(rng.standard_normal((chans[sid], T)).astype(np.float32)).to(dev)
+        reqs.append(Request(i, sid, x, float(t[i])))
+    eng = ServingEngine(table, HipExecutor(dev, 3, precision=precision),
+                        hbm_budget_bytes=64 << 20, max_batch_channels=256, max_inflight=3,
+                        long_samples=4096)
+    stats = eng.run(reqs)
+    torch.cuda.synchronize()
+    print("batches", stats.batches, "p50 ms", stats.p50 * 1e3, "p99 ms", stats.p99 * 1e3)
+    worst = 0.0
+    for sid in sos:
+        rs = sorted([r for r in reqs if r.stream_id == sid], key=lambda r: r.arrival)
+        if not rs:
+            continue
+        xcat = np.concatenate([r.x.cpu().numpy() for r in rs], axis=1).astype(np.float64)
+        ycat = np.concatenate([r.y.cpu().numpy() for r in rs], axis=1).astype(np.float64)
+        coef = sos[sid] if precision == "fp32c" else sos[sid].astype(np.float32)
+        ref, zf = oracle_mod.sosfilt_rows(coef.astype(np.float64), xcat)
+        err = np.max(np.abs(ycat - ref), axis=1) / np.max(np.abs(ref), axis=1)
+        worst = max(worst, float(err.max()))
+        np.testing.assert_allclose(table.state(sid).cpu().numpy(), zf, rtol=1e-4, atol=1e-5)
+    print("worst normwise rel err", worst)
+    assert worst < tol
+
+
+@pytest.mark.parametrize("precision", ["fp32c", "fp32"])
+def test_alone_equals_in_mix(precision):
+    from torchscience_amd.serving import HipExecutor, Request, ServingEngine, StreamTable
+    dev = _dev()
+    rng = np.random.default_rng(12)
+    sos, chans = _streams(rng, 10)
+    table = StreamTable(dev)
+    for s in sos:
+        table.register(s, sos[s], chans[s], zi=rng.standard_normal((chans[s], sos[s].shape[0], 2)))
+    table.ready()
+    snap = {s: table.snapshot(s) for s in sos}
+    xs = {s: torch.from_numpy(rng.standard_normal((chans[s], int(rng.integers(100, 20000))))
+                              .astype(np.float32)).to(dev) for s in sos}
+    # all streams together in one batch
+    reqs = [Request(i, s, xs[s], 0.0) for i, s in enumerate(sos)]
+    ServingEngine(table, HipExecutor(dev, 1, precision=precision)).run(reqs)
+    torch.cuda.synchronize()
+    mixed = {r.stream_id: r.y.clone() for r in reqs}
+    mixed_state = {s: table.snapshot(s) for s in sos}
+    # each stream alone, from the same starting state
+    for s in sos:
+        table.restore(s, snap[s])
+        r = Request(0, s, xs[s], 0.0)
+        ServingEngine(table, HipExecutor(dev, 1, precision=precision)).run([r])
+        torch.cuda.synchronize()
+        assert torch.equal(r.y, mixed[s])
+        assert np.array_equal(table.snapshot(s), mixed_state[s])

@@ -105,6 +105,13 @@ def test_plan_promotion_and_zi_broadcast():
     assert p.out_dtype == f32
     p = plan_sosfilt((4, 6), f32, (3, 100), f32, precision="bf16io")
     assert p.io_dtype == torch.bfloat16 and p.out_dtype == torch.bfloat16
+    f64 = torch.float64
+    for prec in ("fp32", "fp32c"):
+        p = plan_sosfilt((4, 6), f64, (3, 100), f32, precision=prec)
+        assert p.compute_dtype == p.io_dtype == p.out_dtype == f32
+        assert p.compensated == (prec == "fp32c")
+    p = plan_sosfilt((4, 6), f32, (3, 100), f32, precision="fp64")
+    assert p.compute_dtype == p.io_dtype == p.out_dtype == f64
     with pytest.raises(RuntimeError, match="zi must have shape"):
         plan_sosfilt((4, 6), f32, (3, 100), f32, zi_shape=(2, 4, 2), zi_dtype=f32)
     with pytest.raises(NotImplementedError):

@@ -1,0 +1,130 @@
+"""Serving scheduler on the CPU with a fake executor built on the FP64 oracle.
+
+Checks continuous batching invariants: every request completes; a stream's
+chunks run one at a time in arrival order; the HBM budget, the per-batch
+channel limit and the in-flight limit hold; requests arriving while batches
+run are admitted as capacity frees; and the streamed outputs equal one-shot
+filtering of each concatenated stream (the state threads through chunks).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from torchscience_amd.serving.engine import Request, ServingEngine
+
+
+class FakeTable:
+    def __init__(self, sos_by_stream, channels):
+        self.sos = sos_by_stream
+        self.ch = channels
+        self.state = {s: np.zeros((c, sos_by_stream[s].shape[0], 2)) for s, c in channels.items()}
+
+    def sections(self, sid):
+        return self.sos[sid].shape[0]
+
+
+class FakeExecutor:
+    """Filters a batch with the oracle at launch; reports it done after a
+    simulated device time proportional to its samples."""
+
+    def __init__(self, oracle, clock, engine_ref, per_sample=2e-9):
+        self.oracle = oracle
+        self.clock = clock
+        self.per_sample = per_sample
+        self.engine_ref = engine_ref
+        self.inflight = []
+        self.max_inflight_seen = 0
+        self.max_used_bytes = 0
+        self.running_streams = set()
+
+    def launch(self, batch, table, lane=0):
+        sids = [r.stream_id for r in batch]
+        assert len(set(sids)) == len(sids), "two chunks of one stream in a batch"
+        assert not (set(sids) & self.running_streams), "stream already running"
+        work = 0
+        for r in batch:
+            x = r.x.numpy().astype(np.float64)
+            y, zf = self.oracle.sosfilt_rows(table.sos[r.stream_id], x, table.state[r.stream_id])
+            table.state[r.stream_id] = zf
+            r.y = torch.from_numpy(y)
+            work += x.size
+        self.running_streams |= set(sids)
+        h = {"t_end": self.clock() + work * self.per_sample, "sids": set(sids),
+             "bytes": sum(r.nbytes for r in batch)}
+        self.inflight.append(h)
+        self.max_inflight_seen = max(self.max_inflight_seen, len(self.inflight))
+        self.max_used_bytes = max(self.max_used_bytes, sum(x["bytes"] for x in self.inflight))
+        return h
+
+    def done(self, h):
+        if self.clock() >= h["t_end"]:
+            if h in self.inflight:
+                self.inflight.remove(h)
+                self.running_streams -= h["sids"]
+            return True
+        return False
+
+
+def _workload(seed, n_streams=12, n_req=80, rate=20000.0):
+    from torchscience_amd.signal_processing.filter._design import butterworth_sos
+    rng = np.random.default_rng(seed)
+    sos, chans = {}, {}
+    for s in range(n_streams):
+        sec = int(rng.integers(1, 9))
+        sos[s] = butterworth_sos(2 * sec, float(rng.uniform(0.02, 0.45)))
+        chans[s] = int(rng.integers(1, 9))
+    t = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    reqs = []
+    for i in range(n_req):
+        sid = int(rng.integers(0, n_streams))
+        T = int(np.exp(rng.uniform(np.log(16), np.log(600))))
+        x = torch.from_numpy(rng.standard_normal((chans[sid], T)).astype(np.float32))
+        reqs.append(Request(i, sid, x, float(t[i])))
+    return sos, chans, reqs
+
+
+@pytest.mark.parametrize("budget,max_ch,inflight,long_samples",
+                         [(1 << 30, 4096, 4, 16384), (60_000, 12, 2, 16384),
+                          (1 << 30, 4096, 1, 16384), (1 << 30, 4096, 2, 200),
+                          (60_000, 12, 1, 100)])
+def test_scheduler_invariants_and_continuation(oracle_mod, budget, max_ch, inflight,
+                                               long_samples):
+    import time
+    sos, chans, reqs = _workload(7)
+    table = FakeTable(sos, chans)
+    clock = time.perf_counter
+    ex = FakeExecutor(oracle_mod, clock, None)
+    eng = ServingEngine(table, ex, hbm_budget_bytes=budget, max_batch_channels=max_ch,
+                        max_inflight=inflight, long_samples=long_samples, clock=clock)
+    stats = eng.run(reqs)
+    assert stats.n_requests == len(reqs)
+    assert all(r.t_done >= r.t_launch >= r.arrival for r in reqs)
+    # in-flight limit per lane (short chunks / chunks above long_samples)
+    assert ex.max_inflight_seen <= 2 * inflight
+    for b in eng.batches:
+        longs = {reqs[i].samples > long_samples for i in b}
+        assert len(longs) == 1, "a batch mixes the short and the long lane"
+    assert ex.max_used_bytes <= budget
+    for b in eng.batches:
+        assert sum(reqs[i].channels for i in b) <= max_ch
+    # per-stream order: launches follow arrival order, no overlap
+    by_stream = {}
+    for r in sorted(reqs, key=lambda r: r.arrival):
+        by_stream.setdefault(r.stream_id, []).append(r)
+    for sid, rs in by_stream.items():
+        for a, b in zip(rs, rs[1:]):
+            assert b.t_launch >= a.t_done
+        # streamed result equals one-shot filtering of the concatenation
+        xcat = np.concatenate([r.x.numpy() for r in rs], axis=1).astype(np.float64)
+        ycat = np.concatenate([r.y.numpy() for r in rs], axis=1)
+        ref, _ = oracle_mod.sosfilt_rows(sos[sid], xcat)
+        np.testing.assert_allclose(ycat, ref, rtol=1e-12, atol=1e-12)
+    assert np.all(np.isfinite(stats.latencies)) and stats.p99 >= stats.p50 >= 0
+
+
+def test_budget_too_small_is_rejected():
+    sos, chans, reqs = _workload(3, n_req=5)
+    eng = ServingEngine(FakeTable(sos, chans), None, hbm_budget_bytes=16)
+    with pytest.raises(ValueError, match="HBM budget"):
+        eng.run(reqs)

@@ -31,6 +31,9 @@ EXPORTED = (
     "tsiir_lfilter_forward",
     "tsiir_sosfilt_backward_workspace_bytes",
     "tsiir_sosfilt_backward",
+    "tsiir_serve_table_bytes",
+    "tsiir_serve_prepare_filter",
+    "tsiir_serve_batch",
 )
 
 
@@ -48,6 +51,18 @@ class SosfiltDesc(ctypes.Structure):
         ("x_row_stride", ctypes.c_int64),
         ("y_row_stride", ctypes.c_int64),
         ("chans_per_filter", ctypes.c_int64),
+    ]
+
+
+class ServeItem(ctypes.Structure):
+    _fields_ = [
+        ("x", ctypes.c_void_p),
+        ("y", ctypes.c_void_p),
+        ("table", ctypes.c_void_p),
+        ("state", ctypes.c_void_p),
+        ("n_samples", ctypes.c_int64),
+        ("n_sections", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -86,6 +101,12 @@ def load(path: str | None = None) -> ctypes.CDLL:
     L.tsiir_sosfilt_backward_workspace_bytes.argtypes = [dp]
     L.tsiir_sosfilt_backward.restype = ctypes.c_int
     L.tsiir_sosfilt_backward.argtypes = [dp, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, vp]
+    L.tsiir_serve_table_bytes.restype = sz
+    L.tsiir_serve_table_bytes.argtypes = [ctypes.c_int32]
+    L.tsiir_serve_prepare_filter.restype = ctypes.c_int
+    L.tsiir_serve_prepare_filter.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp, vp]
+    L.tsiir_serve_batch.restype = ctypes.c_int
+    L.tsiir_serve_batch.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp]
     if path is None:
         _lib = L
     return L

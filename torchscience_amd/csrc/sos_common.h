@@ -313,15 +313,14 @@ __device__ __forceinline__ void load_window_lds(const TIO* win, TIO* lds, TACC (
   const vec_t* g = reinterpret_cast<const vec_t*>(win);
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
-    vec_t raw[L::kNV / 2];
-#pragma unroll
-    for (int i = 0; i < L::kNV / 2; ++i) raw[i] = g[(r * L::kNV / 2 + i) * kWave + lane];
+    // global -> LDS without a private staging array (a generic load pointer
+    // could alias one and force it to scratch)
 #pragma unroll
     for (int i = 0; i < L::kNV / 2; ++i) {
       const int vi = (r * L::kNV / 2 + i) * kWave + lane;  // vector index in the window
       const int q = vi / L::kNV - 32 * r;                    // row within this round
       const int p = (vi % L::kNV) * L::kVec;                 // element column
-      *reinterpret_cast<vec_t*>(lds + q * L::kRow + p) = raw[i];
+      *reinterpret_cast<vec_t*>(lds + q * L::kRow + p) = g[vi];
     }
     __syncthreads();
     if ((lane >> 5) == r) {

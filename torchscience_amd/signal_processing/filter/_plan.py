@@ -83,9 +83,10 @@ def plan_sosfilt(sos_shape, sos_dtype: torch.dtype, x_shape, x_dtype: torch.dtyp
         if complex_in:
             raise NotImplementedError("precision= is for real signals")
         if precision == "fp64":
-            compute, io = torch.float64, torch.float64
+            compute, io, out = torch.float64, torch.float64, torch.float64
         elif precision in ("fp32", "fp32c"):
-            compute, io = torch.float32, torch.float32
+            # a forced float32 form returns float32 also for float64 sos
+            compute, io, out = torch.float32, torch.float32, torch.float32
         else:
             compute, io, out = torch.float32, torch.bfloat16, torch.bfloat16
     zi_b = False

@@ -40,9 +40,10 @@ def sosfilt(
 
     ``precision`` (keyword-only, not in the reference) selects the kernel
     form: None follows the reference's dtype promotion; "fp32" / "fp64"
-    force the compute type; "fp32c" keeps float32 I/O and inter-section
-    storage but runs every recurrence update in float64 (the compensated
-    form for poles near the unit circle); "bf16io" reads and writes bfloat16
+    force the compute and result type; "fp32c" keeps float32 I/O and inter-section
+    storage but runs every recurrence update in float64 with float64
+    coefficients (the compensated form for poles near the unit circle) and
+    returns float32; "bf16io" reads and writes bfloat16
     with float32 arithmetic and returns bfloat16.
     """
     plan = plan_sosfilt(sos.shape, sos.dtype, x.shape, x.dtype, axis,
@@ -57,7 +58,10 @@ def sosfilt(
         if zi is None:
             return y
         return y, torch.zeros(plan.zf_shape, dtype=zi.dtype, device=zi.device)
-    sos_c = sos.to(device=dev, dtype=comp).reshape(1, S, 6).contiguous()
+    # the compensated form reads its coefficients in float64 (exact for
+    # float32 sos; keeps float64-designed filters at full precision)
+    cdt = torch.float64 if plan.compensated else comp
+    sos_c = sos.to(device=dev, dtype=cdt).reshape(1, S, 6).contiguous()
 
     if plan.complex_input:
         # Real coefficients act on the real and imaginary parts separately.
