@@ -42,8 +42,8 @@ TRAFFIC_BYTES = {"c2a": 2185012224}
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=20)
-    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--warmup", type=int, default=50)
     p.add_argument("--workload", default="c2a",
                    choices=["c2a", "c2b", "c2c", "c5", "c5c", "c5bf16", "c3", "c4"])
     p.add_argument("--requests", type=int, default=10000)

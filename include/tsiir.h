@@ -69,6 +69,12 @@ typedef enum tsiir_mode {
 #define TSIIR_WRITE_ZF 0x2u      /* write the final state to zf            */
 #define TSIIR_ZI_BROADCAST 0x4u  /* one (n_sections, 2) zi for every channel */
 #define TSIIR_REVERSE 0x8u       /* run along decreasing time (adjoint)    */
+/* Forward work decomposition (default: chosen from the shape).  WINDOW:
+ * windows of a channel run in parallel, chained by a look-back over scan
+ * status words.  CHANNEL: one wave per channel walks its windows in order
+ * (for at least as many channels as the GPU holds waves).  At most one. */
+#define TSIIR_PATH_WINDOW 0x10u
+#define TSIIR_PATH_CHANNEL 0x20u
 
 typedef struct tsiir_sosfilt_desc {
   int64_t channels;          /* output channels B                          */

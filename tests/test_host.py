@@ -50,9 +50,9 @@ def test_workspace_size_query_without_gpu():
                          mode=0, flags=0, reserved=0, x_rows=4096, x_row_stride=65536,
                          y_row_stride=65536, chans_per_filter=4096)
     n = L.tsiir_sosfilt_workspace_bytes(ctypes.byref(d))
-    # header + one 576-byte table entry per section (rounded to 256 B) +
+    # header + one 896-byte table entry per section (rounded to 256 B) +
     # 64 B of status per (window, channel, section); windows of 4096 samples
-    assert n == 256 + 3584 + 16 * 4096 * 6 * 64
+    assert n == 256 + 5376 + 16 * 4096 * 6 * 64
     d.n_sections = 0
     assert L.tsiir_sosfilt_workspace_bytes(ctypes.byref(d)) == 0
     assert L.tsiir_status_string(2) == b"unsupported request"

@@ -20,6 +20,7 @@ OK, EINVAL, EUNSUPPORTED, ELAUNCH, EWORKSPACE = range(5)
 F32, F64, BF16 = 0, 1, 2
 MODE_PLAIN, MODE_COMPENSATED = 0, 1
 HAS_ZI, WRITE_ZF, ZI_BROADCAST, REVERSE = 0x1, 0x2, 0x4, 0x8
+PATH_WINDOW, PATH_CHANNEL = 0x10, 0x20
 
 # Every symbol include/tsiir.h declares.
 EXPORTED = (

@@ -10,8 +10,11 @@
 namespace tsiir {
 
 constexpr int kWave = 64;        // CDNA wavefront
-constexpr int kNPow = 15;        // A^(W*2^i), i = 0..14
-constexpr int kTableStride = 72; // doubles per (filter, section) table entry
+constexpr int kNPow = 15;         // A^(W*2^i), i = 0..14
+constexpr int kGOff = 72;         // doubles before the float zero-input response table
+constexpr int kGLen = 65;         // floats in that table: g[0..64]
+constexpr int kTableStride = 112; // doubles per (filter, section) table entry (896 B)
+static_assert(2 * kGOff + kGLen <= 2 * kTableStride, "table entry too small");
 constexpr int kStatusWords = 8;  // 4 words aggregate + 4 words inclusive
 constexpr uint32_t kSpinLimit = 1u << 18;
 
@@ -33,6 +36,30 @@ struct Mat2 {
 __device__ __forceinline__ Mat2 mat_mul(const Mat2& x, const Mat2& y) {
   return Mat2{x.a * y.a + x.b * y.c, x.a * y.b + x.b * y.d,
               x.c * y.a + x.d * y.c, x.c * y.b + x.d * y.d};
+}
+
+// Zero-input response table of a section (float), at kGOff of its entry:
+// g[0] = 0, g[1] = 1, g[j] = -a1 g[j-1] - a2 g[j-2].  The DF2T output j
+// samples after state (s0, s1) with zero input is g[j+1] s0 + g[j] s1.
+__device__ __forceinline__ const float* table_g(const double* t) {
+  return reinterpret_cast<const float*>(t + kGOff);
+}
+
+// d = a * b + c as one VOP3 FMA with a free destination.  The two-address
+// v_fmac form that instruction selection prefers needs a register copy
+// whenever the result must land in a loop-carried register, as the sample
+// registers of the section sweep do.
+__device__ __forceinline__ float fma3(float a, float b, float c) {
+  float d;
+  asm("v_fma_f32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+}
+__device__ __forceinline__ double fma3(double a, double b, double c) { return a * b + c; }
+// The same with a wave-uniform (SGPR) multiplicand.
+__device__ __forceinline__ float fma3s(float a, float b, float c) {
+  float d;
+  asm("v_fma_f32 %0, %1, %2, %3" : "=v"(d) : "s"(a), "v"(b), "v"(c));
+  return d;
 }
 
 __device__ __forceinline__ Mat2 load_mat(const double* t) {
@@ -89,6 +116,24 @@ __device__ __forceinline__ bool read_pair(const uint64_t* words, uint32_t flag,
   return ok;
 }
 
+// A double of lane l (wave-uniform l) in every lane.
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), l);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<long long>(static_cast<unsigned>(lo)));
+}
+
+// A double of lane (addr / 4) for every lane (ds_bpermute on both halves).
+__device__ __forceinline__ double bpermute_d(int addr, double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_bpermute(addr, static_cast<int>(b));
+  const int hi = __builtin_amdgcn_ds_bpermute(addr, static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<long long>(static_cast<unsigned>(lo)));
+}
+
 // ---- wave-level scan of 2x2 affine maps -----------------------------------
 //
 // Lane q holds the zero-state end state e_q of its W samples.  With
@@ -100,18 +145,20 @@ __device__ __forceinline__ void wave_scan(const double* tb, int lane, double e0,
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
     const int d = 1 << i;
-    const double n0 = __shfl_up(e0, d);
-    const double n1 = __shfl_up(e1, d);
+    const int src = ((lane - d) & (kWave - 1)) << 2;  // lanes < d: masked below
+    const double n0 = bpermute_d(src, e0);
+    const double n1 = bpermute_d(src, e1);
     const Mat2 P = load_mat(table_pow(tb, i));
     if (lane >= d) {
-      e0 += P.a * n0 + P.b * n1;
-      e1 += P.c * n0 + P.d * n1;
+      e0 = fma(P.b, n1, fma(P.a, n0, e0));
+      e1 = fma(P.d, n1, fma(P.c, n0, e1));
     }
   }
-  agg0 = __shfl(e0, kWave - 1);
-  agg1 = __shfl(e1, kWave - 1);
-  x0 = __shfl_up(e0, 1);
-  x1 = __shfl_up(e1, 1);
+  agg0 = readlane_d(e0, kWave - 1);
+  agg1 = readlane_d(e1, kWave - 1);
+  const int src1 = ((lane - 1) & (kWave - 1)) << 2;
+  x0 = bpermute_d(src1, e0);
+  x1 = bpermute_d(src1, e1);
   if (lane == 0) {
     x0 = 0.0;
     x1 = 0.0;
@@ -293,47 +340,95 @@ __device__ __forceinline__ void load_run(const TIO* p, bool vec, int m, TACC (&v
 
 // ---- LDS-staged window I/O --------------------------------------------------
 //
-// A wave's window is 64*W contiguous elements; lane q owns [q*W, (q+1)*W).
-// Global memory is read and written with fully coalesced 16-byte accesses
-// (instruction i, lane l -> vector i*64 + l) and transposed through a
-// per-wave LDS buffer of 32 padded rows, in two rounds of 32 lanes.
+// A wave's window is 64*W contiguous elements; lane q owns [q*W, (q+1)*W),
+// i.e. kNV 16-byte vectors.  The window moves in two rounds, round r carrying
+// the half-rows [r*W/2, (r+1)*W/2) of all 64 lanes (kH = kNV/2 vectors per
+// lane).  Global side: instruction i, lane l handles row q = i*(64/kH) + l/kH,
+// vector t = l % kH of that half-row, so every instruction covers whole runs
+// of kH consecutive vectors (fully used cache lines).  LDS side: row q at
+// q*(kH+1) vectors (one vector of padding: the 64 lanes reading column t hit
+// distinct banks).  Every lane takes part in every LDS access of both rounds,
+// so nothing runs under a partial execution mask.
 template <typename TIO, int W>
 struct LdsStage {
   static constexpr int kVec = IOTraits<TIO>::kVec;      // elements per 16 bytes
   static constexpr int kNV = W / kVec;                  // 16-byte vectors per lane
-  static constexpr int kRow = W + kVec;                 // padded row (elements)
-  static constexpr int kElems = 32 * kRow;              // buffer size (elements)
+  static constexpr int kH = kNV / 2;                    // vectors per lane per round
+  static constexpr int kRowV = kH + 1;                  // padded row (vectors)
+  static constexpr int kElems = kWave * kRowV * kVec;   // buffer size (elements)
+  static_assert(kH >= 1 && kWave % kH == 0, "window shape");
+  // vector index within the window of (round r, instruction i, lane l)
+  __device__ static __forceinline__ int gvec(int r, int i, int l) {
+    return (i * (kWave / kH) + l / kH) * kNV + r * kH + l % kH;
+  }
+  // vector offset in LDS of (instruction i, lane l)
+  __device__ static __forceinline__ int lvec(int i, int l) {
+    return (i * (kWave / kH) + l / kH) * kRowV + l % kH;
+  }
 };
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// LDS rows -> the lane's samples [r*W/2, (r+1)*W/2)
+template <typename TIO, typename TACC, int W>
+__device__ __forceinline__ void lds_rows_to_regs(const TIO* lds, int r, TACC (&v)[W]) {
+  using L = LdsStage<TIO, W>;
+  using vec_t = typename IOTraits<TIO>::vec_t;
+  const int lane = threadIdx.x & (kWave - 1);
+  const vec_t* row = reinterpret_cast<const vec_t*>(lds) + lane * L::kRowV;
+#pragma unroll
+  for (int t = 0; t < L::kH; ++t) {
+    const vec_t rv = row[t];
+    const TIO* e = reinterpret_cast<const TIO*>(&rv);
+#pragma unroll
+    for (int j = 0; j < L::kVec; ++j)
+      v[(r * L::kH + t) * L::kVec + j] = static_cast<TACC>(IOTraits<TIO>::to_f(e[j]));
+  }
+}
 
 template <typename TIO, typename TACC, int W>
 __device__ __forceinline__ void load_window_lds(const TIO* win, TIO* lds, TACC (&v)[W]) {
   using L = LdsStage<TIO, W>;
-  using vec_t = typename IOTraits<TIO>::vec_t;
   const int lane = threadIdx.x & (kWave - 1);
-  const vec_t* g = reinterpret_cast<const vec_t*>(win);
+  const u32x4* __restrict__ g = reinterpret_cast<const u32x4*>(win);
+  u32x4* lv = reinterpret_cast<u32x4*>(lds);
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
-    // global -> LDS without a private staging array (a generic load pointer
-    // could alias one and force it to scratch)
 #pragma unroll
-    for (int i = 0; i < L::kNV / 2; ++i) {
-      const int vi = (r * L::kNV / 2 + i) * kWave + lane;  // vector index in the window
-      const int q = vi / L::kNV - 32 * r;                    // row within this round
-      const int p = (vi % L::kNV) * L::kVec;                 // element column
-      *reinterpret_cast<vec_t*>(lds + q * L::kRow + p) = g[vi];
-    }
+    for (int i = 0; i < L::kH; ++i) lv[L::lvec(i, lane)] = g[L::gvec(r, i, lane)];
     __syncthreads();
-    if ((lane >> 5) == r) {
-      const TIO* row = lds + (lane & 31) * L::kRow;
+    lds_rows_to_regs<TIO, TACC, W>(lds, r, v);
+    __syncthreads();
+  }
+}
+
+// Prefetch form of load_window_lds: issue the window's global loads into
+// registers (native 4 x u32 vectors, which stay in VGPRs across the loop
+// that carries them), and transpose them through LDS when the window is due.
+template <typename TIO, int W>
+__device__ __forceinline__ void prefetch_window(const TIO* win,
+                                                u32x4 (&pf)[LdsStage<TIO, W>::kNV]) {
+  using L = LdsStage<TIO, W>;
+  const int lane = threadIdx.x & (kWave - 1);
+  const u32x4* __restrict__ g = reinterpret_cast<const u32x4*>(win);
 #pragma unroll
-      for (int i = 0; i < L::kNV; ++i) {
-        const vec_t rv = *reinterpret_cast<const vec_t*>(row + i * L::kVec);
-        const TIO* e = reinterpret_cast<const TIO*>(&rv);
+  for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int j = 0; j < L::kVec; ++j)
-          v[i * L::kVec + j] = static_cast<TACC>(IOTraits<TIO>::to_f(e[j]));
-      }
-    }
+    for (int i = 0; i < L::kH; ++i) pf[r * L::kH + i] = g[L::gvec(r, i, lane)];
+}
+
+template <typename TIO, typename TACC, int W>
+__device__ __forceinline__ void transpose_window_lds(const u32x4 (&pf)[LdsStage<TIO, W>::kNV],
+                                                     TIO* lds, TACC (&v)[W]) {
+  using L = LdsStage<TIO, W>;
+  const int lane = threadIdx.x & (kWave - 1);
+  u32x4* lv = reinterpret_cast<u32x4*>(lds);
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+#pragma unroll
+    for (int i = 0; i < L::kH; ++i) lv[L::lvec(i, lane)] = pf[r * L::kH + i];
+    __syncthreads();
+    lds_rows_to_regs<TIO, TACC, W>(lds, r, v);
     __syncthreads();
   }
 }
@@ -343,28 +438,23 @@ __device__ __forceinline__ void store_window_lds(TIO* win, TIO* lds, const TACC 
   using L = LdsStage<TIO, W>;
   using vec_t = typename IOTraits<TIO>::vec_t;
   const int lane = threadIdx.x & (kWave - 1);
-  vec_t* g = reinterpret_cast<vec_t*>(win);
+  u32x4* __restrict__ g = reinterpret_cast<u32x4*>(win);
+  const u32x4* lv = reinterpret_cast<const u32x4*>(lds);
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
-    if ((lane >> 5) == r) {
-      TIO* row = lds + (lane & 31) * L::kRow;
+    vec_t* row = reinterpret_cast<vec_t*>(lds) + lane * L::kRowV;
 #pragma unroll
-      for (int i = 0; i < L::kNV; ++i) {
-        vec_t rv;
-        TIO* e = reinterpret_cast<TIO*>(&rv);
+    for (int t = 0; t < L::kH; ++t) {
+      vec_t rv;
+      TIO* e = reinterpret_cast<TIO*>(&rv);
 #pragma unroll
-        for (int j = 0; j < L::kVec; ++j) e[j] = IOTraits<TIO>::from_f(v[i * L::kVec + j]);
-        *reinterpret_cast<vec_t*>(row + i * L::kVec) = rv;
-      }
+      for (int j = 0; j < L::kVec; ++j)
+        e[j] = IOTraits<TIO>::from_f(v[(r * L::kH + t) * L::kVec + j]);
+      row[t] = rv;
     }
     __syncthreads();
 #pragma unroll
-    for (int i = r * L::kNV / 2; i < (r + 1) * L::kNV / 2; ++i) {
-      const int vi = i * kWave + lane;
-      const int q = vi / L::kNV - 32 * r;
-      const int p = (vi % L::kNV) * L::kVec;
-      g[vi] = *reinterpret_cast<const vec_t*>(lds + q * L::kRow + p);
-    }
+    for (int i = 0; i < L::kH; ++i) g[L::gvec(r, i, lane)] = lv[L::lvec(i, lane)];
     __syncthreads();
   }
 }

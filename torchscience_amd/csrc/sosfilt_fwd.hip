@@ -24,6 +24,7 @@
 // wave only ever waits on windows that were issued before it.
 
 #include <stdlib.h>
+#include <type_traits>
 #include <string.h>
 
 #include "sos_common.h"
@@ -72,6 +73,24 @@ struct Coef {
         a2(static_cast<TACC>(tb[4])) {}
 };
 
+// v[j] += zero-input response j samples after state (w0, w1): from the float
+// table g (TABG) or by the DF2T homogeneous recurrence with (a1, a2).
+template <typename TACC, typename TR, int W, bool TABG>
+__device__ __forceinline__ void apply_pending(TACC (&v)[W], const float* g, TR w0, TR w1, TR a1,
+                                              TR a2) {
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    if constexpr (TABG) {
+      v[j] = static_cast<TACC>(fmaf(g[j + 1], w0, fmaf(g[j], w1, static_cast<TR>(v[j]))));
+    } else {
+      const TR cj = w0;
+      w0 = w1 - a1 * cj;
+      w1 = -a2 * cj;
+      v[j] = static_cast<TACC>(static_cast<TR>(v[j]) + cj);
+    }
+  }
+}
+
 // One workgroup = NW waves = one window of NW*64*W samples of one channel;
 // wave w owns the sub-window [w*64*W, (w+1)*64*W) and lane q of it the W
 // samples [q*W, (q+1)*W) of that sub-window.
@@ -99,6 +118,7 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
   constexpr int kLog2NW = NW == 1 ? 0 : NW == 2 ? 1 : NW == 4 ? 2 : 3;
   static_assert((1 << kLog2NW) == NW, "NW must be 1, 2, 4 or 8");
   static_assert(!CHAINED || (NW == 1 && !TAIL), "chained windows are single-wave");
+  static_assert(W < kGLen, "the zero-input response table covers 64 samples");
   __shared__ __align__(16) TIO lds[(LDS && !TAIL) ? NW * LdsStage<TIO, W>::kElems : 8];
   __shared__ double wagg[NW][2];  // wave aggregates of the current section
   __shared__ double wsin[2];      // incoming state of the window
@@ -144,8 +164,14 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
   }
 
   // The zero-input response that corrects section s is applied inside the
-  // pass of section s+1 (one sweep over the registers per section).
+  // pass of section s+1 (one sweep over the registers per section).  FP32
+  // recurrences read it from the section's float table g (two FMAs per
+  // sample: the table is uniform across lanes and scalar-loaded); FP64
+  // recurrences run its 2-term recurrence.  The TAIL variant applies each
+  // correction at once, so none is pending in its sweep.
+  constexpr bool kTabG = std::is_same<TR, float>::value;
   TR pw0 = 0, pw1 = 0, pa1 = 0, pa2 = 0;
+  const float* pg = table_g(tf);  // g of the section whose correction is pending
   for (int s = 0; s < a.S; ++s) {
     const double* tb = tf + static_cast<int64_t>(s) * kTableStride;
     const Coef<TR> q(tb);
@@ -156,11 +182,17 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
 #pragma unroll
     for (int j = 0; j < W; ++j) {
       if (!TAIL || j < m) {
-        const TR cj = pw0;
-        pw0 = pw1 - pa1 * cj;
-        pw1 = -pa2 * cj;
-        const TR u = static_cast<TR>(v[j]) + cj;
-        const TR yv = q.b0 * u + z0;
+        TR u = static_cast<TR>(v[j]);
+        if constexpr (TAIL) {
+        } else if constexpr (kTabG) {
+          u = fma3s(pg[j + 1], pw0, fma3s(pg[j], pw1, u));
+        } else {
+          const TR cj = pw0;
+          pw0 = pw1 - pa1 * cj;
+          pw1 = -pa2 * cj;
+          u += cj;
+        }
+        const TR yv = fma3(q.b0, u, z0);
         z0 = q.b1 * u + (z1 - q.a1 * yv);
         z1 = q.b2 * u + (-q.a2 * yv);
         v[j] = static_cast<TACC>(yv);
@@ -296,6 +328,7 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
     pw1 = static_cast<TR>(s1);
     pa1 = q.a1;
     pa2 = q.a2;
+    pg = table_g(tb);
     if (TAIL) {
       // apply now, freezing the homogeneous state after the last sample
       TR w0 = pw0, w1 = pw1;
@@ -326,13 +359,7 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
       // make the registers exact now (the pending correction becomes zero)
       // and keep section s's output for the backward pass
       if (!TAIL) {
-#pragma unroll
-        for (int j = 0; j < W; ++j) {
-          const TR cj = pw0;
-          pw0 = pw1 - pa1 * cj;
-          pw1 = -pa2 * cj;
-          v[j] = static_cast<TACC>(static_cast<TR>(v[j]) + cj);
-        }
+        apply_pending<TACC, TR, W, kTabG>(v, pg, pw0, pw1, pa1, pa2);
         pw0 = 0;
         pw1 = 0;
       }
@@ -345,15 +372,7 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
     }
     if (NW > 1) __syncthreads();  // wagg / wsin are reused by the next section
   }
-  if (!TAIL) {
-#pragma unroll
-    for (int j = 0; j < W; ++j) {
-      const TR cj = pw0;
-      pw0 = pw1 - pa1 * cj;
-      pw1 = -pa2 * cj;
-      v[j] = static_cast<TACC>(static_cast<TR>(v[j]) + cj);
-    }
-  }
+  if (!TAIL) apply_pending<TACC, TR, W, kTabG>(v, pg, pw0, pw1, pa1, pa2);
 
   if (LDS && !TAIL && yvec)
     store_window_lds<TIO, TACC, W>(yr + t0w, my_lds, v);
@@ -361,9 +380,160 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
     store_run<TIO, TACC, W>(yr + t0, yvec, m, v);
 }
 
+// ---- channel-persistent form ----------------------------------------------
+//
+// One wave per channel walks the channel's windows in order.  The incoming
+// state of every section at the next window is the inclusive state the wave
+// scan produced for the current one, so it stays in registers (lane s holds
+// section s's) and no wave waits on another.  The next window's input is
+// loaded into registers while the current window is filtered, so with as
+// many channels as the GPU holds waves the HBM traffic overlaps the VALU
+// work instead of arriving in bursts at window boundaries.
+
+// One window of the channel-persistent form.  FULL: 64*W samples, the
+// corrections deferred into the next section's sweep.  !FULL: the partial
+// last window (lane q has m samples), corrections applied at once, and the
+// state after the last sample taken from lane `owner`.  (ws0, ws1): lane s
+// holds section s's state at the window start; on return, at the window end.
+template <typename TACC, typename TR, int W, bool FULL>
+__device__ __forceinline__ void chan_window(TACC (&v)[W], const double* __restrict__ tf, int S,
+                                            int m, int owner, double& ws0, double& ws1) {
+  constexpr bool kTabG = std::is_same<TR, float>::value;
+  const int lane = threadIdx.x & (kWave - 1);
+  TR pw0 = 0, pw1 = 0, pa1 = 0, pa2 = 0;
+  const float* pg = table_g(tf);
+  for (int s = 0; s < S; ++s) {
+    const double* tb = tf + static_cast<int64_t>(s) * kTableStride;
+    const Coef<TR> q(tb);
+    TR z0 = 0, z1 = 0;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      if (FULL || j < m) {
+        TR u = static_cast<TR>(v[j]);
+        if constexpr (FULL && kTabG) {
+          u = fma3s(pg[j + 1], pw0, fma3s(pg[j], pw1, u));
+        } else if constexpr (FULL) {
+          const TR cj = pw0;
+          pw0 = pw1 - pa1 * cj;
+          pw1 = -pa2 * cj;
+          u += cj;
+        }
+        const TR yv = fma3(q.b0, u, z0);
+        z0 = q.b1 * u + (z1 - q.a1 * yv);
+        z1 = q.b2 * u + (-q.a2 * yv);
+        v[j] = static_cast<TACC>(yv);
+      }
+    }
+    // the window's incoming state enters through lane 0 before the scan
+    const double sw0 = readlane_d(ws0, s), sw1 = readlane_d(ws1, s);
+    double e0 = static_cast<double>(z0), e1 = static_cast<double>(z1);
+    if (lane == 0) {
+      const Mat2 M = load_mat(table_pow(tb, 0));
+      e0 = fma(M.b, sw1, fma(M.a, sw0, e0));
+      e1 = fma(M.d, sw1, fma(M.c, sw0, e1));
+    }
+    double incl0, incl1, s0, s1;
+    wave_scan(tb, lane, e0, e1, incl0, incl1, s0, s1);
+    if (lane == 0) {
+      s0 = sw0;
+      s1 = sw1;
+    }
+    if constexpr (FULL) {
+      if (lane == s) {
+        ws0 = incl0;
+        ws1 = incl1;
+      }
+      pw0 = static_cast<TR>(s0);
+      pw1 = static_cast<TR>(s1);
+      pa1 = q.a1;
+      pa2 = q.a2;
+      pg = table_g(tb);
+    } else {
+      TR w0 = static_cast<TR>(s0), w1 = static_cast<TR>(s1);
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        if (j < m) {
+          const TR cj = w0;
+          v[j] = static_cast<TACC>(static_cast<TR>(v[j]) + cj);
+          w0 = w1 - q.a1 * cj;
+          w1 = -q.a2 * cj;
+        }
+      }
+      const double f0 = readlane_d(static_cast<double>(w0) + static_cast<double>(z0), owner);
+      const double f1 = readlane_d(static_cast<double>(w1) + static_cast<double>(z1), owner);
+      if (lane == s) {
+        ws0 = f0;
+        ws1 = f1;
+      }
+    }
+  }
+  if constexpr (FULL) apply_pending<TACC, TR, W, kTabG>(v, pg, pw0, pw1, pa1, pa2);
+}
+
+template <typename TIO, typename TACC, typename TR, int W>
+__global__ __launch_bounds__(64) void sos_chan_kernel(FwdArgs a,
+                                                      const double* __restrict__ tables) {
+  using L = LdsStage<TIO, W>;
+  static_assert(W < kGLen, "the zero-input response table covers 64 samples");
+  __shared__ __align__(16) TIO lds[L::kElems];
+  const int lane = threadIdx.x;
+  const int64_t c = blockIdx.x;
+  const int S = a.S;
+  const int64_t T = a.T;
+  const TIO* xr = static_cast<const TIO*>(a.x) + (c % a.x_rows) * a.x_stride;
+  TIO* yr = static_cast<TIO*>(a.y) + c * a.y_stride;
+  const double* tf = tables + (c / a.cpf) * static_cast<int64_t>(S) * kTableStride;
+  const int64_t win = static_cast<int64_t>(kWave) * W;
+  const int64_t nfull = T / win;
+  const bool vec = ((reinterpret_cast<uintptr_t>(xr) | reinterpret_cast<uintptr_t>(yr)) & 15u) == 0;
+
+  double ws0 = 0.0, ws1 = 0.0;  // lane s < S: section s's state
+  if ((a.flags & TSIIR_HAS_ZI) && lane < S) {
+    const int64_t zrow = (a.flags & TSIIR_ZI_BROADCAST) ? 0 : c;
+    const TACC* zp = static_cast<const TACC*>(a.zi) + (zrow * S + lane) * 2;
+    ws0 = static_cast<double>(zp[0]);
+    ws1 = static_cast<double>(zp[1]);
+  }
+
+  u32x4 pf[L::kNV];
+  if (vec && nfull > 0) prefetch_window<TIO, W>(xr, pf);
+  for (int64_t k = 0; k < nfull; ++k) {
+    const int64_t k0 = k * win;
+    TACC v[W];
+    if (vec) {
+      transpose_window_lds<TIO, TACC, W>(pf, lds, v);
+      if (k + 1 < nfull) prefetch_window<TIO, W>(xr + k0 + win, pf);
+    } else {
+      load_run<TIO, TACC, W>(xr + k0 + lane * W, false, W, v);
+    }
+    chan_window<TACC, TR, W, true>(v, tf, S, W, 0, ws0, ws1);
+    if (vec)
+      store_window_lds<TIO, TACC, W>(yr + k0, lds, v);
+    else
+      store_run<TIO, TACC, W>(yr + k0 + lane * W, false, W, v);
+  }
+  if (nfull * win < T) {
+    const int64_t k0 = nfull * win;
+    const int64_t t0 = k0 + static_cast<int64_t>(lane) * W;
+    const int64_t rem = T - t0;
+    const int m = rem <= 0 ? 0 : (rem >= W ? W : static_cast<int>(rem));
+    const int owner = static_cast<int>((T - 1 - k0) / W);
+    TACC v[W];
+    load_run<TIO, TACC, W>(xr + (m > 0 ? t0 : 0), false, m, v);
+    chan_window<TACC, TR, W, false>(v, tf, S, m, owner, ws0, ws1);
+    store_run<TIO, TACC, W>(yr + (m > 0 ? t0 : 0), false, m, v);
+  }
+  if ((a.flags & TSIIR_WRITE_ZF) && lane < S) {
+    TACC* zp = static_cast<TACC*>(a.zf) + (c * S + lane) * 2;
+    zp[0] = static_cast<TACC>(ws0);
+    zp[1] = static_cast<TACC>(ws1);
+  }
+}
+
 // Per (filter, section): normalised coefficients and P^(W*2^i), i < kNPow,
 // where P is the forward DF2T state matrix A = [[-a1, 1], [-a2, 0]] or, for
-// the adjoint recursion, A' = [[-a1, -a2], [1, 0]].
+// the adjoint recursion, A' = [[-a1, -a2], [1, 0]]; then the float
+// zero-input response table g of the forward DF2T (table_g).
 template <typename TC>
 __global__ __launch_bounds__(256) void sos_prep_kernel(const TC* sos, int64_t n_rows, int log2w,
                                                        int adjoint, double* tables,
@@ -400,6 +570,17 @@ __global__ __launch_bounds__(256) void sos_prep_kernel(const TC* sos, int64_t n_
     A = mat_mul(A, A);
   }
   for (int p = 8 + 4 * kNPow; p < kTableStride; ++p) t[p] = 0.0;
+  // zero-input response of the forward DF2T, computed in FP64
+  float* g = reinterpret_cast<float*>(t + kGOff);
+  double gm2 = 0.0, gm1 = 1.0;
+  g[0] = 0.0f;
+  g[1] = 1.0f;
+  for (int j = 2; j < kGLen; ++j) {
+    const double gj = -da1 * gm1 - da2 * gm2;
+    g[j] = static_cast<float>(gj);
+    gm2 = gm1;
+    gm1 = gj;
+  }
 }
 
 void launch_prep(const void* sos, int coef_dtype, int64_t n_rows, int log2w, bool adjoint,
@@ -533,6 +714,62 @@ static tsiir_status launch_fwd(const tsiir_sosfilt_desc* d, const FwdGeometry& g
   return hipGetLastError() == hipSuccess ? TSIIR_OK : TSIIR_ELAUNCH;
 }
 
+// Channel-persistent launch: prep (tables for its W) + one wave per channel.
+template <typename TIO, typename TACC, typename TR, int W>
+static tsiir_status launch_chan(const tsiir_sosfilt_desc* d, const FwdGeometry& g,
+                                const void* x, void* y, const void* sos, const void* zi,
+                                void* zf, char* ws, hipStream_t stream) {
+  double* tables = reinterpret_cast<double*>(ws + sizeof(WsHeader));
+  const int64_t n_rows = g.n_filters * d->n_sections;
+  launch_prep(sos, d->coef_dtype, n_rows, W == 64 ? 6 : (W == 32 ? 5 : 4), false, tables,
+              nullptr, 0, stream);
+  FwdArgs a{};
+  a.x = x;
+  a.y = y;
+  a.zi = zi;
+  a.zf = zf;
+  a.tables = tables;
+  a.B = d->channels;
+  a.T = d->n_samples;
+  a.x_rows = d->x_rows;
+  a.x_stride = d->x_row_stride;
+  a.y_stride = d->y_row_stride;
+  a.cpf = d->chans_per_filter;
+  a.S = d->n_sections;
+  a.flags = d->flags;
+  hipLaunchKernelGGL((sos_chan_kernel<TIO, TACC, TR, W>),
+                     dim3(static_cast<unsigned>(d->channels)), dim3(kWave), 0, stream, a,
+                     a.tables);
+  return hipGetLastError() == hipSuccess ? TSIIR_OK : TSIIR_ELAUNCH;
+}
+
+// The window-parallel form is the default: at steady state it matched or
+// beat the channel-persistent form on every measured shape (its 4 waves per
+// SIMD hide the scan latency better than the 2 that the prefetch registers
+// leave).  TSIIR_PATH_CHANNEL in the flags (or TSIIR_FWD_PATH=chan, for
+// comparison measurements) selects the channel-persistent form when no
+// section outputs are kept for a backward pass; TSIIR_CHAN_W=32 or 64 picks
+// its FP32 samples per lane.
+static bool use_chan(const tsiir_sosfilt_desc* d, const void* saved) {
+  static const int forced = [] {
+    const char* e = getenv("TSIIR_FWD_PATH");
+    if (e && strcmp(e, "window") == 0) return 0;
+    if (e && strcmp(e, "chan") == 0) return 1;
+    return -1;
+  }();
+  if (saved != nullptr || d->n_sections > kWave) return false;
+  if (d->flags & TSIIR_PATH_WINDOW) return false;
+  if (d->flags & TSIIR_PATH_CHANNEL) return true;
+  return forced == 1;
+}
+static int chan_w_f32() {
+  static const int w = [] {
+    const char* e = getenv("TSIIR_CHAN_W");
+    return (e && strcmp(e, "32") == 0) ? 32 : 64;
+  }();
+  return w;
+}
+
 tsiir_status forward_impl(const tsiir_sosfilt_desc* d, const void* x, void* y, const void* sos,
                           const void* zi, void* zf, void* workspace, size_t workspace_bytes,
                           void* stream, void* saved) {
@@ -543,6 +780,7 @@ tsiir_status forward_impl(const tsiir_sosfilt_desc* d, const void* x, void* y, c
   if (d->coef_dtype != TSIIR_F32 && d->coef_dtype != TSIIR_F64) return TSIIR_EINVAL;
   if (d->mode != TSIIR_MODE_PLAIN && d->mode != TSIIR_MODE_COMPENSATED) return TSIIR_EINVAL;
   if (d->flags & TSIIR_REVERSE) return TSIIR_EUNSUPPORTED;
+  if ((d->flags & TSIIR_PATH_WINDOW) && (d->flags & TSIIR_PATH_CHANNEL)) return TSIIR_EINVAL;
   const bool has_zi = (d->flags & TSIIR_HAS_ZI) != 0;
   const bool write_zf = (d->flags & TSIIR_WRITE_ZF) != 0;
   if (has_zi && !zi) return TSIIR_EINVAL;
@@ -572,6 +810,21 @@ tsiir_status forward_impl(const tsiir_sosfilt_desc* d, const void* x, void* y, c
   if (g.n_windows * d->channels > 0x7fffffffLL) return TSIIR_EUNSUPPORTED;
   char* ws = static_cast<char*>(workspace);
   const bool comp = d->mode == TSIIR_MODE_COMPENSATED;
+  if (use_chan(d, saved)) {
+    const bool w64 = chan_w_f32() == 64;
+    switch (d->io_dtype) {
+      case TSIIR_F32:
+        if (comp) return launch_chan<float, float, double, 32>(d, g, x, y, sos, zi, zf, ws, st);
+        return w64 ? launch_chan<float, float, float, 64>(d, g, x, y, sos, zi, zf, ws, st)
+                   : launch_chan<float, float, float, 32>(d, g, x, y, sos, zi, zf, ws, st);
+      case TSIIR_BF16:
+        if (comp) return launch_chan<bf16_t, float, double, 32>(d, g, x, y, sos, zi, zf, ws, st);
+        return w64 ? launch_chan<bf16_t, float, float, 64>(d, g, x, y, sos, zi, zf, ws, st)
+                   : launch_chan<bf16_t, float, float, 32>(d, g, x, y, sos, zi, zf, ws, st);
+      default:
+        return launch_chan<double, double, double, 32>(d, g, x, y, sos, zi, zf, ws, st);
+    }
+  }
   switch (d->io_dtype) {
     case TSIIR_F32:
       return comp ? launch_fwd<float, float, double, 32>(d, g, x, y, sos, zi, zf, ws, st, saved)

@@ -50,10 +50,12 @@ def _desc(*, channels, n_samples, n_sections, io_dtype, coef_dtype, flags, x_row
 def sosfilt_rows_forward(sos: torch.Tensor, x: torch.Tensor, zi: torch.Tensor | None,
                          *, channels: int, chans_per_filter: int, want_zf: bool,
                          zi_broadcast: bool = False, io_dtype: torch.dtype | None = None,
-                         mode: int = _lib.MODE_PLAIN):
+                         mode: int = _lib.MODE_PLAIN, path: str | None = None):
     """y (channels, T) and zf (channels, S, 2) for planned row inputs.
 
-    Channel c reads x row c % x.shape[0] and filter c // chans_per_filter."""
+    Channel c reads x row c % x.shape[0] and filter c // chans_per_filter.
+    path: None (chosen from the shape), "window" or "channel" (forces the
+    work decomposition; results agree to rounding)."""
     dev = _require_device(sos, x, zi)
     L = _lib.load()
     io = io_dtype or x.dtype
@@ -69,6 +71,8 @@ def sosfilt_rows_forward(sos: torch.Tensor, x: torch.Tensor, zi: torch.Tensor | 
             flags |= _lib.ZI_BROADCAST
     if want_zf:
         flags |= _lib.WRITE_ZF
+    if path is not None:
+        flags |= {"window": _lib.PATH_WINDOW, "channel": _lib.PATH_CHANNEL}[path]
     d = _desc(channels=channels, n_samples=T, n_sections=S, io_dtype=io, coef_dtype=sos.dtype,
               flags=flags, x_rows=max(x.shape[0], 1),
               x_row_stride=x.stride(0) if x.dim() == 2 else T, y_row_stride=T,
