@@ -94,8 +94,9 @@ def make_workload(name, dev):
         B, T = 1024, 16384
         g.manual_seed(3)
         sos = butterworth_sos(8, 0.2)
-        desc = ("differentiation: sosfilt fp32 forward+backward, x[1024,16384], learnable "
-                "4-section SOS (order-8 Butterworth lowpass 0.2), zi (1024,4,2)")
+        desc = ("differentiation: sosfilt fp32 training step, x[1024,16384], learnable "
+                "4-section SOS (order-8 Butterworth lowpass 0.2), zi (1024,4,2); MSE loss "
+                "against 3 sines (f ~ U[0.01,0.4] of Nyquist) + 0.1 N(0,1); grads of x, sos, zi")
         cpf = B
     x = torch.randn(B, T, device=dev, generator=g, dtype=torch.float32)
     if name == "c5bf16":
@@ -289,11 +290,21 @@ def main():
         sos_p = sos.clone().requires_grad_(True)
         xg = x.clone().requires_grad_(True)
         zi = torch.zeros(B, S, 2, device=dev, requires_grad=True)
+        gt = torch.Generator(device=dev)
+        gt.manual_seed(33)
+        fr = torch.rand(3, device=dev, generator=gt, dtype=torch.float64) * 0.39 + 0.01
+        n = torch.arange(T, device=dev, dtype=torch.float64)
+        tgt = (torch.sin(torch.pi * fr[:, None] * n).sum(0).float()
+               + 0.1 * torch.randn(B, T, device=dev, generator=gt))
 
         def step():
+            # one training step: fresh gradients, forward, MSE loss, backward
+            sos_p.grad = xg.grad = zi.grad = None
             y, zf = sosfilt(sos_p, xg, zi=zi)
-            y.backward(torch.ones_like(y))
+            loss = torch.nn.functional.mse_loss(y, tgt)
+            loss.backward()
             return y
+        # algorithmic: forward x read + y write, backward dy read + x read + dx write
         algo_bytes = B * T * 4 * 2 + B * T * 4 * 3
     elif per_channel:
         def step():

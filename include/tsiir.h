@@ -146,6 +146,29 @@ tsiir_status tsiir_sosfilt_backward(const tsiir_sosfilt_desc* desc,
                                     void* dzi, void* workspace,
                                     size_t workspace_bytes, void* stream);
 
+/* Training form of tsiir_sosfilt_forward: also writes the output of every
+ * section but the last to `saved`, (n_sections - 1, channels, n_samples)
+ * contiguous in the io dtype (NULL allowed when n_sections == 1), so that
+ * tsiir_sosfilt_backward_saved needs no forward recompute.  Replaces the
+ * activations the reference's autograd tape keeps for _lfilter.py:153-166. */
+tsiir_status tsiir_sosfilt_forward_saved(const tsiir_sosfilt_desc* desc,
+                                         const void* x, void* y, const void* sos,
+                                         const void* zi, void* zf, void* saved,
+                                         void* workspace, size_t workspace_bytes,
+                                         void* stream);
+
+/* tsiir_sosfilt_backward from the forward output y (channels, n_samples) and
+ * the section outputs `saved` written by tsiir_sosfilt_forward_saved, without
+ * recomputing the forward.  Same gradients and conventions. */
+size_t tsiir_sosfilt_backward_saved_workspace_bytes(const tsiir_sosfilt_desc* desc);
+tsiir_status tsiir_sosfilt_backward_saved(const tsiir_sosfilt_desc* desc,
+                                          const void* x, const void* y,
+                                          const void* saved, const void* sos,
+                                          const void* zi, const void* dy,
+                                          const void* dzf, void* dx, void* dsos,
+                                          void* dzi, void* workspace,
+                                          size_t workspace_bytes, void* stream);
+
 /* ---- serving: ragged batches of streaming chunks --------------------------
  * The reference has no serving layer; these entry points serve the streaming
  * use of sosfilt with zi/zf (_sosfilt.py:98-133): a chunk of a persistent

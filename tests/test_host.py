@@ -39,6 +39,16 @@ def test_library_exports_every_declared_symbol():
     assert _lib.load().tsiir_version() == 1
 
 
+def test_binding_arity_matches_header():
+    """Every declared function's parameter count equals its ctypes argtypes."""
+    text = open(os.path.join(ROOT, "include", "tsiir.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    L = _lib.load()
+    for name, params in re.findall(r"\b(tsiir_[a-z_0-9]+)\s*\(([^)]*)\)\s*;", text):
+        n = 0 if params.strip() in ("", "void") else params.count(",") + 1
+        assert len(getattr(L, name).argtypes or []) == n, name
+
+
 def test_desc_layout_matches_header():
     assert ctypes.sizeof(_lib.SosfiltDesc) == 72
     assert _lib.SosfiltDesc.chans_per_filter.offset == 64

@@ -32,6 +32,9 @@ EXPORTED = (
     "tsiir_lfilter_forward",
     "tsiir_sosfilt_backward_workspace_bytes",
     "tsiir_sosfilt_backward",
+    "tsiir_sosfilt_forward_saved",
+    "tsiir_sosfilt_backward_saved_workspace_bytes",
+    "tsiir_sosfilt_backward_saved",
     "tsiir_serve_table_bytes",
     "tsiir_serve_prepare_filter",
     "tsiir_serve_batch",
@@ -102,6 +105,12 @@ def load(path: str | None = None) -> ctypes.CDLL:
     L.tsiir_sosfilt_backward_workspace_bytes.argtypes = [dp]
     L.tsiir_sosfilt_backward.restype = ctypes.c_int
     L.tsiir_sosfilt_backward.argtypes = [dp, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, vp]
+    L.tsiir_sosfilt_forward_saved.restype = ctypes.c_int
+    L.tsiir_sosfilt_forward_saved.argtypes = [dp, vp, vp, vp, vp, vp, vp, vp, sz, vp]
+    L.tsiir_sosfilt_backward_saved_workspace_bytes.restype = sz
+    L.tsiir_sosfilt_backward_saved_workspace_bytes.argtypes = [dp]
+    L.tsiir_sosfilt_backward_saved.restype = ctypes.c_int
+    L.tsiir_sosfilt_backward_saved.argtypes = [dp] + [vp] * 11 + [sz, vp]
     L.tsiir_serve_table_bytes.restype = sz
     L.tsiir_serve_table_bytes.argtypes = [ctypes.c_int32]
     L.tsiir_serve_prepare_filter.restype = ctypes.c_int

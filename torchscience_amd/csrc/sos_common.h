@@ -369,24 +369,28 @@ struct LdsStage {
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-// LDS rows -> the lane's samples [r*W/2, (r+1)*W/2)
-template <typename TIO, typename TACC, int W>
+// LDS rows -> the lane's samples [r*W/2, (r+1)*W/2).  REV: lane q takes
+// row 63-q with its elements in reverse order (time-reversed windows).
+template <typename TIO, typename TACC, int W, bool REV = false>
 __device__ __forceinline__ void lds_rows_to_regs(const TIO* lds, int r, TACC (&v)[W]) {
   using L = LdsStage<TIO, W>;
   using vec_t = typename IOTraits<TIO>::vec_t;
   const int lane = threadIdx.x & (kWave - 1);
-  const vec_t* row = reinterpret_cast<const vec_t*>(lds) + lane * L::kRowV;
+  const vec_t* row =
+      reinterpret_cast<const vec_t*>(lds) + (REV ? kWave - 1 - lane : lane) * L::kRowV;
 #pragma unroll
   for (int t = 0; t < L::kH; ++t) {
     const vec_t rv = row[t];
     const TIO* e = reinterpret_cast<const TIO*>(&rv);
 #pragma unroll
-    for (int j = 0; j < L::kVec; ++j)
-      v[(r * L::kH + t) * L::kVec + j] = static_cast<TACC>(IOTraits<TIO>::to_f(e[j]));
+    for (int j = 0; j < L::kVec; ++j) {
+      const int idx = (r * L::kH + t) * L::kVec + j;
+      v[REV ? W - 1 - idx : idx] = static_cast<TACC>(IOTraits<TIO>::to_f(e[j]));
+    }
   }
 }
 
-template <typename TIO, typename TACC, int W>
+template <typename TIO, typename TACC, int W, bool REV = false>
 __device__ __forceinline__ void load_window_lds(const TIO* win, TIO* lds, TACC (&v)[W]) {
   using L = LdsStage<TIO, W>;
   const int lane = threadIdx.x & (kWave - 1);
@@ -397,7 +401,7 @@ __device__ __forceinline__ void load_window_lds(const TIO* win, TIO* lds, TACC (
 #pragma unroll
     for (int i = 0; i < L::kH; ++i) lv[L::lvec(i, lane)] = g[L::gvec(r, i, lane)];
     __syncthreads();
-    lds_rows_to_regs<TIO, TACC, W>(lds, r, v);
+    lds_rows_to_regs<TIO, TACC, W, REV>(lds, r, v);
     __syncthreads();
   }
 }
@@ -433,7 +437,7 @@ __device__ __forceinline__ void transpose_window_lds(const u32x4 (&pf)[LdsStage<
   }
 }
 
-template <typename TIO, typename TACC, int W>
+template <typename TIO, typename TACC, int W, bool REV = false>
 __device__ __forceinline__ void store_window_lds(TIO* win, TIO* lds, const TACC (&v)[W]) {
   using L = LdsStage<TIO, W>;
   using vec_t = typename IOTraits<TIO>::vec_t;
@@ -442,14 +446,16 @@ __device__ __forceinline__ void store_window_lds(TIO* win, TIO* lds, const TACC 
   const u32x4* lv = reinterpret_cast<const u32x4*>(lds);
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
-    vec_t* row = reinterpret_cast<vec_t*>(lds) + lane * L::kRowV;
+    vec_t* row = reinterpret_cast<vec_t*>(lds) + (REV ? kWave - 1 - lane : lane) * L::kRowV;
 #pragma unroll
     for (int t = 0; t < L::kH; ++t) {
       vec_t rv;
       TIO* e = reinterpret_cast<TIO*>(&rv);
 #pragma unroll
-      for (int j = 0; j < L::kVec; ++j)
-        e[j] = IOTraits<TIO>::from_f(v[(r * L::kH + t) * L::kVec + j]);
+      for (int j = 0; j < L::kVec; ++j) {
+        const int idx = (r * L::kH + t) * L::kVec + j;
+        e[j] = IOTraits<TIO>::from_f(v[REV ? W - 1 - idx : idx]);
+      }
       row[t] = rv;
     }
     __syncthreads();

@@ -878,4 +878,14 @@ tsiir_status tsiir_sosfilt_forward(const tsiir_sosfilt_desc* d, const void* x, v
   return forward_impl(d, x, y, sos, zi, zf, workspace, workspace_bytes, stream, nullptr);
 }
 
+tsiir_status tsiir_sosfilt_forward_saved(const tsiir_sosfilt_desc* desc, const void* x, void* y,
+                                         const void* sos, const void* zi, void* zf, void* saved,
+                                         void* workspace, size_t workspace_bytes, void* stream) {
+  if (!desc) return TSIIR_EINVAL;
+  if (desc->n_sections > 1 && !saved) return TSIIR_EINVAL;
+  if (desc->flags & TSIIR_PATH_CHANNEL) return TSIIR_EINVAL;
+  return forward_impl(desc, x, y, sos, zi, zf, workspace, workspace_bytes, stream,
+                      desc->n_sections > 1 ? saved : nullptr);
+}
+
 }  // extern "C"

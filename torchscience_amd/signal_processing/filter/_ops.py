@@ -50,12 +50,15 @@ def _desc(*, channels, n_samples, n_sections, io_dtype, coef_dtype, flags, x_row
 def sosfilt_rows_forward(sos: torch.Tensor, x: torch.Tensor, zi: torch.Tensor | None,
                          *, channels: int, chans_per_filter: int, want_zf: bool,
                          zi_broadcast: bool = False, io_dtype: torch.dtype | None = None,
-                         mode: int = _lib.MODE_PLAIN, path: str | None = None):
+                         mode: int = _lib.MODE_PLAIN, path: str | None = None,
+                         keep_sections: bool = False):
     """y (channels, T) and zf (channels, S, 2) for planned row inputs.
 
     Channel c reads x row c % x.shape[0] and filter c // chans_per_filter.
     path: None (chosen from the shape), "window" or "channel" (forces the
-    work decomposition; results agree to rounding)."""
+    work decomposition; results agree to rounding).  keep_sections: also
+    return the outputs of sections 0..S-2, (S-1, channels, T), for the
+    backward (else None)."""
     dev = _require_device(sos, x, zi)
     L = _lib.load()
     io = io_dtype or x.dtype
@@ -82,6 +85,14 @@ def sosfilt_rows_forward(sos: torch.Tensor, x: torch.Tensor, zi: torch.Tensor | 
         raise _lib.TsiirError("invalid sosfilt descriptor")
     ws = _lib.workspaces.get(dev, nbytes, "fwd")
     st = _lib.stream_handle(dev)
+    if keep_sections:
+        saved = torch.empty((max(S - 1, 0), channels, T), dtype=io, device=dev)
+        _lib.check(L.tsiir_sosfilt_forward_saved(d, _lib.ptr(x), _lib.ptr(y), _lib.ptr(sos),
+                                                 _lib.ptr(zi), _lib.ptr(zf),
+                                                 _lib.ptr(saved) if S > 1 else None,
+                                                 ws.data_ptr(), ws.numel(), st),
+                   "tsiir_sosfilt_forward_saved")
+        return y, zf, saved
     _lib.check(L.tsiir_sosfilt_forward(d, _lib.ptr(x), _lib.ptr(y), _lib.ptr(sos), _lib.ptr(zi),
                                        _lib.ptr(zf), ws.data_ptr(), ws.numel(), st),
                "tsiir_sosfilt_forward")
@@ -89,8 +100,12 @@ def sosfilt_rows_forward(sos: torch.Tensor, x: torch.Tensor, zi: torch.Tensor | 
 
 
 def sosfilt_rows_backward(sos, x, zi, dy, dzf, *, chans_per_filter: int,
-                          zi_broadcast: bool = False):
-    """(dx, dsos, dzi) for sosfilt_rows_forward with x rows == channels."""
+                          zi_broadcast: bool = False, y=None, saved=None):
+    """(dx, dsos, dzi) for sosfilt_rows_forward with x rows == channels.
+
+    With y (the forward output) and saved (the section outputs that
+    sosfilt_rows_forward(keep_sections=True) returned) the forward is not
+    recomputed."""
     dev = _require_device(sos, x, zi, dy, dzf)
     L = _lib.load()
     B, T = dy.shape
@@ -106,7 +121,9 @@ def sosfilt_rows_backward(sos, x, zi, dy, dzf, *, chans_per_filter: int,
     d = _desc(channels=B, n_samples=T, n_sections=S, io_dtype=x.dtype, coef_dtype=sos.dtype,
               flags=flags, x_rows=max(B, 1), x_row_stride=T, y_row_stride=T,
               chans_per_filter=max(chans_per_filter, 1))
-    nbytes = L.tsiir_sosfilt_backward_workspace_bytes(d)
+    use_saved = y is not None
+    nbytes = (L.tsiir_sosfilt_backward_saved_workspace_bytes(d) if use_saved
+              else L.tsiir_sosfilt_backward_workspace_bytes(d))
     if nbytes == 0:
         raise _lib.TsiirError("invalid sosfilt backward descriptor")
     ws = _lib.workspaces.get(dev, nbytes, "bwd")
@@ -114,6 +131,12 @@ def sosfilt_rows_backward(sos, x, zi, dy, dzf, *, chans_per_filter: int,
     dsos = torch.empty_like(sos)
     dzi = torch.empty((B, S, 2), dtype=acc, device=dev)
     st = _lib.stream_handle(dev)
+    if use_saved:
+        _lib.check(L.tsiir_sosfilt_backward_saved(
+            d, _lib.ptr(x), _lib.ptr(y), _lib.ptr(saved) if S > 1 else None, _lib.ptr(sos),
+            _lib.ptr(zi), _lib.ptr(dy), _lib.ptr(dzf), _lib.ptr(dx), _lib.ptr(dsos),
+            _lib.ptr(dzi), ws.data_ptr(), ws.numel(), st), "tsiir_sosfilt_backward_saved")
+        return dx, dsos, dzi
     _lib.check(L.tsiir_sosfilt_backward(d, _lib.ptr(x), _lib.ptr(sos), _lib.ptr(zi),
                                         _lib.ptr(dy), _lib.ptr(dzf), _lib.ptr(dx),
                                         _lib.ptr(dsos), _lib.ptr(dzi), ws.data_ptr(),
@@ -123,16 +146,17 @@ def sosfilt_rows_backward(sos, x, zi, dy, dzf, *, chans_per_filter: int,
 
 
 class SosfiltRows(torch.autograd.Function):
-    """Autograd over sosfilt_rows_forward.  Backward runs the hand-written
+    """Autograd over sosfilt_rows_forward.  The forward keeps every section's
+    output (tsiir_sosfilt_forward_saved); backward runs the hand-written
     adjoint kernels (time-reversed section filters with coefficient-gradient
-    accumulation) through tsiir_sosfilt_backward."""
+    accumulation) on them through tsiir_sosfilt_backward_saved."""
 
     @staticmethod
     def forward(ctx, x, sos, zi, channels, chans_per_filter, want_zf, zi_broadcast):
-        y, zf = sosfilt_rows_forward(sos, x, zi, channels=channels,
-                                     chans_per_filter=chans_per_filter, want_zf=want_zf,
-                                     zi_broadcast=zi_broadcast)
-        ctx.save_for_backward(x, sos, zi)
+        y, zf, saved = sosfilt_rows_forward(sos, x, zi, channels=channels,
+                                            chans_per_filter=chans_per_filter, want_zf=want_zf,
+                                            zi_broadcast=zi_broadcast, keep_sections=True)
+        ctx.save_for_backward(x, sos, zi, y, saved)
         ctx.meta = (channels, chans_per_filter, zi_broadcast, want_zf)
         if want_zf:
             return y, zf
@@ -140,7 +164,7 @@ class SosfiltRows(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, dy, dzf):
-        x, sos, zi = ctx.saved_tensors
+        x, sos, zi, y, saved = ctx.saved_tensors
         channels, cpf, zi_b, want_zf = ctx.meta
         if dy is None:
             dy = torch.zeros((channels, x.shape[1]), dtype=x.dtype, device=x.device)
@@ -152,7 +176,8 @@ class SosfiltRows(torch.autograd.Function):
             xr = x.repeat(channels // x.shape[0], 1) if channels % x.shape[0] == 0 else \
                 x[torch.arange(channels, device=x.device) % x.shape[0]]
         dx, dsos, dzi = sosfilt_rows_backward(sos, xr.contiguous(), zi, dy, dzf,
-                                              chans_per_filter=cpf, zi_broadcast=zi_b)
+                                              chans_per_filter=cpf, zi_broadcast=zi_b,
+                                              y=y, saved=saved)
         gx = gsos = gzi = None
         if ctx.needs_input_grad[0]:
             gx = dx if x.shape[0] == channels else \
