@@ -13,7 +13,14 @@ It restates, in numpy over the C DF2T core in ``oracle.c``:
   (1-D checks and messages, a0 normalisation, b/a padding, order-0 gain,
   zi reshape, DF2T loop);
 * ``batched_filter_apply`` - src/torchscience/signal_processing/filter/_batched.py:449-566;
-* ``sosfilt_zi`` / ``lfilter_zi`` - _sosfilt_zi.py:54-86 / _lfilter_zi.py:70-128.
+* ``sosfilt_zi`` / ``lfilter_zi`` - _sosfilt_zi.py:54-86 / _lfilter_zi.py:70-128;
+* ``sosfiltfilt`` - _sosfiltfilt.py:96-173 (default padlen from the count of
+  zeros at the origin, padding _pad_signal :176-217, forward pass from
+  sosfilt_zi * first sample, reversed pass from sosfilt_zi * last output,
+  crop);
+* ``filtfilt`` - _filtfilt.py:86-165 (a0 normalisation, padlen
+  3 * max(len(a), len(b)), the same two passes with lfilter_zi, order-0
+  gain b0^2).
 
 Deliberate difference from the reference: zi with ``axis != -1`` follows the
 documented meaning (zi batch dims are x's batch dims in order), not the
@@ -309,6 +316,92 @@ def sosfilt_zi(sos) -> np.ndarray:
         zi[s] = scale * lfilter_zi(b, a)
         scale = scale * b.sum() / a.sum()
     return zi
+
+
+def _pad_signal(x, padtype, padlen):
+    """Edge extension along the last axis (_sosfiltfilt.py:176-217,
+    _filtfilt.py:168-214)."""
+    if padtype == "odd":
+        left = 2 * x[..., 0:1] - x[..., 1:padlen + 1][..., ::-1]
+        right = 2 * x[..., -1:] - x[..., -padlen - 1:-1][..., ::-1]
+        return np.concatenate([left, x, right], axis=-1)
+    if padtype == "even":
+        left = x[..., 1:padlen + 1][..., ::-1]
+        right = x[..., -padlen - 1:-1][..., ::-1]
+        return np.concatenate([left, x, right], axis=-1)
+    if padtype == "constant":
+        left = np.repeat(x[..., 0:1], padlen, axis=-1)
+        right = np.repeat(x[..., -1:], padlen, axis=-1)
+        return np.concatenate([left, x, right], axis=-1)
+    raise ValueError(f"Unknown padtype: {padtype}")
+
+
+def sosfiltfilt(sos, x, axis: int = -1, padtype="odd", padlen=None):
+    """Zero-phase SOS filtering with the reference's semantics
+    (_sosfiltfilt.py:96-173)."""
+    sos = np.asarray(sos, dtype=np.float64)
+    if sos.ndim != 2 or sos.shape[1] != 6:
+        raise ValueError("sos must be shape (n_sections, 6)")
+    S = sos.shape[0]
+    if padlen is None:
+        ntaps = 2 * S + 1
+        ntaps -= min(int((sos[:, 2] == 0).sum()), int((sos[:, 5] == 0).sum()))
+        padlen = 3 * ntaps
+    xm = np.moveaxis(np.asarray(x, dtype=np.float64), axis, -1)
+    T = xm.shape[-1]
+    if padtype is not None and padlen >= T:
+        raise ValueError(f"padlen ({padlen}) must be less than x.shape[axis] ({T})")
+    xp = _pad_signal(xm, padtype, padlen) if padtype is not None and padlen > 0 else xm
+    batch, Tp = xp.shape[:-1], xp.shape[-1]
+    rows = np.ascontiguousarray(xp.reshape(-1, Tp))
+    if S > 0:
+        zi = sosfilt_zi(sos)
+        y1, _ = sosfilt_rows(sos, rows, zi[None] * rows[:, 0, None, None])
+        yr = np.ascontiguousarray(y1[:, ::-1])
+        y2, _ = sosfilt_rows(sos, yr, zi[None] * yr[:, 0, None, None])
+        y = y2[:, ::-1]
+    else:
+        y = rows
+    y = y.reshape(batch + (Tp,))
+    if padtype is not None and padlen > 0:
+        y = y[..., padlen:-padlen]
+    return np.moveaxis(y, -1, axis)
+
+
+def filtfilt(b, a, x, axis: int = -1, padtype="odd", padlen=None):
+    """Zero-phase (b, a) filtering with the reference's semantics
+    (_filtfilt.py:86-165)."""
+    b = np.atleast_1d(np.asarray(b, dtype=np.float64))
+    a = np.atleast_1d(np.asarray(a, dtype=np.float64))
+    if b.ndim != 1:
+        raise ValueError("b must be a 1-D tensor")
+    if a.ndim != 1:
+        raise ValueError("a must be a 1-D tensor")
+    if a[0] != 1.0:
+        b = b / a[0]
+        a = a / a[0]
+    order = max(b.size, a.size) - 1
+    if padlen is None:
+        padlen = 3 * max(b.size, a.size)
+    xm = np.moveaxis(np.asarray(x, dtype=np.float64), axis, -1)
+    T = xm.shape[-1]
+    if padtype is not None and padlen >= T:
+        raise ValueError(f"padlen ({padlen}) must be less than x.shape[axis] ({T})")
+    xp = _pad_signal(xm, padtype, padlen) if padtype is not None and padlen > 0 else xm
+    batch, Tp = xp.shape[:-1], xp.shape[-1]
+    rows = np.ascontiguousarray(xp.reshape(-1, Tp))
+    if order > 0:
+        zi = lfilter_zi(b, a)
+        y1, _ = lfilter_rows(b, a, rows, zi[None] * rows[:, 0:1])
+        yr = np.ascontiguousarray(y1[:, ::-1])
+        y2, _ = lfilter_rows(b, a, yr, zi[None] * yr[:, 0:1])
+        y = y2[:, ::-1]
+    else:
+        y = b[0] * b[0] * rows
+    y = y.reshape(batch + (Tp,))
+    if padtype is not None and padlen > 0:
+        y = y[..., padlen:-padlen]
+    return np.moveaxis(y, -1, axis)
 
 
 def impulse_state_response(sos_row: np.ndarray, n: int) -> np.ndarray:

@@ -80,6 +80,41 @@ def lf_cases():
     ]
 
 
+def ff_cases():
+    """(name, kind, filter, x-shape, axis, padtype, padlen) mirroring
+    test__sosfiltfilt.py and test__filtfilt.py (kind "sos": filter = sos;
+    kind "ba": filter = (b, a))."""
+    sb = ss.butter
+    return [
+        ("sos_butter5", "sos", sb(5, 0.25, output="sos"), (200,), -1, "odd", None),      # :13
+        ("sos_cheby1_4", "sos", ss.cheby1(4, 0.5, 0.3, output="sos"), (150,), -1, "odd", None),  # :29
+        ("sos_ellip3", "sos", ss.ellip(3, 0.5, 40, 0.2, output="sos"), (100,), -1, "odd", None),  # :45
+        ("sos_bessel4", "sos", ss.bessel(4, 0.3, output="sos"), (100,), -1, "odd", None),  # :61
+        ("sos_pad_even", "sos", sb(3, 0.3, output="sos"), (100,), -1, "even", None),     # :160
+        ("sos_pad_const", "sos", sb(3, 0.3, output="sos"), (100,), -1, "constant", None),  # :176
+        ("sos_pad_none", "sos", sb(3, 0.3, output="sos"), (100,), -1, None, None),       # :194
+        ("sos_padlen20", "sos", sb(3, 0.3, output="sos"), (100,), -1, "odd", 20),        # :214
+        ("sos_padlen0", "sos", sb(3, 0.3, output="sos"), (100,), -1, "odd", 0),          # :230
+        ("sos_axis_m1", "sos", sb(3, 0.3, output="sos"), (10, 100), -1, "odd", None),    # :250
+        ("sos_axis0", "sos", sb(3, 0.3, output="sos"), (100, 10), 0, "odd", None),       # :268
+        ("sos_long", "sos", sb(4, 0.2, output="sos"), (10000,), -1, "odd", None),        # :354
+        ("sos_ragged", "sos", sb(6, [0.05, 0.3], btype="band", output="sos"), (3, 9001), -1,
+         "odd", None),
+        ("ba_butter5", "ba", ss.butter(5, 0.25), (200,), -1, "odd", None),                # :13
+        ("ba_cheby1_4", "ba", ss.cheby1(4, 0.5, 0.3), (150,), -1, "odd", None),          # :30
+        ("ba_ellip3", "ba", ss.ellip(3, 0.5, 40, 0.2), (100,), -1, "odd", None),         # :47
+        ("ba_fir", "ba", (np.array([0.2, 0.3, 0.3, 0.2]), np.array([1.0])), (100,), -1,
+         "odd", None),                                                                    # :64
+        ("ba_pad_even", "ba", ss.butter(3, 0.3), (100,), -1, "even", None),              # :169
+        ("ba_pad_const", "ba", ss.butter(3, 0.3), (100,), -1, "constant", None),         # :186
+        ("ba_pad_none", "ba", ss.butter(3, 0.3), (100,), -1, None, None),                # :205
+        ("ba_padlen20", "ba", ss.butter(3, 0.3), (100,), -1, "odd", 20),                 # :226
+        ("ba_padlen0", "ba", ss.butter(3, 0.3), (100,), -1, "odd", 0),                   # :243
+        ("ba_axis_m1", "ba", ss.butter(3, 0.3), (10, 100), -1, "odd", None),             # :264
+        ("ba_order2", "ba", ss.butter(2, 0.3), (4, 300), -1, "odd", None),
+    ]
+
+
 def main():
     out = {}
     manifest = {"scipy": scipy.__version__, "numpy": np.__version__, "cases": []}
@@ -113,6 +148,22 @@ def main():
         for k, v in rec.items():
             out[f"lf/{name}/{k}"] = np.asarray(v)
         manifest["cases"].append({"kind": "lfilter", "name": name})
+    for i, (name, kind, filt, shape, axis, padtype, padlen) in enumerate(ff_cases()):
+        x = _rng(500 + i).standard_normal(shape)
+        rec = {"x": x, "axis": np.array(axis),
+               "padtype": np.array("none" if padtype is None else padtype),
+               "padlen": np.array(-1 if padlen is None else padlen)}
+        if kind == "sos":
+            rec["sos"] = filt
+            y = ss.sosfiltfilt(filt, x, axis=axis, padtype=padtype, padlen=padlen)
+        else:
+            rec["b"], rec["a"] = filt
+            y = ss.filtfilt(filt[0], filt[1], x, axis=axis, padtype=padtype, padlen=padlen)
+        rec["y"] = y
+        for k, v in rec.items():
+            out[f"ff/{name}/{k}"] = np.asarray(v)
+        manifest["cases"].append({"kind": "sosfiltfilt" if kind == "sos" else "filtfilt",
+                                  "name": name})
     # steady-state helpers
     for name, sos in [("butter5", ss.butter(5, 0.25, output="sos")),
                       ("butter6", ss.butter(6, 0.2, output="sos"))]:

@@ -145,3 +145,22 @@ def test_design_matches_scipy_magnitude():
     # the reference's even gain split: every section carries |k|^(1/S)
     sos = butterworth_sos(4, 0.125)
     assert np.allclose(sos[:, 0], sos[0, 0]) and sos.shape == (2, 6)
+
+
+def test_zero_phase_validation_matches_reference():
+    """sosfiltfilt / filtfilt raise the reference's ValueErrors before any
+    device work (_sosfiltfilt.py:96-121, _filtfilt.py:86-115)."""
+    from torchscience_amd.signal_processing.filter import filtfilt, sosfiltfilt
+    sos = torch.tensor([[0.2, 0.4, 0.2, 1.0, -0.5, 0.2]], dtype=torch.float64)
+    with pytest.raises(ValueError, match=r"sos must be shape \(n_sections, 6\)"):
+        sosfiltfilt(torch.ones(2, 5), torch.randn(100))
+    with pytest.raises(ValueError, match=r"padlen \(9\) must be less than x.shape\[axis\] \(9\)"):
+        sosfiltfilt(sos, torch.randn(9, dtype=torch.float64))
+    with pytest.raises(ValueError, match="Unknown padtype: wrap"):
+        sosfiltfilt(sos, torch.randn(100, dtype=torch.float64), padtype="wrap")
+    with pytest.raises(ValueError, match="a must be a 1-D tensor"):
+        filtfilt(torch.ones(3), torch.ones(2, 2), torch.randn(100))
+    with pytest.raises(ValueError, match=r"padlen \(12\) must be less"):
+        filtfilt(torch.ones(3), torch.ones(4), torch.randn(12))
+    with pytest.raises(RuntimeError, match="HIP"):
+        sosfiltfilt(sos, torch.randn(100, dtype=torch.float64))

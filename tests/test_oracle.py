@@ -163,3 +163,53 @@ def test_mixed_complex(golden, oracle_mod):
     xr = rng.standard_normal(100)
     np.testing.assert_allclose(oracle_mod.lfilter(b, a, xr), ss.lfilter(b, a, xr), rtol=1e-10,
                                atol=1e-10)
+
+
+def _ff_args(case):
+    padtype = str(case["padtype"])
+    padtype = None if padtype == "none" else padtype
+    padlen = int(case["padlen"])
+    return int(case["axis"]), padtype, (None if padlen < 0 else padlen)
+
+
+def test_oracle_zero_phase_goldens(golden, oracle_mod):
+    """sosfiltfilt / filtfilt restatements against scipy.signal's (the
+    reference tests' oracle, rtol = atol = 1e-10 as in
+    test__sosfiltfilt.py / test__filtfilt.py)."""
+    cases = golden_cases(golden, "ff")
+    assert len(cases) >= 20
+    for name, c in cases.items():
+        axis, padtype, padlen = _ff_args(c)
+        if "sos" in c:
+            y = oracle_mod.sosfiltfilt(c["sos"], c["x"], axis=axis, padtype=padtype,
+                                       padlen=padlen)
+        else:
+            y = oracle_mod.filtfilt(c["b"], c["a"], c["x"], axis=axis, padtype=padtype,
+                                    padlen=padlen)
+        np.testing.assert_allclose(y, c["y"], rtol=1e-10, atol=1e-10, err_msg=name)
+
+
+def test_oracle_zero_phase_properties(oracle_mod):
+    """Closed forms that pin the two-pass structure: the response is the
+    squared magnitude (a DC input passes with gain |H(1)|^2, a sinusoid far
+    from the edges with |H(w)|^2 and zero phase), and a mistake planted in
+    the second pass (no reversal) is caught."""
+    import scipy.signal as ss
+    sos = ss.butter(4, 0.25, output="sos")
+    _, h = ss.sosfreqz(sos, worN=[0.0, 0.05 * np.pi])
+    x = np.ones(400)
+    y = oracle_mod.sosfiltfilt(sos, x)
+    np.testing.assert_allclose(y[150:250], abs(h[0]) ** 2, rtol=1e-9)
+    n = np.arange(4000)
+    xs = np.cos(0.05 * np.pi * n)
+    ys = oracle_mod.sosfiltfilt(sos, xs)
+    np.testing.assert_allclose(ys[1000:3000], abs(h[1]) ** 2 * xs[1000:3000], atol=1e-6)
+    # planted: second pass run forward in time (no reversal) -> phase shift
+    zi = oracle_mod.sosfilt_zi(sos)
+    y1, _ = oracle_mod.sosfilt_rows(sos, xs[None], zi[None] * xs[0])
+    y2, _ = oracle_mod.sosfilt_rows(sos, y1, zi[None] * y1[0, 0])
+    assert np.max(np.abs(y2[0, 1000:3000] - abs(h[1]) ** 2 * xs[1000:3000])) > 1e-2
+    with pytest.raises(ValueError, match="padlen"):
+        oracle_mod.sosfiltfilt(sos, np.ones(15))
+    with pytest.raises(ValueError, match="Unknown padtype"):
+        oracle_mod.sosfiltfilt(sos, np.ones(200), padtype="wrap")

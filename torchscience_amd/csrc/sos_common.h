@@ -338,6 +338,50 @@ __device__ __forceinline__ void load_run(const TIO* p, bool vec, int m, TACC (&v
   }
 }
 
+// Reverse-order runs: v[j] = p[n_hi - j] for j < m (zero beyond m); with
+// `vec`, the W elements [n_hi - W + 1, n_hi] are read as 16-byte vectors
+// (n_hi - W + 1 must then be 16-byte aligned).
+template <typename TIO, typename TACC, int W>
+__device__ __forceinline__ void load_run_rev(const TIO* p, int64_t n_hi, bool vec, int m,
+                                             TACC (&v)[W]) {
+  using Tr = IOTraits<TIO>;
+  if (vec) {
+    const typename Tr::vec_t* q = reinterpret_cast<const typename Tr::vec_t*>(p + (n_hi - (W - 1)));
+#pragma unroll
+    for (int i = 0; i < W / Tr::kVec; ++i) {
+      const typename Tr::vec_t r = q[i];
+      const TIO* e = reinterpret_cast<const TIO*>(&r);
+#pragma unroll
+      for (int j = 0; j < Tr::kVec; ++j)
+        v[W - 1 - (i * Tr::kVec + j)] = static_cast<TACC>(Tr::to_f(e[j]));
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < W; ++j) v[j] = (j < m) ? static_cast<TACC>(Tr::to_f(p[n_hi - j])) : TACC(0);
+  }
+}
+
+template <typename TIO, typename TACC, int W>
+__device__ __forceinline__ void store_run_rev(TIO* p, int64_t n_hi, bool vec, int m,
+                                              const TACC (&v)[W]) {
+  using Tr = IOTraits<TIO>;
+  if (vec) {
+    typename Tr::vec_t* q = reinterpret_cast<typename Tr::vec_t*>(p + (n_hi - (W - 1)));
+#pragma unroll
+    for (int i = 0; i < W / Tr::kVec; ++i) {
+      typename Tr::vec_t r;
+      TIO* e = reinterpret_cast<TIO*>(&r);
+#pragma unroll
+      for (int j = 0; j < Tr::kVec; ++j) e[j] = Tr::from_f(v[W - 1 - (i * Tr::kVec + j)]);
+      q[i] = r;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+      if (j < m) p[n_hi - j] = Tr::from_f(v[j]);
+  }
+}
+
 // ---- LDS-staged window I/O --------------------------------------------------
 //
 // A wave's window is 64*W contiguous elements; lane q owns [q*W, (q+1)*W),

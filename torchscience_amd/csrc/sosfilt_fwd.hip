@@ -139,15 +139,26 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
 
   const TIO* xr = static_cast<const TIO*>(a.x) + (c % a.x_rows) * a.x_stride;
   TIO* yr = static_cast<TIO*>(a.y) + c * a.y_stride;
-  const bool xvec = !TAIL && (reinterpret_cast<uintptr_t>(xr) & 15u) == 0;
-  const bool yvec = !TAIL && (reinterpret_cast<uintptr_t>(yr) & 15u) == 0;
+  // REVERSE: processing index t maps to time T-1-t (input and output), so
+  // the wave's samples are the time-ordered run ending at T-1-t0w.
+  const bool rev = (a.flags & TSIIR_REVERSE) != 0;
+  const bool tvec = !rev || (a.T % IOTraits<TIO>::kVec) == 0;
+  const bool xvec = !TAIL && tvec && (reinterpret_cast<uintptr_t>(xr) & 15u) == 0;
+  const bool yvec = !TAIL && tvec && (reinterpret_cast<uintptr_t>(yr) & 15u) == 0;
+  const int64_t t_rev = a.T - t0w - wwin;  // first time of the wave's run (REVERSE)
   TIO* my_lds = lds + ((LDS && !TAIL) ? wv * LdsStage<TIO, W>::kElems : 0);
 
   TACC v[W];
-  if (LDS && !TAIL && xvec)
+  if (rev) {
+    if (LDS && !TAIL && xvec)
+      load_window_lds<TIO, TACC, W, true>(xr + t_rev, my_lds, v);
+    else
+      load_run_rev<TIO, TACC, W>(xr, a.T - 1 - t0, xvec, m, v);
+  } else if (LDS && !TAIL && xvec) {
     load_window_lds<TIO, TACC, W>(xr + t0w, my_lds, v);
-  else
+  } else {
     load_run<TIO, TACC, W>(xr + t0, xvec, m, v);
+  }
 
   const double* tf = tables + (c / a.cpf) * static_cast<int64_t>(a.S) * kTableStride;
   const bool has_zi = (a.flags & TSIIR_HAS_ZI) != 0;
@@ -374,10 +385,16 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
   }
   if (!TAIL) apply_pending<TACC, TR, W, kTabG>(v, pg, pw0, pw1, pa1, pa2);
 
-  if (LDS && !TAIL && yvec)
+  if (rev) {
+    if (LDS && !TAIL && yvec)
+      store_window_lds<TIO, TACC, W, true>(yr + t_rev, my_lds, v);
+    else
+      store_run_rev<TIO, TACC, W>(yr, a.T - 1 - t0, yvec, m, v);
+  } else if (LDS && !TAIL && yvec) {
     store_window_lds<TIO, TACC, W>(yr + t0w, my_lds, v);
-  else
+  } else {
     store_run<TIO, TACC, W>(yr + t0, yvec, m, v);
+  }
 }
 
 // ---- channel-persistent form ----------------------------------------------
@@ -758,7 +775,7 @@ static bool use_chan(const tsiir_sosfilt_desc* d, const void* saved) {
     return -1;
   }();
   if (saved != nullptr || d->n_sections > kWave) return false;
-  if (d->flags & TSIIR_PATH_WINDOW) return false;
+  if (d->flags & (TSIIR_PATH_WINDOW | TSIIR_REVERSE)) return false;
   if (d->flags & TSIIR_PATH_CHANNEL) return true;
   return forced == 1;
 }
@@ -779,7 +796,8 @@ tsiir_status forward_impl(const tsiir_sosfilt_desc* d, const void* x, void* y, c
     return TSIIR_EINVAL;
   if (d->coef_dtype != TSIIR_F32 && d->coef_dtype != TSIIR_F64) return TSIIR_EINVAL;
   if (d->mode != TSIIR_MODE_PLAIN && d->mode != TSIIR_MODE_COMPENSATED) return TSIIR_EINVAL;
-  if (d->flags & TSIIR_REVERSE) return TSIIR_EUNSUPPORTED;
+  if ((d->flags & TSIIR_REVERSE) && (saved != nullptr || (d->flags & TSIIR_PATH_CHANNEL)))
+    return TSIIR_EUNSUPPORTED;
   if ((d->flags & TSIIR_PATH_WINDOW) && (d->flags & TSIIR_PATH_CHANNEL)) return TSIIR_EINVAL;
   const bool has_zi = (d->flags & TSIIR_HAS_ZI) != 0;
   const bool write_zf = (d->flags & TSIIR_WRITE_ZF) != 0;
