@@ -61,8 +61,10 @@ def test_workspace_size_query_without_gpu():
                          y_row_stride=65536, chans_per_filter=4096)
     n = L.tsiir_sosfilt_workspace_bytes(ctypes.byref(d))
     # header + one 896-byte table entry per section (rounded to 256 B) +
-    # 64 B of status per (window, channel, section); windows of 4096 samples
-    assert n == 256 + 5376 + 16 * 4096 * 6 * 64
+    # 64 B of status per (window, channel, section); windows of 4096 samples;
+    # then (256-B aligned) one 1088-byte FP32-kernel entry per section
+    fp64_part = 256 + 5376 + 16 * 4096 * 6 * 64
+    assert n == (fp64_part + 255) // 256 * 256 + 6 * 1088
     d.n_sections = 0
     assert L.tsiir_sosfilt_workspace_bytes(ctypes.byref(d)) == 0
     assert L.tsiir_status_string(2) == b"unsupported request"

@@ -74,6 +74,43 @@ __device__ __forceinline__ const double* table_pow(const double* t, int i) {
   return t + 8 + 4 * i;
 }
 
+// Table entry of one (filter, section) for the FP32-recurrence forward kernel
+// (sosfilt_fwd32.hip), built next to the FP64 entry above.
+//  c     DF1 coefficients {b0, b1, b2, -a1, -a2} (normalised by a0).  When
+//        `unit` is set (per filter) every row is also divided by its b0, so
+//        the section computes y/b0 and the first input tap is 1; p is then
+//        the running product of the b0 of sections 0..s (the factor between
+//        the true and the computed output of section s) and rp = 1/p.
+//        Otherwise b0 stays in c[0] and p = rp = 1.
+//  mq    M^q for q < 64 in FP32, M = A^64 the DF2T state matrix over one
+//        lane's 64 samples, rows {a, b, c, d} of [[a, b], [c, d]].
+struct Fast32 {
+  float c[5];
+  int unit;
+  double p, rp;
+  float pad[6];
+  float4 mq[64];
+};
+static_assert(sizeof(Fast32) == 1088, "Fast32 layout");
+
+// Arguments of the forward window kernels.
+struct FwdArgs {
+  const void* x;
+  void* y;
+  const void* zi;
+  void* zf;
+  const double* tables;
+  uint64_t* status;
+  WsHeader* hdr;
+  int64_t B, T;
+  int64_t x_rows, x_stride, y_stride, cpf;
+  int64_t n_windows;
+  void* saved;  // optional (S-1, B, T) section outputs, for the backward
+  int S;
+  uint32_t flags;
+  int io_dtype;
+};
+
 // 64-bit status word = (flag << 32) | 32-bit payload.  A double is stored
 // as two words (low half, high half).  Each word is written and read with a
 // single 64-bit relaxed atomic at agent scope, so a reader that sees the
@@ -434,7 +471,8 @@ __device__ __forceinline__ void lds_rows_to_regs(const TIO* lds, int r, TACC (&v
   }
 }
 
-template <typename TIO, typename TACC, int W, bool REV = false>
+// NT: non-temporal global accesses (streamed data that is not re-read).
+template <typename TIO, typename TACC, int W, bool REV = false, bool NT = false>
 __device__ __forceinline__ void load_window_lds(const TIO* win, TIO* lds, TACC (&v)[W]) {
   using L = LdsStage<TIO, W>;
   const int lane = threadIdx.x & (kWave - 1);
@@ -443,7 +481,10 @@ __device__ __forceinline__ void load_window_lds(const TIO* win, TIO* lds, TACC (
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
 #pragma unroll
-    for (int i = 0; i < L::kH; ++i) lv[L::lvec(i, lane)] = g[L::gvec(r, i, lane)];
+    for (int i = 0; i < L::kH; ++i) {
+      const u32x4* src = g + L::gvec(r, i, lane);
+      lv[L::lvec(i, lane)] = NT ? __builtin_nontemporal_load(src) : *src;
+    }
     __syncthreads();
     lds_rows_to_regs<TIO, TACC, W, REV>(lds, r, v);
     __syncthreads();
@@ -481,7 +522,7 @@ __device__ __forceinline__ void transpose_window_lds(const u32x4 (&pf)[LdsStage<
   }
 }
 
-template <typename TIO, typename TACC, int W, bool REV = false>
+template <typename TIO, typename TACC, int W, bool REV = false, bool NT = false>
 __device__ __forceinline__ void store_window_lds(TIO* win, TIO* lds, const TACC (&v)[W]) {
   using L = LdsStage<TIO, W>;
   using vec_t = typename IOTraits<TIO>::vec_t;
@@ -504,7 +545,13 @@ __device__ __forceinline__ void store_window_lds(TIO* win, TIO* lds, const TACC 
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < L::kH; ++i) g[L::gvec(r, i, lane)] = lv[L::lvec(i, lane)];
+    for (int i = 0; i < L::kH; ++i) {
+      u32x4* dst = g + L::gvec(r, i, lane);
+      if (NT)
+        __builtin_nontemporal_store(lv[L::lvec(i, lane)], dst);
+      else
+        *dst = lv[L::lvec(i, lane)];
+    }
     __syncthreads();
   }
 }

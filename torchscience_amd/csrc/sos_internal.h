@@ -18,6 +18,7 @@ struct FwdGeometry {
   int64_t n_filters;
   size_t table_bytes;
   size_t status_bytes;
+  size_t fast_off;     // Fast32 tables of the FP32-recurrence kernel
   size_t total;        // workspace bytes
 };
 
@@ -35,5 +36,15 @@ void launch_prep(const void* sos, int coef_dtype, int64_t n_rows, int log2w, boo
 tsiir_status forward_impl(const tsiir_sosfilt_desc* d, const void* x, void* y, const void* sos,
                           const void* zi, void* zf, void* workspace, size_t workspace_bytes,
                           void* stream, void* saved);
+
+// FP32-recurrence forward (sosfilt_fwd32.hip): builds the Fast32 tables and
+// filters every full window (fwd32_window() samples) of every channel whose
+// filter is `unit` (chained windows).  The FP64 tables of a.tables must be
+// prepared and the epoch advanced first.
+struct FwdArgs;
+struct Fast32;
+int64_t fwd32_window();
+tsiir_status launch_fwd32_full(const FwdArgs& a, const void* sos, int coef_dtype,
+                               int64_t n_filters, Fast32* fast, hipStream_t stream);
 
 }  // namespace tsiir

@@ -42,26 +42,27 @@ struct FwdPolicy {
   int nw;
   bool chained;
 };
+static bool fwd_lds_io();
+// Full windows of FP32 recurrences (FP32 or BF16 I/O, plain mode) over many
+// channels run in the FP32-recurrence kernel of sosfilt_fwd32.hip (1-wave
+// chained windows), unless TSIIR_FWD32=0 selects this file's kernel (kept
+// for comparison measurements).
+static bool fwd32_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("TSIIR_FWD32");
+    return !(e && strcmp(e, "0") == 0);
+  }();
+  return on;
+}
+static inline bool use_fwd32(const tsiir_sosfilt_desc* d) {
+  return (d->io_dtype == TSIIR_F32 || d->io_dtype == TSIIR_BF16) &&
+         d->mode == TSIIR_MODE_PLAIN && d->channels >= 1024 && d->n_sections <= kWave &&
+         fwd32_enabled() && fwd_lds_io();
+}
 static inline FwdPolicy fwd_policy(const tsiir_sosfilt_desc* d) {
   if (d->channels >= 1024 && d->n_sections <= 64) return FwdPolicy{1, true};
   return FwdPolicy{4, false};
 }
-
-struct FwdArgs {
-  const void* x;
-  void* y;
-  const void* zi;
-  void* zf;
-  const double* tables;
-  uint64_t* status;
-  WsHeader* hdr;
-  int64_t B, T;
-  int64_t x_rows, x_stride, y_stride, cpf;
-  int64_t n_windows;
-  void* saved;  // optional (S-1, B, T) section outputs, for the backward
-  int S;
-  uint32_t flags;
-};
 
 // Coefficients of one section, in the compute type.
 template <typename TACC>
@@ -651,7 +652,10 @@ bool fwd_geometry(const tsiir_sosfilt_desc* d, FwdGeometry* g) {
   g->table_bytes = align_up(static_cast<size_t>(g->n_filters) * d->n_sections * kTableStride * 8, 256);
   g->status_bytes = static_cast<size_t>(g->n_windows) * d->channels * d->n_sections *
                     kStatusWords * 8;
-  g->total = sizeof(WsHeader) + g->table_bytes + g->status_bytes;
+  g->fast_off = align_up(sizeof(WsHeader) + g->table_bytes + g->status_bytes, 256);
+  const size_t fast_bytes =
+      use_fwd32(d) ? static_cast<size_t>(g->n_filters) * d->n_sections * sizeof(Fast32) : 0;
+  g->total = g->fast_off + fast_bytes;
   return true;
 }
 
@@ -693,7 +697,22 @@ static tsiir_status launch_fwd(const tsiir_sosfilt_desc* d, const FwdGeometry& g
   a.S = d->n_sections;
   a.flags = d->flags;
   a.saved = saved;
+  a.io_dtype = d->io_dtype;
   const FwdPolicy pol = fwd_policy(d);
+  if constexpr (std::is_same<TR, float>::value && W == 64) {
+    if (use_fwd32(d)) {
+      Fast32* fast = reinterpret_cast<Fast32*>(ws + g.fast_off);
+      const tsiir_status rc = launch_fwd32_full(a, sos, d->coef_dtype, g.n_filters, fast,
+                                                stream);
+      if (rc != TSIIR_OK) return rc;
+      const int64_t full32 = d->n_samples / fwd32_window();
+      if (full32 * fwd32_window() < d->n_samples)
+        hipLaunchKernelGGL((sos_fwd_kernel<TIO, TACC, TR, W, 1, false, true, false>),
+                           dim3(static_cast<unsigned>(d->channels)), dim3(kWave), 0, stream, a,
+                           full32, a.tables);
+      return hipGetLastError() == hipSuccess ? TSIIR_OK : TSIIR_ELAUNCH;
+    }
+  }
   const int64_t win = static_cast<int64_t>(kWave) * W * pol.nw;
   const int64_t full = d->n_samples / win;
   const int64_t groups = full * d->channels;

@@ -1,0 +1,512 @@
+// sosfilt_fwd32.hip - forward SOS cascade with FP32 recurrences (FP32 or
+// BF16 I/O) over the full 4096-sample windows of every channel, on gfx950.
+//
+// Replaces the section loop of sosfilt (_sosfilt.py:117-128) and the
+// per-sample DF2T loop of lfilter (_lfilter.py:153-166) it calls, for FP32
+// compute.  The reference's DF2T update (_lfilter.py:158-166) is used by the
+// FP64 kernel in sosfilt_fwd.hip; this kernel computes the same linear
+// recurrence in direct form I, which takes one operation fewer per sample.
+//
+// Work decomposition (as in sosfilt_fwd.hip): one wavefront = one window of
+// 64 lanes x 64 consecutive samples of one channel, held in registers for the
+// whole cascade; workgroup i = k * B + c (window-major).  Per section:
+//   1. zero-state sweep of the lane's samples, which first adds the pending
+//      zero-input response of the previous section (2 FMAs per sample, from
+//      the section's float table g) and then runs direct form I with the
+//      first input tap normalised to 1 (4 FMAs per sample):
+//        y[n] = u[n] + b1' u[n-1] + b2' u[n-2] - a1 y[n-1] - a2 y[n-2].
+//      With b' = b / b0 the section outputs y / b0; the factor is carried
+//      into the next section, so the whole cascade runs scaled by the
+//      running product p_s of the b0 (table entry Fast32::p), which is
+//      restored on output, on stored states and on saved section outputs.
+//      Filters with a zero b0 or a running product outside [2^-50, 2^50]
+//      are not `unit` and keep b0 as a tap (5 FMAs per sample, p = 1).
+//   2. FP32 scan over the lanes of the affine maps z -> M z + e_q (e_q the
+//      lane's zero-state end state in DF2T coordinates, M = A^64):
+//      Kogge-Stone inside each 16-lane row with DPP row shifts, then two
+//      DPP row broadcasts that carry the row totals across the wave.
+//   3. the window's incoming state, in FP64: zi, or the inclusive state of
+//      window k-1 (read once for all sections up front), composed with the
+//      window aggregate in FP64 and published for window k+1.
+//   4. each lane's incoming state M^q s_window + exclusive_q (M^q per lane
+//      from the table) becomes the pending zero-input response.
+// x is read from HBM once and y written once, through LDS with non-temporal
+// 16-byte accesses.  Partial last windows run in the FP64-carry kernel of
+// sosfilt_fwd.hip, which reads the inclusive states published here (true,
+// unscaled DF2T states).
+
+#include <math.h>
+
+#include "sos_common.h"
+#include "sos_internal.h"
+#include "tsiir.h"
+
+namespace tsiir {
+
+namespace {
+
+constexpr int kW32 = 64;                              // samples per lane
+constexpr int64_t kWin32 = static_cast<int64_t>(kWave) * kW32;  // samples per window
+
+// DPP controls (GFX9 encoding)
+constexpr int kDppRowShr = 0x110;      // + shift 1..15, within each row of 16 lanes
+constexpr int kDppWaveShr1 = 0x138;    // whole wave, one lane towards higher lanes
+constexpr int kDppRowBcast15 = 0x142;  // lane 15 of each row to the next row
+constexpr int kDppRowBcast31 = 0x143;  // lane 31 to rows 2 and 3
+
+// v of the DPP source lane; 0 where the source lane is out of range or the
+// destination row is masked off.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, true));
+}
+
+// e += P * (e of the DPP source lane), P = [[x, y], [z, w]]
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void scan_step(const float4 P, float& e0, float& e1) {
+  const float n0 = dpp_f32<CTRL, ROW_MASK>(e0);
+  const float n1 = dpp_f32<CTRL, ROW_MASK>(e1);
+  e0 = fmaf(P.x, n0, fmaf(P.y, n1, e0));
+  e1 = fmaf(P.z, n0, fmaf(P.w, n1, e1));
+}
+
+// Inclusive scan of the lanes' end states with zero incoming state,
+//   X_q = sum_{p <= q} M^(q-p) e_p,
+// in FP32.  Rows of 16 lanes: Kogge-Stone steps 1, 2, 4, 8 with the uniform
+// powers M^d.  Across rows: rows 1 and 3 add M^((q mod 16)+1) times the last
+// lane of the row before (pa), then rows 2 and 3 add M^(q-31) times lane 31
+// (pb), which by then holds the total of rows 0-1.  Returns the exclusive
+// value X_(q-1) (0 in lane 0) and the wave total X_63 (uniform).
+__device__ __forceinline__ void lane_scan32(const Fast32* __restrict__ f, const float4 pa,
+                                            const float4 pb, float e0, float e1, float& x0,
+                                            float& x1, float& ag0, float& ag1) {
+  scan_step<kDppRowShr + 1, 0xf>(f->mq[1], e0, e1);
+  scan_step<kDppRowShr + 2, 0xf>(f->mq[2], e0, e1);
+  scan_step<kDppRowShr + 4, 0xf>(f->mq[4], e0, e1);
+  scan_step<kDppRowShr + 8, 0xf>(f->mq[8], e0, e1);
+  scan_step<kDppRowBcast15, 0xa>(pa, e0, e1);
+  scan_step<kDppRowBcast31, 0xc>(pb, e0, e1);
+  ag0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e0), kWave - 1));
+  ag1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e1), kWave - 1));
+  x0 = dpp_f32<kDppWaveShr1, 0xf>(e0);
+  x1 = dpp_f32<kDppWaveShr1, 0xf>(e1);
+}
+
+// Zero-state sweep of one section over the lane's samples, in three passes
+// that each update the sample registers in place (two FMAs per sample each,
+// no register copies):
+//   correction (PEND), ascending: u[j] = v[j] + g[j+1] w0 + g[j] w1, the
+//     previous section's zero-input response;
+//   feed-forward, descending: t[j] = u[j] + b1' u[j-1] + b2' u[j-2] (UNIT;
+//     b0 u[j] + b1 u[j-1] + b2 u[j-2] otherwise), so u[j-1] and u[j-2] are
+//     still in place when t[j] forms;
+//   feedback, ascending: y[j] = t[j] - a2 y[j-2] - a1 y[j-1].
+// Returns the DF2T state after the last sample n,
+//   z0 = b1 u[n] + b2 u[n-1] - a1 y[n] - a2 y[n-1],  z1 = b2 u[n] - a2 y[n].
+template <bool UNIT, bool PEND>
+__device__ __forceinline__ void df1_sweep(float (&v)[kW32], const Fast32* __restrict__ f,
+                                          const float* __restrict__ g, float w0, float w1,
+                                          float& z0, float& z1) {
+  const float b0 = f->c[0], b1 = f->c[1], b2 = f->c[2], na1 = f->c[3], na2 = f->c[4];
+  if constexpr (PEND) {
+#pragma unroll
+    for (int j = 0; j < kW32; ++j) v[j] = fmaf(g[j + 1], w0, fmaf(g[j], w1, v[j]));
+  }
+  const float un = v[kW32 - 1], un1 = v[kW32 - 2];
+  // the passes stay apart, and the feed-forward pass stays descending:
+  // either reordering would need a second register per sample
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int j = kW32 - 1; j >= (UNIT ? 1 : 0); --j) {
+    float t = UNIT ? v[j] : b0 * v[j];
+    if (j >= 1) t = fmaf(b1, v[j - 1], t);
+    if (j >= 2) t = fmaf(b2, v[j - 2], t);
+    v[j] = t;
+    __builtin_amdgcn_sched_barrier(0);
+  }
+#pragma unroll
+  for (int j = 1; j < kW32; ++j) {
+    float t = v[j];
+    if (j >= 2) t = fmaf(na2, v[j - 2], t);
+    v[j] = fmaf(na1, v[j - 1], t);
+  }
+  z0 = fmaf(b1, un, fmaf(b2, un1, fmaf(na1, v[kW32 - 1], na2 * v[kW32 - 2])));
+  z1 = fmaf(b2, un, na2 * v[kW32 - 1]);
+}
+
+// v = scale * (v + g[j+1] w0 + g[j] w1); SCALE = false: scale == 1.
+template <bool SCALE>
+__device__ __forceinline__ void apply_corr(float (&v)[kW32], const float* __restrict__ g,
+                                           float w0, float w1, float scale) {
+  if constexpr (SCALE) {
+    w0 *= scale;
+    w1 *= scale;
+  }
+#pragma unroll
+  for (int j = 0; j < kW32; ++j) {
+    const float b = SCALE ? v[j] * scale : v[j];
+    v[j] = fmaf(g[j + 1], w0, fmaf(g[j], w1, b));
+  }
+}
+
+// Window store through LDS: lane q's samples are row q of the window (row
+// 63 - q, reversed, when REV), written with coalesced non-temporal 16-byte
+// stores; every value is multiplied by `scale` on the way.
+template <typename TIO, bool REV>
+__device__ __forceinline__ void store_window32(TIO* win, TIO* lds, const float (&v)[kW32],
+                                               float scale) {
+  using L = LdsStage<TIO, kW32>;
+  using vec_t = typename IOTraits<TIO>::vec_t;
+  const int lane = threadIdx.x & (kWave - 1);
+  u32x4* __restrict__ gp = reinterpret_cast<u32x4*>(win);
+  const u32x4* lv = reinterpret_cast<const u32x4*>(lds);
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    vec_t* row = reinterpret_cast<vec_t*>(lds) + (REV ? kWave - 1 - lane : lane) * L::kRowV;
+#pragma unroll
+    for (int t = 0; t < L::kH; ++t) {
+      vec_t rv;
+      TIO* e = reinterpret_cast<TIO*>(&rv);
+#pragma unroll
+      for (int j = 0; j < L::kVec; ++j) {
+        const int idx = (r * L::kH + t) * L::kVec + j;
+        e[j] = IOTraits<TIO>::from_f(v[REV ? kW32 - 1 - idx : idx] * scale);
+      }
+      row[t] = rv;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < L::kH; ++i)
+      __builtin_nontemporal_store(lv[L::lvec(i, lane)], gp + L::gvec(r, i, lane));
+    __syncthreads();
+  }
+}
+
+// Direct per-lane store of the lane's samples (rows not 16-byte aligned):
+// forward, p[j] = v[j]; REVERSE, p[-j] = v[j] (p = the lane's last time).
+template <typename TIO>
+__device__ __forceinline__ void store_lane32(TIO* p, bool rev, const float (&v)[kW32],
+                                             float scale) {
+#pragma unroll
+  for (int j = 0; j < kW32; ++j) p[rev ? -j : j] = IOTraits<TIO>::from_f(v[j] * scale);
+}
+
+// Per-window state shared by the section steps of one wave.
+struct Win32 {
+  int64_t k, c, t0w;
+  const Fast32* ff;   // the channel's filter, section 0
+  const double* tf;   // its FP64 table entries
+  int64_t zi_row;
+  bool has_zi, write_zf;
+  double pv0, pv1;    // chained: lane l holds window k-1's inclusive state of section l
+  int pok;
+};
+
+// Steps 2-4 of section s after its sweep: lane scan, window carry and
+// publication, and the lane's incoming state (pw0, pw1) in the scaled units
+// of the section, which becomes the pending zero-input response.
+// The lane's powers of M for section s: pm[0] = M^((q mod 16)+1) and
+// pm[1] = M^(max(q-31, 0)) for the scan, pm[2] = M^q for the incoming state.
+// Loaded one section ahead so their latency hides behind the sweep.
+struct LanePow {
+  float4 pm[3];
+};
+__device__ __forceinline__ LanePow lane_pow(const Fast32* __restrict__ fs) {
+  const int lane = threadIdx.x;
+  return LanePow{{fs->mq[(lane & 15) + 1], fs->mq[lane > 31 ? lane - 31 : 0], fs->mq[lane]}};
+}
+
+__device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32& w, int s,
+                                              const LanePow& lp, float z0, float z1, float& pw0,
+                                              float& pw1) {
+  const int lane = threadIdx.x;
+  const int S = a.S;
+  const uint32_t flag = launch_flag(a.hdr, 0);
+  const Fast32* fs = w.ff + s;
+  const double* tb = w.tf + static_cast<int64_t>(s) * kTableStride;
+  const float4 pa = lp.pm[0], pb = lp.pm[1], pq = lp.pm[2];
+
+  // lane scan (FP32, zero incoming state)
+  float x0, x1, ag0, ag1;
+  lane_scan32(fs, pa, pb, z0, z1, x0, x1, ag0, ag1);
+  const double P = fs->p;
+  const double A0 = P * static_cast<double>(ag0);  // window aggregate, true units
+  const double A1 = P * static_cast<double>(ag1);
+
+  // window incoming state (true units, FP64): zi, or the inclusive state
+  // window k-1 published (normally read up front, see window32)
+  double sw0 = 0.0, sw1 = 0.0;
+  if (w.k == 0) {
+    if (w.has_zi) {
+      const float* zp = static_cast<const float*>(a.zi) + (w.zi_row * S + s) * 2;
+      sw0 = static_cast<double>(zp[0]);
+      sw1 = static_cast<double>(zp[1]);
+    }
+  } else {
+    sw0 = readlane_d(w.pv0, s);
+    sw1 = readlane_d(w.pv1, s);
+    if (!__builtin_amdgcn_readlane(w.pok, s)) {
+      const uint64_t* sp = a.status + (((w.k - 1) * a.B + w.c) * S + s) * kStatusWords;
+      uint32_t spins = 0;
+      while (!read_pair(sp + 4, flag, sw0, sw1)) {
+        if (++spins > kSpinLimit) {
+          if (lane == 0) atomicOr(&a.hdr->error, 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+  }
+  const Mat2 Mw = load_mat(table_pow(tb, 6));  // A^4096, one window
+  const double i0 = fma(Mw.a, sw0, fma(Mw.b, sw1, A0));
+  const double i1 = fma(Mw.c, sw0, fma(Mw.d, sw1, A1));
+  uint64_t* st = a.status + ((w.k * a.B + w.c) * S + s) * kStatusWords;
+  publish_pair(st + 4, i0, i1, flag, lane);
+  if (w.write_zf && lane == 0) {
+    float* zp = static_cast<float*>(a.zf) + (w.c * S + s) * 2;
+    zp[0] = static_cast<float>(i0);
+    zp[1] = static_cast<float>(i1);
+  }
+
+  // the lane's incoming state, scaled units
+  const double rp = fs->rp;
+  const float sws0 = static_cast<float>(sw0 * rp), sws1 = static_cast<float>(sw1 * rp);
+  pw0 = fmaf(pq.x, sws0, fmaf(pq.y, sws1, x0));
+  pw1 = fmaf(pq.z, sws0, fmaf(pq.w, sws1, x1));
+}
+
+// Section s's exact output in true units, for the backward pass: applies the
+// pending correction now (which clears it) and stores p_s * v.
+template <typename TIO>
+__device__ __forceinline__ void keep_section(const FwdArgs& a, const Win32& w, int s,
+                                             float (&v)[kW32], float& pw0, float& pw1) {
+  const double* tb = w.tf + static_cast<int64_t>(s) * kTableStride;
+  apply_corr<false>(v, table_g(tb), pw0, pw1, 1.0f);
+  pw0 = 0.0f;
+  pw1 = 0.0f;
+  TIO* sr = static_cast<TIO*>(a.saved) + (static_cast<int64_t>(s) * a.B + w.c) * a.T +
+            w.t0w + static_cast<int64_t>(threadIdx.x) * kW32;
+  // per-lane 16-byte stores straight from the sample registers (a staged
+  // store would hold a second copy of the window in registers)
+  const float scale = static_cast<float>(w.ff[s].p);
+  using Tr = IOTraits<TIO>;
+  if ((reinterpret_cast<uintptr_t>(sr) & 15u) == 0) {
+    typename Tr::vec_t* q = reinterpret_cast<typename Tr::vec_t*>(sr);
+#pragma unroll
+    for (int i = 0; i < kW32 / Tr::kVec; ++i) {
+      typename Tr::vec_t r;
+      TIO* e = reinterpret_cast<TIO*>(&r);
+#pragma unroll
+      for (int j = 0; j < Tr::kVec; ++j) e[j] = Tr::from_f(v[i * Tr::kVec + j] * scale);
+      q[i] = r;
+    }
+  } else {
+    store_lane32<TIO>(sr, false, v, scale);
+  }
+}
+
+// All sections of the window (section 0 has no pending correction, the
+// others take the previous section's); on return v holds the cascade output
+// in true units.  SAVED: the outputs of sections 0..S-2 are also stored.
+template <typename TIO, bool SAVED, bool UNIT>
+__device__ __forceinline__ void run_cascade(const FwdArgs& a, const Win32& w, float (&v)[kW32]) {
+  const int S = a.S;
+  float pw0 = 0.0f, pw1 = 0.0f, z0, z1;
+  LanePow lp = lane_pow(w.ff);
+  df1_sweep<UNIT, false>(v, w.ff, table_g(w.tf), 0.0f, 0.0f, z0, z1);
+  section_carry(a, w, 0, lp, z0, z1, pw0, pw1);
+  if (SAVED && S > 1) keep_section<TIO>(a, w, 0, v, pw0, pw1);
+  for (int s = 1; s < S; ++s) {
+    lp = lane_pow(w.ff + s);
+    const float* pg = table_g(w.tf + static_cast<int64_t>(s - 1) * kTableStride);
+    df1_sweep<UNIT, true>(v, w.ff + s, pg, pw0, pw1, z0, z1);
+    section_carry(a, w, s, lp, z0, z1, pw0, pw1);
+    if (SAVED && s + 1 < S) keep_section<TIO>(a, w, s, v, pw0, pw1);
+  }
+  const float* pg = table_g(w.tf + static_cast<int64_t>(S - 1) * kTableStride);
+  if constexpr (UNIT)
+    apply_corr<true>(v, pg, pw0, pw1, static_cast<float>(w.ff[S - 1].p));
+  else
+    apply_corr<false>(v, pg, pw0, pw1, 1.0f);
+}
+
+// One full window of a channel: load, cascade, store.  Windows of a channel
+// are chained: window k waits for the inclusive states window k-1 publishes,
+// which it reads for all sections up front (workgroup i = k * B + c, so
+// window k-1 was dispatched B workgroups earlier and has normally finished).
+template <typename TIO, bool REV, bool SAVED, bool UNIT>
+__device__ __forceinline__ void window32(const FwdArgs& a, Win32& w, TIO* lds) {
+  const int lane = threadIdx.x;
+  const int S = a.S;
+  w.pv0 = 0.0;
+  w.pv1 = 0.0;
+  w.pok = 1;
+  if (w.k > 0 && lane < S) {
+    const uint64_t* st = a.status + (((w.k - 1) * a.B + w.c) * S + lane) * kStatusWords;
+    w.pok = read_pair(st + 4, launch_flag(a.hdr, 0), w.pv0, w.pv1) ? 1 : 0;
+  }
+
+  const TIO* xr = static_cast<const TIO*>(a.x) + (w.c % a.x_rows) * a.x_stride;
+  TIO* yr = static_cast<TIO*>(a.y) + w.c * a.y_stride;
+  // REV: processing index t is time T-1-t, so the window is the time-ordered
+  // run [T - t0w - 4096, T - t0w)
+  const bool tvec = !REV || (a.T % IOTraits<TIO>::kVec) == 0;
+  const bool xvec = tvec && (reinterpret_cast<uintptr_t>(xr) & 15u) == 0;
+  const bool yvec = tvec && (reinterpret_cast<uintptr_t>(yr) & 15u) == 0;
+  const int64_t t0 = w.t0w + static_cast<int64_t>(lane) * kW32;
+
+  float v[kW32];
+  if (REV) {
+    if (xvec)
+      load_window_lds<TIO, float, kW32, true, true>(xr + (a.T - w.t0w - kWin32), lds, v);
+    else
+      load_run_rev<TIO, float, kW32>(xr, a.T - 1 - t0, false, kW32, v);
+  } else if (xvec) {
+    load_window_lds<TIO, float, kW32, false, true>(xr + w.t0w, lds, v);
+  } else {
+    load_run<TIO, float, kW32>(xr + t0, false, kW32, v);
+  }
+  run_cascade<TIO, SAVED, UNIT>(a, w, v);
+  if (REV) {
+    if (yvec)
+      store_window32<TIO, true>(yr + (a.T - w.t0w - kWin32), lds, v, 1.0f);
+    else
+      store_lane32<TIO>(yr + (a.T - 1 - t0), true, v, 1.0f);
+  } else if (yvec) {
+    store_window32<TIO, false>(yr + w.t0w, lds, v, 1.0f);
+  } else {
+    store_lane32<TIO>(yr + t0, false, v, 1.0f);
+  }
+}
+
+// Full windows of every channel.  The two forms (first input tap 1 with the
+// running b0 product, or b0 kept as a tap) branch before anything is loaded,
+// so no window data is live across the branch.  REV: along decreasing time
+// (TSIIR_REVERSE); SAVED: keep the section outputs (forward_saved).
+template <typename TIO, bool REV, bool SAVED>
+__global__ __launch_bounds__(64) void sos_fwd32_kernel(FwdArgs a, const Fast32* __restrict__ fast,
+                                                       const double* __restrict__ tables) {
+  __shared__ __align__(16) TIO lds[LdsStage<TIO, kW32>::kElems];
+  const int S = a.S;
+  Win32 w;
+  w.k = static_cast<int64_t>(blockIdx.x) / a.B;
+  w.c = static_cast<int64_t>(blockIdx.x) % a.B;
+  const int64_t fi = w.c / a.cpf;
+  w.ff = fast + fi * S;
+  w.tf = tables + fi * static_cast<int64_t>(S) * kTableStride;
+  w.t0w = w.k * kWin32;
+  w.has_zi = (a.flags & TSIIR_HAS_ZI) != 0;
+  w.zi_row = (a.flags & TSIIR_ZI_BROADCAST) ? 0 : w.c;
+  w.write_zf = (a.flags & TSIIR_WRITE_ZF) != 0 && w.t0w + kWin32 == a.T;
+  if (w.ff->unit != 0)
+    window32<TIO, REV, SAVED, true>(a, w, lds);
+  else
+    window32<TIO, REV, SAVED, false>(a, w, lds);
+}
+
+// One thread per (filter, section) row: the Fast32 entry (see sos_common.h).
+// Coefficients are normalised by a0 in the coefficient type, exactly as the
+// FP64 table prep does, then promoted.
+template <typename TC>
+__global__ __launch_bounds__(256) void sos_prep32_kernel(const TC* __restrict__ sos,
+                                                         int64_t n_rows, int S,
+                                                         Fast32* __restrict__ fast) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n_rows) return;
+  const int64_t f = i / S;
+  const int s = static_cast<int>(i % S);
+  bool unit = true;
+  double run = 1.0, ps = 1.0;
+  for (int r = 0; r < S; ++r) {
+    const TC* row = sos + (f * S + r) * 6;
+    TC b0 = row[0];
+    const TC a0 = row[3];
+    if (a0 != TC(1)) b0 = b0 / a0;
+    run *= static_cast<double>(b0);
+    const double m = fabs(run);
+    if (!(m >= 0x1p-50 && m <= 0x1p50)) unit = false;  // also zero, inf, NaN
+    if (r == s) ps = run;
+  }
+  const TC* row = sos + i * 6;
+  TC b0 = row[0], b1 = row[1], b2 = row[2], a0 = row[3], a1 = row[4], a2 = row[5];
+  if (a0 != TC(1)) {
+    b0 = b0 / a0;
+    b1 = b1 / a0;
+    b2 = b2 / a0;
+    a1 = a1 / a0;
+    a2 = a2 / a0;
+  }
+  const double db0 = b0, db1 = b1, db2 = b2, da1 = a1, da2 = a2;
+  Fast32* o = fast + i;
+  if (unit) {
+    o->c[0] = 1.0f;
+    o->c[1] = static_cast<float>(db1 / db0);
+    o->c[2] = static_cast<float>(db2 / db0);
+    o->p = ps;
+    o->rp = 1.0 / ps;
+  } else {
+    o->c[0] = static_cast<float>(db0);
+    o->c[1] = static_cast<float>(db1);
+    o->c[2] = static_cast<float>(db2);
+    o->p = 1.0;
+    o->rp = 1.0;
+  }
+  o->c[3] = static_cast<float>(-da1);
+  o->c[4] = static_cast<float>(-da2);
+  o->unit = unit ? 1 : 0;
+  for (int p = 0; p < 6; ++p) o->pad[p] = 0.0f;
+  Mat2 M{-da1, 1.0, -da2, 0.0};
+  for (int l = 0; l < 6; ++l) M = mat_mul(M, M);  // A^64
+  Mat2 Q{1.0, 0.0, 0.0, 1.0};
+  for (int q = 0; q < 64; ++q) {
+    o->mq[q] = make_float4(static_cast<float>(Q.a), static_cast<float>(Q.b),
+                           static_cast<float>(Q.c), static_cast<float>(Q.d));
+    Q = mat_mul(Q, M);
+  }
+}
+
+}  // namespace
+
+int64_t fwd32_window() { return kWin32; }
+
+template <typename TIO, bool REV>
+static void launch_fwd32_io(const FwdArgs& a, const Fast32* fast, dim3 grid, hipStream_t stream) {
+  if (a.saved != nullptr)
+    hipLaunchKernelGGL((sos_fwd32_kernel<TIO, REV, true>), grid, dim3(kWave), 0, stream, a, fast,
+                       a.tables);
+  else
+    hipLaunchKernelGGL((sos_fwd32_kernel<TIO, REV, false>), grid, dim3(kWave), 0, stream, a,
+                       fast, a.tables);
+}
+
+tsiir_status launch_fwd32_full(const FwdArgs& a, const void* sos, int coef_dtype,
+                               int64_t n_filters, Fast32* fast, hipStream_t stream) {
+  const int64_t n_rows = n_filters * a.S;
+  const unsigned pblocks = static_cast<unsigned>((n_rows + 255) / 256);
+  if (coef_dtype == TSIIR_F64)
+    hipLaunchKernelGGL(sos_prep32_kernel<double>, dim3(pblocks), dim3(256), 0, stream,
+                       static_cast<const double*>(sos), n_rows, a.S, fast);
+  else
+    hipLaunchKernelGGL(sos_prep32_kernel<float>, dim3(pblocks), dim3(256), 0, stream,
+                       static_cast<const float*>(sos), n_rows, a.S, fast);
+  const int64_t groups = (a.T / kWin32) * a.B;
+  if (groups > 0) {
+    const dim3 grid(static_cast<unsigned>(groups));
+    const bool rev = (a.flags & TSIIR_REVERSE) != 0;
+    if (a.io_dtype == TSIIR_BF16) {
+      if (rev)
+        launch_fwd32_io<bf16_t, true>(a, fast, grid, stream);
+      else
+        launch_fwd32_io<bf16_t, false>(a, fast, grid, stream);
+    } else {
+      if (rev)
+        launch_fwd32_io<float, true>(a, fast, grid, stream);
+      else
+        launch_fwd32_io<float, false>(a, fast, grid, stream);
+    }
+  }
+  return hipGetLastError() == hipSuccess ? TSIIR_OK : TSIIR_ELAUNCH;
+}
+
+}  // namespace tsiir
