@@ -31,7 +31,8 @@
 //   4. each lane's incoming state M^q s_window + exclusive_q (M^q per lane
 //      from the table) becomes the pending zero-input response.
 // x is read from HBM once and y written once, through LDS with non-temporal
-// 16-byte accesses.  Partial last windows run in the FP64-carry kernel of
+// 16-byte accesses (an unpadded, XOR-swizzled 8 KB staging buffer: with 95
+// VGPRs, 5 waves fit per SIMD).  Partial last windows run in the FP64-carry kernel of
 // sosfilt_fwd.hip, which reads the inclusive states published here (true,
 // unscaled DF2T states).
 
@@ -150,20 +151,71 @@ __device__ __forceinline__ void apply_corr(float (&v)[kW32], const float* __rest
   }
 }
 
-// Window store through LDS: lane q's samples are row q of the window (row
-// 63 - q, reversed, when REV), written with coalesced non-temporal 16-byte
-// stores; every value is multiplied by `scale` on the way.
+// ---- window I/O through LDS ------------------------------------------------
+//
+// A window is 64 rows (lanes) of kNV 16-byte vectors.  It moves in two
+// rounds of half-rows (kH vectors per lane per round).  Global side:
+// instruction i, lane l handles row q = i * (64 / kH) + l / kH, vector
+// t = l % kH of the half-row, so every instruction covers whole 128-byte
+// lines.  LDS side: half-row q at q * kH vectors, vector t stored at slot
+// t ^ (q mod kH), so a column read by 8 consecutive lanes hits 8 distinct
+// slots (no bank conflicts) without padding: the buffer is 64 * kH vectors
+// (8 KB for FP32 samples).
+template <typename TIO>
+struct Stage32 {
+  static constexpr int kVec = IOTraits<TIO>::kVec;  // elements per 16 bytes
+  static constexpr int kNV = kW32 / kVec;           // vectors per lane
+  static constexpr int kH = kNV / 2;                // vectors per lane per round
+  static constexpr int kBytes = kWave * kH * 16;
+  static_assert(kWave % kH == 0 && (kH & (kH - 1)) == 0, "window shape");
+  __device__ static __forceinline__ int grow(int i, int l) { return i * (kWave / kH) + l / kH; }
+  __device__ static __forceinline__ int gvec(int r, int i, int l) {
+    return grow(i, l) * kNV + r * kH + l % kH;
+  }
+  __device__ static __forceinline__ int slot(int q, int t) { return q * kH + (t ^ (q & (kH - 1))); }
+};
+
+// Window -> registers: lane q gets row q (REV: row 63 - q, elements reversed).
 template <typename TIO, bool REV>
-__device__ __forceinline__ void store_window32(TIO* win, TIO* lds, const float (&v)[kW32],
+__device__ __forceinline__ void load_window32(const TIO* win, u32x4* lds, float (&v)[kW32]) {
+  using L = Stage32<TIO>;
+  using vec_t = typename IOTraits<TIO>::vec_t;
+  const int lane = threadIdx.x & (kWave - 1);
+  const u32x4* __restrict__ gp = reinterpret_cast<const u32x4*>(win);
+  const int q = REV ? kWave - 1 - lane : lane;
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+#pragma unroll
+    for (int i = 0; i < L::kH; ++i)
+      lds[L::slot(L::grow(i, lane), lane % L::kH)] =
+          __builtin_nontemporal_load(gp + L::gvec(r, i, lane));
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < L::kH; ++t) {
+      const u32x4 raw = lds[L::slot(q, t)];
+      const vec_t rv = *reinterpret_cast<const vec_t*>(&raw);
+      const TIO* e = reinterpret_cast<const TIO*>(&rv);
+#pragma unroll
+      for (int j = 0; j < L::kVec; ++j) {
+        const int idx = (r * L::kH + t) * L::kVec + j;
+        v[REV ? kW32 - 1 - idx : idx] = IOTraits<TIO>::to_f(e[j]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Registers -> window, each value multiplied by `scale` (non-temporal stores).
+template <typename TIO, bool REV>
+__device__ __forceinline__ void store_window32(TIO* win, u32x4* lds, const float (&v)[kW32],
                                                float scale) {
-  using L = LdsStage<TIO, kW32>;
+  using L = Stage32<TIO>;
   using vec_t = typename IOTraits<TIO>::vec_t;
   const int lane = threadIdx.x & (kWave - 1);
   u32x4* __restrict__ gp = reinterpret_cast<u32x4*>(win);
-  const u32x4* lv = reinterpret_cast<const u32x4*>(lds);
+  const int q = REV ? kWave - 1 - lane : lane;
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
-    vec_t* row = reinterpret_cast<vec_t*>(lds) + (REV ? kWave - 1 - lane : lane) * L::kRowV;
 #pragma unroll
     for (int t = 0; t < L::kH; ++t) {
       vec_t rv;
@@ -173,12 +225,13 @@ __device__ __forceinline__ void store_window32(TIO* win, TIO* lds, const float (
         const int idx = (r * L::kH + t) * L::kVec + j;
         e[j] = IOTraits<TIO>::from_f(v[REV ? kW32 - 1 - idx : idx] * scale);
       }
-      row[t] = rv;
+      lds[L::slot(q, t)] = *reinterpret_cast<const u32x4*>(&rv);
     }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < L::kH; ++i)
-      __builtin_nontemporal_store(lv[L::lvec(i, lane)], gp + L::gvec(r, i, lane));
+      __builtin_nontemporal_store(lds[L::slot(L::grow(i, lane), lane % L::kH)],
+                                  gp + L::gvec(r, i, lane));
     __syncthreads();
   }
 }
@@ -192,20 +245,23 @@ __device__ __forceinline__ void store_lane32(TIO* p, bool rev, const float (&v)[
   for (int j = 0; j < kW32; ++j) p[rev ? -j : j] = IOTraits<TIO>::from_f(v[j] * scale);
 }
 
-// Per-window state shared by the section steps of one wave.
+// Per-wave state shared by the section steps.  A workgroup of NWV waves
+// filters NWV consecutive windows of one channel (a group); wave v takes
+// window k = kg + v.
+template <int NWV>
 struct Win32 {
-  int64_t k, c, t0w;
-  const Fast32* ff;   // the channel's filter, section 0
-  const double* tf;   // its FP64 table entries
+  int64_t kg, c, t0w;  // first window of the group, channel, first sample of this wave
+  int wv;              // wave within the group
+  const Fast32* ff;    // the channel's filter, section 0
+  const double* tf;    // its FP64 table entries
   int64_t zi_row;
   bool has_zi, write_zf;
-  double pv0, pv1;    // chained: lane l holds window k-1's inclusive state of section l
-  int pok;
+  // window kg-1's inclusive state of every section, read up front
+  const double* prev;  // (S, 2)
+  const int* prev_ok;  // (S): all words carried the current flag
+  double* wagg;        // (2, NWV, 2): wave aggregates, double-buffered by section
 };
 
-// Steps 2-4 of section s after its sweep: lane scan, window carry and
-// publication, and the lane's incoming state (pw0, pw1) in the scaled units
-// of the section, which becomes the pending zero-input response.
 // The lane's powers of M for section s: pm[0] = M^((q mod 16)+1) and
 // pm[1] = M^(max(q-31, 0)) for the scan, pm[2] = M^q for the incoming state.
 // Loaded one section ahead so their latency hides behind the sweep.
@@ -213,14 +269,20 @@ struct LanePow {
   float4 pm[3];
 };
 __device__ __forceinline__ LanePow lane_pow(const Fast32* __restrict__ fs) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
   return LanePow{{fs->mq[(lane & 15) + 1], fs->mq[lane > 31 ? lane - 31 : 0], fs->mq[lane]}};
 }
 
-__device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32& w, int s,
+// Steps 2-4 of section s after its sweep: lane scan, composition of the
+// wave aggregates of the group with the group's incoming state (FP64),
+// publication of the group's inclusive state, and the lane's incoming state
+// (pw0, pw1) in the scaled units of the section, which becomes the pending
+// zero-input response.
+template <int NWV>
+__device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32<NWV>& w, int s,
                                               const LanePow& lp, float z0, float z1, float& pw0,
                                               float& pw1) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
   const int S = a.S;
   const uint32_t flag = launch_flag(a.hdr, 0);
   const Fast32* fs = w.ff + s;
@@ -233,21 +295,29 @@ __device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32& w, 
   const double P = fs->p;
   const double A0 = P * static_cast<double>(ag0);  // window aggregate, true units
   const double A1 = P * static_cast<double>(ag1);
+  double* wagg = w.wagg + (s & 1) * NWV * 2;
+  if (NWV > 1) {
+    if (lane == 0) {
+      wagg[2 * w.wv] = A0;
+      wagg[2 * w.wv + 1] = A1;
+    }
+    __syncthreads();
+  }
 
-  // window incoming state (true units, FP64): zi, or the inclusive state
-  // window k-1 published (normally read up front, see window32)
+  // the group's incoming state (true units, FP64): zi, or the inclusive
+  // state window kg-1 published
   double sw0 = 0.0, sw1 = 0.0;
-  if (w.k == 0) {
+  if (w.kg == 0) {
     if (w.has_zi) {
       const float* zp = static_cast<const float*>(a.zi) + (w.zi_row * S + s) * 2;
       sw0 = static_cast<double>(zp[0]);
       sw1 = static_cast<double>(zp[1]);
     }
   } else {
-    sw0 = readlane_d(w.pv0, s);
-    sw1 = readlane_d(w.pv1, s);
-    if (!__builtin_amdgcn_readlane(w.pok, s)) {
-      const uint64_t* sp = a.status + (((w.k - 1) * a.B + w.c) * S + s) * kStatusWords;
+    sw0 = w.prev[2 * s];
+    sw1 = w.prev[2 * s + 1];
+    if (!w.prev_ok[s]) {
+      const uint64_t* sp = a.status + (((w.kg - 1) * a.B + w.c) * S + s) * kStatusWords;
       uint32_t spins = 0;
       while (!read_pair(sp + 4, flag, sw0, sw1)) {
         if (++spins > kSpinLimit) {
@@ -258,15 +328,29 @@ __device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32& w, 
       }
     }
   }
+  // this wave's incoming state: the group's, carried over the windows of
+  // the waves before it
   const Mat2 Mw = load_mat(table_pow(tb, 6));  // A^4096, one window
-  const double i0 = fma(Mw.a, sw0, fma(Mw.b, sw1, A0));
-  const double i1 = fma(Mw.c, sw0, fma(Mw.d, sw1, A1));
-  uint64_t* st = a.status + ((w.k * a.B + w.c) * S + s) * kStatusWords;
-  publish_pair(st + 4, i0, i1, flag, lane);
-  if (w.write_zf && lane == 0) {
-    float* zp = static_cast<float*>(a.zf) + (w.c * S + s) * 2;
-    zp[0] = static_cast<float>(i0);
-    zp[1] = static_cast<float>(i1);
+#pragma unroll
+  for (int j = 0; j < NWV - 1; ++j) {
+    if (j < w.wv) {
+      const double n0 = fma(Mw.a, sw0, fma(Mw.b, sw1, wagg[2 * j]));
+      const double n1 = fma(Mw.c, sw0, fma(Mw.d, sw1, wagg[2 * j + 1]));
+      sw0 = n0;
+      sw1 = n1;
+    }
+  }
+  if (w.wv == NWV - 1) {
+    // the last wave publishes the group's inclusive state (window kg+NWV-1)
+    const double i0 = fma(Mw.a, sw0, fma(Mw.b, sw1, A0));
+    const double i1 = fma(Mw.c, sw0, fma(Mw.d, sw1, A1));
+    uint64_t* st = a.status + (((w.kg + NWV - 1) * a.B + w.c) * S + s) * kStatusWords;
+    publish_pair(st + 4, i0, i1, flag, lane);
+    if (w.write_zf && lane == 0) {
+      float* zp = static_cast<float*>(a.zf) + (w.c * S + s) * 2;
+      zp[0] = static_cast<float>(i0);
+      zp[1] = static_cast<float>(i1);
+    }
   }
 
   // the lane's incoming state, scaled units
@@ -278,15 +362,15 @@ __device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32& w, 
 
 // Section s's exact output in true units, for the backward pass: applies the
 // pending correction now (which clears it) and stores p_s * v.
-template <typename TIO>
-__device__ __forceinline__ void keep_section(const FwdArgs& a, const Win32& w, int s,
+template <typename TIO, int NWV>
+__device__ __forceinline__ void keep_section(const FwdArgs& a, const Win32<NWV>& w, int s,
                                              float (&v)[kW32], float& pw0, float& pw1) {
   const double* tb = w.tf + static_cast<int64_t>(s) * kTableStride;
   apply_corr<false>(v, table_g(tb), pw0, pw1, 1.0f);
   pw0 = 0.0f;
   pw1 = 0.0f;
   TIO* sr = static_cast<TIO*>(a.saved) + (static_cast<int64_t>(s) * a.B + w.c) * a.T +
-            w.t0w + static_cast<int64_t>(threadIdx.x) * kW32;
+            w.t0w + static_cast<int64_t>(threadIdx.x & (kWave - 1)) * kW32;
   // per-lane 16-byte stores straight from the sample registers (a staged
   // store would hold a second copy of the window in registers)
   const float scale = static_cast<float>(w.ff[s].p);
@@ -309,20 +393,21 @@ __device__ __forceinline__ void keep_section(const FwdArgs& a, const Win32& w, i
 // All sections of the window (section 0 has no pending correction, the
 // others take the previous section's); on return v holds the cascade output
 // in true units.  SAVED: the outputs of sections 0..S-2 are also stored.
-template <typename TIO, bool SAVED, bool UNIT>
-__device__ __forceinline__ void run_cascade(const FwdArgs& a, const Win32& w, float (&v)[kW32]) {
+template <typename TIO, bool SAVED, bool UNIT, int NWV>
+__device__ __forceinline__ void run_cascade(const FwdArgs& a, const Win32<NWV>& w,
+                                            float (&v)[kW32]) {
   const int S = a.S;
   float pw0 = 0.0f, pw1 = 0.0f, z0, z1;
   LanePow lp = lane_pow(w.ff);
   df1_sweep<UNIT, false>(v, w.ff, table_g(w.tf), 0.0f, 0.0f, z0, z1);
-  section_carry(a, w, 0, lp, z0, z1, pw0, pw1);
-  if (SAVED && S > 1) keep_section<TIO>(a, w, 0, v, pw0, pw1);
+  section_carry<NWV>(a, w, 0, lp, z0, z1, pw0, pw1);
+  if (SAVED && S > 1) keep_section<TIO, NWV>(a, w, 0, v, pw0, pw1);
   for (int s = 1; s < S; ++s) {
     lp = lane_pow(w.ff + s);
     const float* pg = table_g(w.tf + static_cast<int64_t>(s - 1) * kTableStride);
     df1_sweep<UNIT, true>(v, w.ff + s, pg, pw0, pw1, z0, z1);
-    section_carry(a, w, s, lp, z0, z1, pw0, pw1);
-    if (SAVED && s + 1 < S) keep_section<TIO>(a, w, s, v, pw0, pw1);
+    section_carry<NWV>(a, w, s, lp, z0, z1, pw0, pw1);
+    if (SAVED && s + 1 < S) keep_section<TIO, NWV>(a, w, s, v, pw0, pw1);
   }
   const float* pg = table_g(w.tf + static_cast<int64_t>(S - 1) * kTableStride);
   if constexpr (UNIT)
@@ -331,22 +416,10 @@ __device__ __forceinline__ void run_cascade(const FwdArgs& a, const Win32& w, fl
     apply_corr<false>(v, pg, pw0, pw1, 1.0f);
 }
 
-// One full window of a channel: load, cascade, store.  Windows of a channel
-// are chained: window k waits for the inclusive states window k-1 publishes,
-// which it reads for all sections up front (workgroup i = k * B + c, so
-// window k-1 was dispatched B workgroups earlier and has normally finished).
-template <typename TIO, bool REV, bool SAVED, bool UNIT>
-__device__ __forceinline__ void window32(const FwdArgs& a, Win32& w, TIO* lds) {
-  const int lane = threadIdx.x;
-  const int S = a.S;
-  w.pv0 = 0.0;
-  w.pv1 = 0.0;
-  w.pok = 1;
-  if (w.k > 0 && lane < S) {
-    const uint64_t* st = a.status + (((w.k - 1) * a.B + w.c) * S + lane) * kStatusWords;
-    w.pok = read_pair(st + 4, launch_flag(a.hdr, 0), w.pv0, w.pv1) ? 1 : 0;
-  }
-
+// One window per wave: load, cascade, store.
+template <typename TIO, bool REV, bool SAVED, bool UNIT, int NWV>
+__device__ __forceinline__ void window32(const FwdArgs& a, Win32<NWV>& w, u32x4* stage) {
+  const int lane = threadIdx.x & (kWave - 1);
   const TIO* xr = static_cast<const TIO*>(a.x) + (w.c % a.x_rows) * a.x_stride;
   TIO* yr = static_cast<TIO*>(a.y) + w.c * a.y_stride;
   // REV: processing index t is time T-1-t, so the window is the time-ordered
@@ -359,50 +432,81 @@ __device__ __forceinline__ void window32(const FwdArgs& a, Win32& w, TIO* lds) {
   float v[kW32];
   if (REV) {
     if (xvec)
-      load_window_lds<TIO, float, kW32, true, true>(xr + (a.T - w.t0w - kWin32), lds, v);
+      load_window32<TIO, true>(xr + (a.T - w.t0w - kWin32), stage, v);
     else
       load_run_rev<TIO, float, kW32>(xr, a.T - 1 - t0, false, kW32, v);
   } else if (xvec) {
-    load_window_lds<TIO, float, kW32, false, true>(xr + w.t0w, lds, v);
+    load_window32<TIO, false>(xr + w.t0w, stage, v);
   } else {
     load_run<TIO, float, kW32>(xr + t0, false, kW32, v);
   }
-  run_cascade<TIO, SAVED, UNIT>(a, w, v);
+  run_cascade<TIO, SAVED, UNIT, NWV>(a, w, v);
   if (REV) {
     if (yvec)
-      store_window32<TIO, true>(yr + (a.T - w.t0w - kWin32), lds, v, 1.0f);
+      store_window32<TIO, true>(yr + (a.T - w.t0w - kWin32), stage, v, 1.0f);
     else
       store_lane32<TIO>(yr + (a.T - 1 - t0), true, v, 1.0f);
   } else if (yvec) {
-    store_window32<TIO, false>(yr + w.t0w, lds, v, 1.0f);
+    store_window32<TIO, false>(yr + w.t0w, stage, v, 1.0f);
   } else {
     store_lane32<TIO>(yr + t0, false, v, 1.0f);
   }
 }
 
-// Full windows of every channel.  The two forms (first input tap 1 with the
-// running b0 product, or b0 kept as a tap) branch before anything is loaded,
-// so no window data is live across the branch.  REV: along decreasing time
-// (TSIIR_REVERSE); SAVED: keep the section outputs (forward_saved).
-template <typename TIO, bool REV, bool SAVED>
-__global__ __launch_bounds__(64) void sos_fwd32_kernel(FwdArgs a, const Fast32* __restrict__ fast,
-                                                       const double* __restrict__ tables) {
-  __shared__ __align__(16) TIO lds[LdsStage<TIO, kW32>::kElems];
+// Full windows of every channel, NWV consecutive windows (a group) per
+// workgroup: workgroup i = g * B + c takes windows k0 + NWV g .. k0 + NWV g +
+// NWV - 1 of channel c.  Groups of a channel are chained: a group waits for
+// the inclusive states the group before it published (dispatched B
+// workgroups earlier, normally finished), read for all sections up front.
+// Inside a group the wave aggregates are composed through LDS every section.
+// Four-window groups make each workgroup's HBM traffic 64 KB contiguous per
+// direction, which streams at the copy rate where single windows scattered
+// over the channels do not.  The shared memory is sized to 10 KB per wave so
+// that 4 waves share a SIMD (5 measured slower).  The two forms (first input
+// tap 1 with the running b0 product, or b0 kept as a tap) branch before
+// anything is loaded, so no window data is live across the branch.  REV:
+// along decreasing time (TSIIR_REVERSE); SAVED: keep the section outputs
+// (forward_saved).
+template <typename TIO, bool REV, bool SAVED, int NWV>
+__global__ __launch_bounds__(64 * NWV) void sos_fwd32_kernel(FwdArgs a, const Fast32* __restrict__ fast,
+                                                             const double* __restrict__ tables,
+                                                             int64_t k0) {
+  constexpr int kStageV = Stage32<TIO>::kBytes / 16;
+  constexpr int kSmemV = 10240 * NWV / 16;
+  constexpr int kAuxV = (2 * kWave * 8 + kWave * 4 + 2 * NWV * 2 * 8) / 16;
+  static_assert(NWV * kStageV + kAuxV <= kSmemV, "shared memory layout");
+  __shared__ u32x4 smem[kSmemV];
+  const int lane = threadIdx.x & (kWave - 1);
   const int S = a.S;
-  Win32 w;
-  w.k = static_cast<int64_t>(blockIdx.x) / a.B;
+  Win32<NWV> w;
+  w.wv = threadIdx.x >> 6;
+  w.kg = k0 + static_cast<int64_t>(blockIdx.x) / a.B * NWV;
   w.c = static_cast<int64_t>(blockIdx.x) % a.B;
   const int64_t fi = w.c / a.cpf;
   w.ff = fast + fi * S;
   w.tf = tables + fi * static_cast<int64_t>(S) * kTableStride;
-  w.t0w = w.k * kWin32;
+  w.t0w = (w.kg + w.wv) * kWin32;
   w.has_zi = (a.flags & TSIIR_HAS_ZI) != 0;
   w.zi_row = (a.flags & TSIIR_ZI_BROADCAST) ? 0 : w.c;
-  w.write_zf = (a.flags & TSIIR_WRITE_ZF) != 0 && w.t0w + kWin32 == a.T;
+  w.write_zf = (a.flags & TSIIR_WRITE_ZF) != 0 && (w.kg + NWV) * kWin32 == a.T;
+  double* prev = reinterpret_cast<double*>(smem + NWV * kStageV);
+  int* prev_ok = reinterpret_cast<int*>(prev + 2 * kWave);
+  w.prev = prev;
+  w.prev_ok = prev_ok;
+  w.wagg = reinterpret_cast<double*>(prev_ok + kWave);
+  if (w.wv == 0 && w.kg > 0 && lane < S) {
+    const uint64_t* st = a.status + (((w.kg - 1) * a.B + w.c) * S + lane) * kStatusWords;
+    double p0, p1;
+    prev_ok[lane] = read_pair(st + 4, launch_flag(a.hdr, 0), p0, p1) ? 1 : 0;
+    prev[2 * lane] = p0;
+    prev[2 * lane + 1] = p1;
+  }
+  __syncthreads();
+  u32x4* stage = smem + w.wv * kStageV;
   if (w.ff->unit != 0)
-    window32<TIO, REV, SAVED, true>(a, w, lds);
+    window32<TIO, REV, SAVED, true, NWV>(a, w, stage);
   else
-    window32<TIO, REV, SAVED, false>(a, w, lds);
+    window32<TIO, REV, SAVED, false, NWV>(a, w, stage);
 }
 
 // One thread per (filter, section) row: the Fast32 entry (see sos_common.h).
@@ -470,14 +574,24 @@ __global__ __launch_bounds__(256) void sos_prep32_kernel(const TC* __restrict__ 
 
 int64_t fwd32_window() { return kWin32; }
 
-template <typename TIO, bool REV>
-static void launch_fwd32_io(const FwdArgs& a, const Fast32* fast, dim3 grid, hipStream_t stream) {
+template <typename TIO, bool REV, int NWV>
+static void launch_fwd32_io(const FwdArgs& a, const Fast32* fast, int64_t groups, int64_t k0,
+                            hipStream_t stream) {
+  const dim3 grid(static_cast<unsigned>(groups)), block(kWave * NWV);
   if (a.saved != nullptr)
-    hipLaunchKernelGGL((sos_fwd32_kernel<TIO, REV, true>), grid, dim3(kWave), 0, stream, a, fast,
-                       a.tables);
+    hipLaunchKernelGGL((sos_fwd32_kernel<TIO, REV, true, NWV>), grid, block, 0, stream, a, fast,
+                       a.tables, k0);
   else
-    hipLaunchKernelGGL((sos_fwd32_kernel<TIO, REV, false>), grid, dim3(kWave), 0, stream, a,
-                       fast, a.tables);
+    hipLaunchKernelGGL((sos_fwd32_kernel<TIO, REV, false, NWV>), grid, block, 0, stream, a, fast,
+                       a.tables, k0);
+}
+
+template <typename TIO, bool REV>
+static void launch_fwd32_windows(const FwdArgs& a, const Fast32* fast, hipStream_t stream) {
+  const int64_t full = a.T / kWin32;
+  const int64_t n4 = full / 4;
+  if (n4 > 0) launch_fwd32_io<TIO, REV, 4>(a, fast, n4 * a.B, 0, stream);
+  if (full > 4 * n4) launch_fwd32_io<TIO, REV, 1>(a, fast, (full - 4 * n4) * a.B, 4 * n4, stream);
 }
 
 tsiir_status launch_fwd32_full(const FwdArgs& a, const void* sos, int coef_dtype,
@@ -490,21 +604,17 @@ tsiir_status launch_fwd32_full(const FwdArgs& a, const void* sos, int coef_dtype
   else
     hipLaunchKernelGGL(sos_prep32_kernel<float>, dim3(pblocks), dim3(256), 0, stream,
                        static_cast<const float*>(sos), n_rows, a.S, fast);
-  const int64_t groups = (a.T / kWin32) * a.B;
-  if (groups > 0) {
-    const dim3 grid(static_cast<unsigned>(groups));
-    const bool rev = (a.flags & TSIIR_REVERSE) != 0;
-    if (a.io_dtype == TSIIR_BF16) {
-      if (rev)
-        launch_fwd32_io<bf16_t, true>(a, fast, grid, stream);
-      else
-        launch_fwd32_io<bf16_t, false>(a, fast, grid, stream);
-    } else {
-      if (rev)
-        launch_fwd32_io<float, true>(a, fast, grid, stream);
-      else
-        launch_fwd32_io<float, false>(a, fast, grid, stream);
-    }
+  const bool rev = (a.flags & TSIIR_REVERSE) != 0;
+  if (a.io_dtype == TSIIR_BF16) {
+    if (rev)
+      launch_fwd32_windows<bf16_t, true>(a, fast, stream);
+    else
+      launch_fwd32_windows<bf16_t, false>(a, fast, stream);
+  } else {
+    if (rev)
+      launch_fwd32_windows<float, true>(a, fast, stream);
+    else
+      launch_fwd32_windows<float, false>(a, fast, stream);
   }
   return hipGetLastError() == hipSuccess ? TSIIR_OK : TSIIR_ELAUNCH;
 }
