@@ -35,8 +35,8 @@ METRIC = ("G sample·sections/s (1 GPU) vs plain; serving p50/p99 latency at "
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs scaled by the same counters over a 1 GiB copy
-# (profiles/sos_fwd_c2a_pmc.txt)
-TRAFFIC_BYTES = {"c2a": 2185012224}
+# (profiles/sos_fwd32_c2a_pmc.txt)
+TRAFFIC_BYTES = {"c2a": 2155991426}
 
 
 def parse():
@@ -132,26 +132,34 @@ def cpu_baseline(w, budget_s):
                       f"fp32 I/O, OpenMP {threads} threads, {dt:.1f} s"}
 
 
-def c5_accuracy(y, x, sos_np, coef_dtype):
-    """Normwise errors of the first 64 channels against the FP64 oracle, on the
-    coefficient values the kernel consumed (rounded to coef_dtype) and on the
-    FP64-designed coefficients."""
+def accuracy(y, x, sos_np, coef_dtype, n):
+    """Normwise errors of the first n channels against the FP64 oracle: per
+    channel max|y - y64| / max|y64| (worst channel) and the global RMS ratio,
+    on the coefficient values the kernel consumed (rounded to coef_dtype) and
+    on the FP64-designed coefficients."""
     import numpy as np
     from oracle import oracle
-    xs = x[:64].float().cpu().numpy().astype(np.float64)
+    xs = x[:n].float().cpu().numpy().astype(np.float64)
     ys = y.numpy().astype(np.float64)
+    per_channel = sos_np.ndim == 3
     res = {}
     for key, coef in (("vs_fp64_oracle_same_coefficients", sos_np.astype(coef_dtype)),
                       ("vs_fp64_designed_coefficients", sos_np)):
-        ref, _ = oracle.sosfilt_rows(coef.astype(np.float64), xs,
+        c64 = coef.astype(np.float64)
+        ref, _ = oracle.sosfilt_rows(c64[:n] if per_channel else c64, xs,
+                                     chans_per_filter=1 if per_channel else None,
                                      n_threads=min(16, os.cpu_count() or 1))
         d = ys - ref
         res[key] = {
             "max_rel": float(np.max(np.max(np.abs(d), axis=1) / np.max(np.abs(ref), axis=1))),
             "rms_rel": float(np.sqrt(np.sum(d * d) / np.sum(ref * ref))),
         }
-    res["channels_checked"] = 64
+    res["channels_checked"] = n
     return res
+
+
+# channels whose output is checked against the FP64 oracle after timing
+ACCURACY_CHANNELS = {"c2a": 256, "c2b": 256, "c2c": 8, "c5": 64, "c5c": 64, "c5bf16": 64}
 
 
 def serving_bench(args, dev):
@@ -342,7 +350,8 @@ def main():
     assert _lib.workspaces.header_error(dev) == 0, "look-back spin limit hit"
 
     # achievable streaming bandwidth on this device: a 1 GiB device copy
-    step_out = step_out[:64].float().cpu() if args.workload.startswith("c5") else None
+    n_acc = ACCURACY_CHANNELS.get(args.workload, 0)
+    step_out = step_out[:n_acc].float().cpu() if n_acc else None
     del step
     src = torch.empty(1 << 28, device=dev)
     dst = torch.empty_like(src)
@@ -382,9 +391,9 @@ def main():
                      "measured_copy_GBs": round(copy_gbs, 1),
                      "frac_of_measured_copy": round(achieved / copy_gbs, 4)},
     }
-    if args.workload.startswith("c5"):
-        out["accuracy"] = c5_accuracy(step_out, x, sos_np,
-                                      np.float64 if sos.dtype == torch.float64 else np.float32)
+    if n_acc:
+        out["accuracy"] = accuracy(step_out, x, sos_np,
+                                   np.float64 if sos.dtype == torch.float64 else np.float32, n_acc)
     if not args.no_cpu_baseline and args.workload != "c3":
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
     else:

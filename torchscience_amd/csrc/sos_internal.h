@@ -37,10 +37,10 @@ tsiir_status forward_impl(const tsiir_sosfilt_desc* d, const void* x, void* y, c
                           const void* zi, void* zf, void* workspace, size_t workspace_bytes,
                           void* stream, void* saved);
 
-// FP32-recurrence forward (sosfilt_fwd32.hip): builds the Fast32 tables and
-// filters every full window (fwd32_window() samples) of every channel whose
-// filter is `unit` (chained windows).  The FP64 tables of a.tables must be
-// prepared and the epoch advanced first.
+// FP32-recurrence forward (sosfilt_fwd32.hip): builds the FP64 tables of
+// a.tables (for 64-sample lanes) and the Fast32 tables, advances the
+// workspace epoch, and filters every full window (fwd32_window() samples) of
+// every channel (chained windows).
 struct FwdArgs;
 struct Fast32;
 int64_t fwd32_window();

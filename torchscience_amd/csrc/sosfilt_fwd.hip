@@ -678,7 +678,6 @@ static tsiir_status launch_fwd(const tsiir_sosfilt_desc* d, const FwdGeometry& g
   uint64_t* status = reinterpret_cast<uint64_t*>(ws + sizeof(WsHeader) + g.table_bytes);
   const int64_t n_rows = g.n_filters * d->n_sections;
   const int log2w = (W == 64) ? 6 : 5;
-  launch_prep(sos, d->coef_dtype, n_rows, log2w, false, tables, hdr, 1, stream);
   FwdArgs a;
   a.x = x;
   a.y = y;
@@ -701,6 +700,7 @@ static tsiir_status launch_fwd(const tsiir_sosfilt_desc* d, const FwdGeometry& g
   const FwdPolicy pol = fwd_policy(d);
   if constexpr (std::is_same<TR, float>::value && W == 64) {
     if (use_fwd32(d)) {
+      // one fused launch builds both table sets (and advances the epoch)
       Fast32* fast = reinterpret_cast<Fast32*>(ws + g.fast_off);
       const tsiir_status rc = launch_fwd32_full(a, sos, d->coef_dtype, g.n_filters, fast,
                                                 stream);
@@ -713,6 +713,7 @@ static tsiir_status launch_fwd(const tsiir_sosfilt_desc* d, const FwdGeometry& g
       return hipGetLastError() == hipSuccess ? TSIIR_OK : TSIIR_ELAUNCH;
     }
   }
+  launch_prep(sos, d->coef_dtype, n_rows, log2w, false, tables, hdr, 1, stream);
   const int64_t win = static_cast<int64_t>(kWave) * W * pol.nw;
   const int64_t full = d->n_samples / win;
   const int64_t groups = full * d->channels;

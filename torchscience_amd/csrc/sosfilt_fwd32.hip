@@ -509,14 +509,24 @@ __global__ __launch_bounds__(64 * NWV) void sos_fwd32_kernel(FwdArgs a, const Fa
     window32<TIO, REV, SAVED, false, NWV>(a, w, stage);
 }
 
-// One thread per (filter, section) row: the Fast32 entry (see sos_common.h).
-// Coefficients are normalised by a0 in the coefficient type, exactly as the
-// FP64 table prep does, then promoted.
+// One 64-lane group (one wave) per (filter, section) row: both the FP64
+// table entry the kernels of sosfilt_fwd.hip read for 64-sample lanes (the
+// values sos_prep_kernel writes) and the Fast32 entry (see sos_common.h).
+// Lane j computes its own matrix powers, by repeated squaring (A^(64 2^j),
+// the same operation sequence as the one-thread prep) or binary powering
+// (M^j and the zero-input response pair (g[j+1], g[j]) = C^j (1, 0), C the
+// companion matrix), so no lane walks a long product chain.  Coefficients are
+// normalised by a0 in the coefficient type, then promoted.  The first thread
+// advances the workspace epoch (one launch publishes status words after it).
 template <typename TC>
 __global__ __launch_bounds__(256) void sos_prep32_kernel(const TC* __restrict__ sos,
                                                          int64_t n_rows, int S,
-                                                         Fast32* __restrict__ fast) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+                                                         double* __restrict__ tables,
+                                                         Fast32* __restrict__ fast,
+                                                         WsHeader* hdr) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  const int j = threadIdx.x & (kWave - 1);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && hdr != nullptr) hdr->epoch += 1;
   if (i >= n_rows) return;
   const int64_t f = i / S;
   const int s = static_cast<int>(i % S);
@@ -542,32 +552,77 @@ __global__ __launch_bounds__(256) void sos_prep32_kernel(const TC* __restrict__ 
     a2 = a2 / a0;
   }
   const double db0 = b0, db1 = b1, db2 = b2, da1 = a1, da2 = a2;
-  Fast32* o = fast + i;
-  if (unit) {
-    o->c[0] = 1.0f;
-    o->c[1] = static_cast<float>(db1 / db0);
-    o->c[2] = static_cast<float>(db2 / db0);
-    o->p = ps;
-    o->rp = 1.0 / ps;
-  } else {
-    o->c[0] = static_cast<float>(db0);
-    o->c[1] = static_cast<float>(db1);
-    o->c[2] = static_cast<float>(db2);
-    o->p = 1.0;
-    o->rp = 1.0;
+
+  // FP64 table entry
+  double* t = tables + i * kTableStride;
+  const Mat2 A{-da1, 1.0, -da2, 0.0};
+  if (j == 0) {
+    t[0] = db0;
+    t[1] = db1;
+    t[2] = db2;
+    t[3] = da1;
+    t[4] = da2;
+    t[5] = static_cast<double>(a0);
+    t[6] = t[7] = 0.0;
   }
-  o->c[3] = static_cast<float>(-da1);
-  o->c[4] = static_cast<float>(-da2);
-  o->unit = unit ? 1 : 0;
-  for (int p = 0; p < 6; ++p) o->pad[p] = 0.0f;
-  Mat2 M{-da1, 1.0, -da2, 0.0};
+  if (j < kNPow) {  // A^(64 * 2^j)
+    Mat2 Q = A;
+    for (int l = 0; l < 6 + j; ++l) Q = mat_mul(Q, Q);
+    double* q = t + 8 + 4 * j;
+    q[0] = Q.a;
+    q[1] = Q.b;
+    q[2] = Q.c;
+    q[3] = Q.d;
+  }
+  // zero-input response g[0..64] (floats from double kGOff), zeros around it
+  const Mat2 C{-da1, -da2, 1.0, 0.0};
+  Mat2 R{1.0, 0.0, 0.0, 1.0}, Pw = C;
+#pragma unroll
+  for (int bit = 0; bit < 6; ++bit) {
+    if ((j >> bit) & 1) R = mat_mul(R, Pw);
+    Pw = mat_mul(Pw, Pw);
+  }
+  float* g = reinterpret_cast<float*>(t + kGOff);
+  g[j] = static_cast<float>(R.c);
+  if (j == kWave - 1) {
+    g[kWave] = static_cast<float>(R.a);
+    g[kWave + 1] = 0.0f;
+  }
+  if (j < kGOff - (8 + 4 * kNPow)) t[8 + 4 * kNPow + j] = 0.0;
+  constexpr int kGEnd = kGOff + (kGLen + 1) / 2;  // first double after g
+  if (kGEnd + j < kTableStride) t[kGEnd + j] = 0.0;
+
+  // Fast32 entry
+  Fast32* o = fast + i;
+  if (j == 0) {
+    if (unit) {
+      o->c[0] = 1.0f;
+      o->c[1] = static_cast<float>(db1 / db0);
+      o->c[2] = static_cast<float>(db2 / db0);
+      o->p = ps;
+      o->rp = 1.0 / ps;
+    } else {
+      o->c[0] = static_cast<float>(db0);
+      o->c[1] = static_cast<float>(db1);
+      o->c[2] = static_cast<float>(db2);
+      o->p = 1.0;
+      o->rp = 1.0;
+    }
+    o->c[3] = static_cast<float>(-da1);
+    o->c[4] = static_cast<float>(-da2);
+    o->unit = unit ? 1 : 0;
+    for (int p = 0; p < 6; ++p) o->pad[p] = 0.0f;
+  }
+  Mat2 M = A;
   for (int l = 0; l < 6; ++l) M = mat_mul(M, M);  // A^64
   Mat2 Q{1.0, 0.0, 0.0, 1.0};
-  for (int q = 0; q < 64; ++q) {
-    o->mq[q] = make_float4(static_cast<float>(Q.a), static_cast<float>(Q.b),
-                           static_cast<float>(Q.c), static_cast<float>(Q.d));
-    Q = mat_mul(Q, M);
+#pragma unroll
+  for (int bit = 0; bit < 6; ++bit) {
+    if ((j >> bit) & 1) Q = mat_mul(Q, M);
+    M = mat_mul(M, M);
   }
+  o->mq[j] = make_float4(static_cast<float>(Q.a), static_cast<float>(Q.b),
+                         static_cast<float>(Q.c), static_cast<float>(Q.d));
 }
 
 }  // namespace
@@ -597,13 +652,14 @@ static void launch_fwd32_windows(const FwdArgs& a, const Fast32* fast, hipStream
 tsiir_status launch_fwd32_full(const FwdArgs& a, const void* sos, int coef_dtype,
                                int64_t n_filters, Fast32* fast, hipStream_t stream) {
   const int64_t n_rows = n_filters * a.S;
-  const unsigned pblocks = static_cast<unsigned>((n_rows + 255) / 256);
+  const unsigned pblocks = static_cast<unsigned>((n_rows + 3) / 4);
+  double* tables = const_cast<double*>(a.tables);
   if (coef_dtype == TSIIR_F64)
     hipLaunchKernelGGL(sos_prep32_kernel<double>, dim3(pblocks), dim3(256), 0, stream,
-                       static_cast<const double*>(sos), n_rows, a.S, fast);
+                       static_cast<const double*>(sos), n_rows, a.S, tables, fast, a.hdr);
   else
     hipLaunchKernelGGL(sos_prep32_kernel<float>, dim3(pblocks), dim3(256), 0, stream,
-                       static_cast<const float*>(sos), n_rows, a.S, fast);
+                       static_cast<const float*>(sos), n_rows, a.S, tables, fast, a.hdr);
   const bool rev = (a.flags & TSIIR_REVERSE) != 0;
   if (a.io_dtype == TSIIR_BF16) {
     if (rev)
