@@ -43,10 +43,9 @@ struct FwdPolicy {
   bool chained;
 };
 static bool fwd_lds_io();
-// Full windows of FP32 recurrences (FP32 or BF16 I/O, plain mode) over many
-// channels run in the FP32-recurrence kernel of sosfilt_fwd32.hip (1-wave
-// chained windows), unless TSIIR_FWD32=0 selects this file's kernel (kept
-// for comparison measurements).
+// Full windows of FP32 recurrences (FP32 or BF16 I/O, plain mode) run in the
+// FP32-recurrence kernel of sosfilt_fwd32.hip, unless TSIIR_FWD32=0 selects
+// this file's kernels (kept for comparison measurements).
 static bool fwd32_enabled() {
   static const bool on = [] {
     const char* e = getenv("TSIIR_FWD32");
@@ -56,11 +55,12 @@ static bool fwd32_enabled() {
 }
 static inline bool use_fwd32(const tsiir_sosfilt_desc* d) {
   return (d->io_dtype == TSIIR_F32 || d->io_dtype == TSIIR_BF16) &&
-         d->mode == TSIIR_MODE_PLAIN && d->channels >= 1024 && d->n_sections <= kWave &&
-         fwd32_enabled() && fwd_lds_io();
+         d->mode == TSIIR_MODE_PLAIN && d->n_sections <= kWave && fwd32_enabled() &&
+         fwd_lds_io();
 }
 static inline FwdPolicy fwd_policy(const tsiir_sosfilt_desc* d) {
   if (d->channels >= 1024 && d->n_sections <= 64) return FwdPolicy{1, true};
+  if (use_fwd32(d)) return FwdPolicy{1, false};  // status words per 4096-sample window
   return FwdPolicy{4, false};
 }
 

@@ -26,8 +26,9 @@
 //      Kogge-Stone inside each 16-lane row with DPP row shifts, then two
 //      DPP row broadcasts that carry the row totals across the wave.
 //   3. the window's incoming state, in FP64: zi, or the inclusive state of
-//      window k-1 (read once for all sections up front), composed with the
-//      window aggregate in FP64 and published for window k+1.
+//      the window before (many channels: read once for all sections up
+//      front; few channels: decoupled look-back), composed with the window
+//      aggregate in FP64 and published for the windows after it.
 //   4. each lane's incoming state M^q s_window + exclusive_q (M^q per lane
 //      from the table) becomes the pending zero-input response.
 // x is read from HBM once and y written once, through LDS with non-temporal
@@ -47,6 +48,7 @@ namespace tsiir {
 namespace {
 
 constexpr int kW32 = 64;                              // samples per lane
+constexpr int64_t kChainedChannels = 1024;  // chained groups from this many channels on
 constexpr int64_t kWin32 = static_cast<int64_t>(kWave) * kW32;  // samples per window
 
 // DPP controls (GFX9 encoding)
@@ -260,7 +262,84 @@ struct Win32 {
   const double* prev;  // (S, 2)
   const int* prev_ok;  // (S): all words carried the current flag
   double* wagg;        // (2, NWV, 2): wave aggregates, double-buffered by section
+  double* sgrp;        // (2, 2): the group's incoming state (look-back form)
 };
+
+// Decoupled look-back over the groups before group gi of channel c (NWV
+// windows each; a group's status words sit at the index of its last window):
+// the incoming state of group gi, the same value in every lane.  Lane l
+// inspects group gi-1-l; the nearest group with an inclusive state (or the
+// virtual group -1 holding zi) ends the search, groups in between contribute
+// their aggregates,
+//   in = sum_{l < f} Mg^l agg_(gi-1-l) + Mg^f incl_(gi-1-f),
+// Mg = A^(4096 NWV) = table power 6 + log2(NWV), combined by a shuffle-down
+// reduction with the powers Mg^(2^i).  A wait longer than kSpinLimit polls
+// sets the error word and gives up, so every wave terminates.
+template <int NWV>
+__device__ __forceinline__ void lookback_groups(const uint64_t* status, int64_t B, int S,
+                                                WsHeader* hdr, int64_t gi, int64_t c, int s,
+                                                uint32_t flag, const double* tb, double zi0,
+                                                double zi1, double& sw0, double& sw1) {
+  constexpr int kPbase = 6 + (NWV == 4 ? 2 : (NWV == 2 ? 1 : 0));
+  const int lane = threadIdx.x & (kWave - 1);
+  double acc0 = 0.0, acc1 = 0.0;
+  Mat2 P{1.0, 0.0, 0.0, 1.0};
+  int64_t base = gi - 1;
+  uint32_t spins = 0;
+  while (true) {
+    const int64_t j = base - lane;
+    int st = 2;
+    double v0 = 0.0, v1 = 0.0;
+    if (j >= 0) {
+      const uint64_t* e = status + (((j * NWV + NWV - 1) * B + c) * S + s) * kStatusWords;
+      if (read_pair(e + 4, flag, v0, v1))
+        st = 2;
+      else if (read_pair(e, flag, v0, v1))
+        st = 1;
+      else
+        st = 0;
+    } else if (j == -1) {
+      v0 = zi0;
+      v1 = zi1;
+    }
+    const uint64_t incl = __ballot(st == 2);
+    const uint64_t ready = __ballot(st != 0);
+    const int f = incl ? (__ffsll(static_cast<unsigned long long>(incl)) - 1) : 64;
+    const uint64_t need = (f >= 63) ? ~0ull : ((1ull << (f + 1)) - 1ull);
+    if ((ready & need) != need) {
+      if (++spins > kSpinLimit) {
+        if (lane == 0) atomicOr(&hdr->error, 1u);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+      continue;
+    }
+    if (lane > f) {
+      v0 = 0.0;
+      v1 = 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const int d = 1 << i;
+      const double n0 = __shfl_down(v0, d);
+      const double n1 = __shfl_down(v1, d);
+      const Mat2 Q = load_mat(table_pow(tb, kPbase + i));
+      if (lane + d < kWave) {
+        v0 += Q.a * n0 + Q.b * n1;
+        v1 += Q.c * n0 + Q.d * n1;
+      }
+    }
+    const double r0 = __shfl(v0, 0);
+    const double r1 = __shfl(v1, 0);
+    acc0 += P.a * r0 + P.b * r1;
+    acc1 += P.c * r0 + P.d * r1;
+    if (f < 64) break;
+    P = mat_mul(P, load_mat(table_pow(tb, kPbase + 6)));
+    base -= kWave;
+  }
+  sw0 = acc0;
+  sw1 = acc1;
+}
 
 // The lane's powers of M for section s: pm[0] = M^((q mod 16)+1) and
 // pm[1] = M^(max(q-31, 0)) for the scan, pm[2] = M^q for the incoming state.
@@ -278,7 +357,7 @@ __device__ __forceinline__ LanePow lane_pow(const Fast32* __restrict__ fs) {
 // publication of the group's inclusive state, and the lane's incoming state
 // (pw0, pw1) in the scaled units of the section, which becomes the pending
 // zero-input response.
-template <int NWV>
+template <int NWV, bool LB>
 __device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32<NWV>& w, int s,
                                               const LanePow& lp, float z0, float z1, float& pw0,
                                               float& pw1) {
@@ -305,9 +384,60 @@ __device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32<NWV>
   }
 
   // the group's incoming state (true units, FP64): zi, or the inclusive
-  // state window kg-1 published
+  // state window kg-1 published (chained), or the look-back's result
   double sw0 = 0.0, sw1 = 0.0;
-  if (w.kg == 0) {
+  const Mat2 Mw = load_mat(table_pow(tb, 6));  // A^4096, one window
+  if constexpr (LB) {
+    // wave 0 publishes the group's aggregate, resolves its incoming state by
+    // the look-back, publishes its inclusive state and hands the incoming
+    // state to the other waves through LDS
+    double* sg = w.sgrp + (s & 1) * 2;
+    if (w.wv == 0) {
+      double g0 = A0, g1 = A1;
+      if (NWV > 1) {
+        g0 = wagg[0];
+        g1 = wagg[1];
+#pragma unroll
+        for (int j = 1; j < NWV; ++j) {
+          const double n0 = fma(Mw.a, g0, fma(Mw.b, g1, wagg[2 * j]));
+          const double n1 = fma(Mw.c, g0, fma(Mw.d, g1, wagg[2 * j + 1]));
+          g0 = n0;
+          g1 = n1;
+        }
+      }
+      uint64_t* st = a.status + (((w.kg + NWV - 1) * a.B + w.c) * S + s) * kStatusWords;
+      publish_pair(st, g0, g1, flag, lane);
+      double zi0 = 0.0, zi1 = 0.0;
+      if (w.has_zi) {
+        const float* zp = static_cast<const float*>(a.zi) + (w.zi_row * S + s) * 2;
+        zi0 = static_cast<double>(zp[0]);
+        zi1 = static_cast<double>(zp[1]);
+      }
+      sw0 = zi0;
+      sw1 = zi1;
+      if (w.kg > 0)
+        lookback_groups<NWV>(a.status, a.B, S, a.hdr, w.kg / NWV, w.c, s, flag, tb, zi0, zi1,
+                             sw0, sw1);
+      const Mat2 Mg = load_mat(table_pow(tb, 6 + (NWV == 4 ? 2 : (NWV == 2 ? 1 : 0))));
+      const double i0 = fma(Mg.a, sw0, fma(Mg.b, sw1, g0));
+      const double i1 = fma(Mg.c, sw0, fma(Mg.d, sw1, g1));
+      publish_pair(st + 4, i0, i1, flag, lane);
+      if (w.write_zf && lane == 0) {
+        float* zp = static_cast<float*>(a.zf) + (w.c * S + s) * 2;
+        zp[0] = static_cast<float>(i0);
+        zp[1] = static_cast<float>(i1);
+      }
+      if (NWV > 1 && lane == 0) {
+        sg[0] = sw0;
+        sg[1] = sw1;
+      }
+    }
+    if (NWV > 1) {
+      __syncthreads();
+      sw0 = sg[0];
+      sw1 = sg[1];
+    }
+  } else if (w.kg == 0) {
     if (w.has_zi) {
       const float* zp = static_cast<const float*>(a.zi) + (w.zi_row * S + s) * 2;
       sw0 = static_cast<double>(zp[0]);
@@ -330,7 +460,6 @@ __device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32<NWV>
   }
   // this wave's incoming state: the group's, carried over the windows of
   // the waves before it
-  const Mat2 Mw = load_mat(table_pow(tb, 6));  // A^4096, one window
 #pragma unroll
   for (int j = 0; j < NWV - 1; ++j) {
     if (j < w.wv) {
@@ -340,7 +469,7 @@ __device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32<NWV>
       sw1 = n1;
     }
   }
-  if (w.wv == NWV - 1) {
+  if (!LB && w.wv == NWV - 1) {
     // the last wave publishes the group's inclusive state (window kg+NWV-1)
     const double i0 = fma(Mw.a, sw0, fma(Mw.b, sw1, A0));
     const double i1 = fma(Mw.c, sw0, fma(Mw.d, sw1, A1));
@@ -393,20 +522,20 @@ __device__ __forceinline__ void keep_section(const FwdArgs& a, const Win32<NWV>&
 // All sections of the window (section 0 has no pending correction, the
 // others take the previous section's); on return v holds the cascade output
 // in true units.  SAVED: the outputs of sections 0..S-2 are also stored.
-template <typename TIO, bool SAVED, bool UNIT, int NWV>
+template <typename TIO, bool SAVED, bool UNIT, int NWV, bool LB>
 __device__ __forceinline__ void run_cascade(const FwdArgs& a, const Win32<NWV>& w,
                                             float (&v)[kW32]) {
   const int S = a.S;
   float pw0 = 0.0f, pw1 = 0.0f, z0, z1;
   LanePow lp = lane_pow(w.ff);
   df1_sweep<UNIT, false>(v, w.ff, table_g(w.tf), 0.0f, 0.0f, z0, z1);
-  section_carry<NWV>(a, w, 0, lp, z0, z1, pw0, pw1);
+  section_carry<NWV, LB>(a, w, 0, lp, z0, z1, pw0, pw1);
   if (SAVED && S > 1) keep_section<TIO, NWV>(a, w, 0, v, pw0, pw1);
   for (int s = 1; s < S; ++s) {
     lp = lane_pow(w.ff + s);
     const float* pg = table_g(w.tf + static_cast<int64_t>(s - 1) * kTableStride);
     df1_sweep<UNIT, true>(v, w.ff + s, pg, pw0, pw1, z0, z1);
-    section_carry<NWV>(a, w, s, lp, z0, z1, pw0, pw1);
+    section_carry<NWV, LB>(a, w, s, lp, z0, z1, pw0, pw1);
     if (SAVED && s + 1 < S) keep_section<TIO, NWV>(a, w, s, v, pw0, pw1);
   }
   const float* pg = table_g(w.tf + static_cast<int64_t>(S - 1) * kTableStride);
@@ -417,7 +546,7 @@ __device__ __forceinline__ void run_cascade(const FwdArgs& a, const Win32<NWV>& 
 }
 
 // One window per wave: load, cascade, store.
-template <typename TIO, bool REV, bool SAVED, bool UNIT, int NWV>
+template <typename TIO, bool REV, bool SAVED, bool UNIT, int NWV, bool LB>
 __device__ __forceinline__ void window32(const FwdArgs& a, Win32<NWV>& w, u32x4* stage) {
   const int lane = threadIdx.x & (kWave - 1);
   const TIO* xr = static_cast<const TIO*>(a.x) + (w.c % a.x_rows) * a.x_stride;
@@ -440,7 +569,7 @@ __device__ __forceinline__ void window32(const FwdArgs& a, Win32<NWV>& w, u32x4*
   } else {
     load_run<TIO, float, kW32>(xr + t0, false, kW32, v);
   }
-  run_cascade<TIO, SAVED, UNIT, NWV>(a, w, v);
+  run_cascade<TIO, SAVED, UNIT, NWV, LB>(a, w, v);
   if (REV) {
     if (yvec)
       store_window32<TIO, true>(yr + (a.T - w.t0w - kWin32), stage, v, 1.0f);
@@ -455,10 +584,13 @@ __device__ __forceinline__ void window32(const FwdArgs& a, Win32<NWV>& w, u32x4*
 
 // Full windows of every channel, NWV consecutive windows (a group) per
 // workgroup: workgroup i = g * B + c takes windows k0 + NWV g .. k0 + NWV g +
-// NWV - 1 of channel c.  Groups of a channel are chained: a group waits for
-// the inclusive states the group before it published (dispatched B
-// workgroups earlier, normally finished), read for all sections up front.
-// Inside a group the wave aggregates are composed through LDS every section.
+// NWV - 1 of channel c.  Chained (!LB): a group waits for the inclusive
+// states the group before it published (dispatched B workgroups earlier,
+// normally finished), read for all sections up front.  LB: every section, the
+// group publishes its aggregate and resolves its incoming state by a
+// decoupled look-back over the groups before it (few channels: those run at
+// the same time).  Inside a group the wave aggregates are composed through
+// LDS every section.
 // Four-window groups make each workgroup's HBM traffic 64 KB contiguous per
 // direction, which streams at the copy rate where single windows scattered
 // over the channels do not.  The shared memory is sized to 10 KB per wave so
@@ -467,13 +599,13 @@ __device__ __forceinline__ void window32(const FwdArgs& a, Win32<NWV>& w, u32x4*
 // anything is loaded, so no window data is live across the branch.  REV:
 // along decreasing time (TSIIR_REVERSE); SAVED: keep the section outputs
 // (forward_saved).
-template <typename TIO, bool REV, bool SAVED, int NWV>
+template <typename TIO, bool REV, bool SAVED, int NWV, bool LB>
 __global__ __launch_bounds__(64 * NWV) void sos_fwd32_kernel(FwdArgs a, const Fast32* __restrict__ fast,
                                                              const double* __restrict__ tables,
                                                              int64_t k0) {
   constexpr int kStageV = Stage32<TIO>::kBytes / 16;
   constexpr int kSmemV = 10240 * NWV / 16;
-  constexpr int kAuxV = (2 * kWave * 8 + kWave * 4 + 2 * NWV * 2 * 8) / 16;
+  constexpr int kAuxV = (2 * kWave * 8 + kWave * 4 + 2 * NWV * 2 * 8 + 2 * 2 * 8) / 16;
   static_assert(NWV * kStageV + kAuxV <= kSmemV, "shared memory layout");
   __shared__ u32x4 smem[kSmemV];
   const int lane = threadIdx.x & (kWave - 1);
@@ -494,7 +626,8 @@ __global__ __launch_bounds__(64 * NWV) void sos_fwd32_kernel(FwdArgs a, const Fa
   w.prev = prev;
   w.prev_ok = prev_ok;
   w.wagg = reinterpret_cast<double*>(prev_ok + kWave);
-  if (w.wv == 0 && w.kg > 0 && lane < S) {
+  w.sgrp = w.wagg + 2 * NWV * 2;
+  if (!LB && w.wv == 0 && w.kg > 0 && lane < S) {
     const uint64_t* st = a.status + (((w.kg - 1) * a.B + w.c) * S + lane) * kStatusWords;
     double p0, p1;
     prev_ok[lane] = read_pair(st + 4, launch_flag(a.hdr, 0), p0, p1) ? 1 : 0;
@@ -504,9 +637,9 @@ __global__ __launch_bounds__(64 * NWV) void sos_fwd32_kernel(FwdArgs a, const Fa
   __syncthreads();
   u32x4* stage = smem + w.wv * kStageV;
   if (w.ff->unit != 0)
-    window32<TIO, REV, SAVED, true, NWV>(a, w, stage);
+    window32<TIO, REV, SAVED, true, NWV, LB>(a, w, stage);
   else
-    window32<TIO, REV, SAVED, false, NWV>(a, w, stage);
+    window32<TIO, REV, SAVED, false, NWV, LB>(a, w, stage);
 }
 
 // One 64-lane group (one wave) per (filter, section) row: both the FP64
@@ -629,24 +762,34 @@ __global__ __launch_bounds__(256) void sos_prep32_kernel(const TC* __restrict__ 
 
 int64_t fwd32_window() { return kWin32; }
 
-template <typename TIO, bool REV, int NWV>
+template <typename TIO, bool REV, int NWV, bool LB>
 static void launch_fwd32_io(const FwdArgs& a, const Fast32* fast, int64_t groups, int64_t k0,
                             hipStream_t stream) {
   const dim3 grid(static_cast<unsigned>(groups)), block(kWave * NWV);
   if (a.saved != nullptr)
-    hipLaunchKernelGGL((sos_fwd32_kernel<TIO, REV, true, NWV>), grid, block, 0, stream, a, fast,
-                       a.tables, k0);
+    hipLaunchKernelGGL((sos_fwd32_kernel<TIO, REV, true, NWV, LB>), grid, block, 0, stream, a,
+                       fast, a.tables, k0);
   else
-    hipLaunchKernelGGL((sos_fwd32_kernel<TIO, REV, false, NWV>), grid, block, 0, stream, a, fast,
-                       a.tables, k0);
+    hipLaunchKernelGGL((sos_fwd32_kernel<TIO, REV, false, NWV, LB>), grid, block, 0, stream, a,
+                       fast, a.tables, k0);
 }
 
+// Groups of four windows, chained (many channels) or with the look-back
+// (fewer than kChainedChannels channels: a group's predecessor runs at the
+// same time), then the leftover full windows one per workgroup, chained to
+// the last group (their predecessors in the same launch are few).
 template <typename TIO, bool REV>
 static void launch_fwd32_windows(const FwdArgs& a, const Fast32* fast, hipStream_t stream) {
   const int64_t full = a.T / kWin32;
   const int64_t n4 = full / 4;
-  if (n4 > 0) launch_fwd32_io<TIO, REV, 4>(a, fast, n4 * a.B, 0, stream);
-  if (full > 4 * n4) launch_fwd32_io<TIO, REV, 1>(a, fast, (full - 4 * n4) * a.B, 4 * n4, stream);
+  if (n4 > 0) {
+    if (a.B >= kChainedChannels)
+      launch_fwd32_io<TIO, REV, 4, false>(a, fast, n4 * a.B, 0, stream);
+    else
+      launch_fwd32_io<TIO, REV, 4, true>(a, fast, n4 * a.B, 0, stream);
+  }
+  if (full > 4 * n4)
+    launch_fwd32_io<TIO, REV, 1, false>(a, fast, (full - 4 * n4) * a.B, 4 * n4, stream);
 }
 
 tsiir_status launch_fwd32_full(const FwdArgs& a, const void* sos, int coef_dtype,
