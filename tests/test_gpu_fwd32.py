@@ -1,13 +1,16 @@
 """FP32-recurrence forward kernel (many channels, chained 4096-sample windows)
 against the FP64 oracle (GPU).
 
-At least 1024 channels select this kernel for FP32 and BF16 I/O.  Covers:
+FP32 and BF16 I/O with up to 64 sections select this kernel: with at least
+1024 channels the 16384-sample groups of a channel are chained, with fewer
+each group looks back over the groups before it.  Covers:
 filters whose b0 can be normalised (first input tap 1, running b0 product
 restored on output) and filters that keep b0 as a tap (a zero b0, or an
 overall gain far from 1), mixed in one per-channel batch; lengths around the
 window (4096 samples) boundary; zi per channel and broadcast; zf; input rows
 shared by several channels; rows that are not 16-byte aligned; time reversal;
-the saved section outputs of the training forward.
+the saved section outputs of the training forward; few channels with more
+than 64 predecessor groups (several look-back rounds).
 
 Tolerances are per-channel normwise relative errors max|y - y64| / max|y64|
 against the FP64 oracle run on the same (rounded) coefficients and inputs:
@@ -209,3 +212,32 @@ def test_error_next_to_channel_persistent_kernel(oracle_mod):
     print(f"relerr fp32 window kernel {e_new:.3e}, channel kernel {e_chan:.3e}")
     assert e_new < F32_TOL and e_chan < F32_TOL
     assert e_new <= 2 * e_chan
+
+
+@pytest.mark.parametrize("reverse", [False, True])
+def test_few_channels_lookback(oracle_mod, reverse):
+    """5 channels of 70 groups of 16384 samples, two more full windows and a
+    partial one: the look-back combines 64 predecessors per round, so group
+    69 needs two rounds.  Per-channel filters, two of them keeping b0 as a
+    tap; zi and zf."""
+    dev = _dev()
+    B, S = 5, 4
+    T = 70 * 16384 + 2 * 4096 + 37
+    rng = np.random.default_rng(8 + reverse)
+    sos = _butter(2 * S, rng.uniform(0.02, 0.45, B))
+    sos[1:3] = _nonunit_filters(2, S, rng)
+    sos = sos.astype(np.float32)
+    x = rng.standard_normal((B, T)).astype(np.float32)
+    zi = rng.standard_normal((B, S, 2)).astype(np.float32)
+    print("few channels lookback, reverse", reverse, flush=True)
+    y, zf = _fwd(torch.tensor(sos, device=dev), torch.tensor(x, device=dev),
+                 torch.tensor(zi, device=dev), channels=B, chans_per_filter=1, want_zf=True,
+                 reverse=reverse)
+    xo = x[:, ::-1] if reverse else x
+    y64, zf64 = oracle_mod.sosfilt_rows(sos.astype(np.float64), xo.astype(np.float64),
+                                        zi.astype(np.float64), chans_per_filter=1, n_threads=8)
+    yh = y.cpu().numpy()
+    err = relerr(yh[:, ::-1] if reverse else yh, y64)
+    print("lookback relerr", err)
+    assert err < F32_TOL
+    np.testing.assert_allclose(zf.cpu().numpy(), zf64, rtol=1e-4, atol=1e-4 * np.abs(zf64).max())

@@ -32,8 +32,8 @@
 //   4. each lane's incoming state M^q s_window + exclusive_q (M^q per lane
 //      from the table) becomes the pending zero-input response.
 // x is read from HBM once and y written once, through LDS with non-temporal
-// 16-byte accesses (an unpadded, XOR-swizzled 8 KB staging buffer: with 95
-// VGPRs, 5 waves fit per SIMD).  Partial last windows run in the FP64-carry kernel of
+// 16-byte accesses (an unpadded, XOR-swizzled 8 KB staging buffer per wave,
+// all 16 loads of a lane in flight at once).  Partial last windows run in the FP64-carry kernel of
 // sosfilt_fwd.hip, which reads the inclusive states published here (true,
 // unscaled DF2T states).
 
@@ -50,6 +50,16 @@ namespace {
 constexpr int kW32 = 64;                              // samples per lane
 constexpr int64_t kChainedChannels = 1024;  // chained groups from this many channels on
 constexpr int64_t kWin32 = static_cast<int64_t>(kWave) * kW32;  // samples per window
+
+// The lane index, re-read at each use: the empty asm hides that it is
+// loop-invariant, so values derived from it (LDS slots, lane offsets) are
+// formed where they are needed instead of being hoisted out of the section
+// loop and held in registers across the cascade.
+__device__ __forceinline__ int lane_now() {
+  uint32_t t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return static_cast<int>(t & (kWave - 1));
+}
 
 // DPP controls (GFX9 encoding)
 constexpr int kDppRowShr = 0x110;      // + shift 1..15, within each row of 16 lanes
@@ -177,25 +187,52 @@ struct Stage32 {
   __device__ static __forceinline__ int slot(int q, int t) { return q * kH + (t ^ (q & (kH - 1))); }
 };
 
-// Window -> registers: lane q gets row q (REV: row 63 - q, elements reversed).
+// Wave-local ordering of the staging buffer: each wave stages through its
+// own LDS region, and a wave's LDS accesses complete in issue order, so a
+// wait for its outstanding LDS operations (and a compiler barrier) suffices.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+// The window's 16-byte vectors as this lane loads them (both rounds), not
+// waited for.
+template <typename TIO>
+struct RawWin {
+  u32x4 r[2][Stage32<TIO>::kH];
+};
+template <typename TIO>
+__device__ __forceinline__ void issue_window32(const TIO* win, RawWin<TIO>& raw) {
+  using L = Stage32<TIO>;
+  const int lane = lane_now();
+  const u32x4* __restrict__ gp = reinterpret_cast<const u32x4*>(win);
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int i = 0; i < L::kH; ++i) raw.r[r][i] = __builtin_nontemporal_load(gp + L::gvec(r, i, lane));
+}
+
+// Loaded vectors -> registers through the wave's staging buffer: lane q gets
+// row q (REV: row 63 - q, elements reversed).  The raw vectors of a round
+// are dead once staged, so the samples take their registers.
 template <typename TIO, bool REV>
-__device__ __forceinline__ void load_window32(const TIO* win, u32x4* lds, float (&v)[kW32]) {
+__device__ __forceinline__ void stage_window32(const RawWin<TIO>& raw, u32x4* lds,
+                                               float (&v)[kW32]) {
   using L = Stage32<TIO>;
   using vec_t = typename IOTraits<TIO>::vec_t;
-  const int lane = threadIdx.x & (kWave - 1);
-  const u32x4* __restrict__ gp = reinterpret_cast<const u32x4*>(win);
+  const int lane = lane_now();
   const int q = REV ? kWave - 1 - lane : lane;
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
 #pragma unroll
-    for (int i = 0; i < L::kH; ++i)
-      lds[L::slot(L::grow(i, lane), lane % L::kH)] =
-          __builtin_nontemporal_load(gp + L::gvec(r, i, lane));
-    __syncthreads();
+    for (int i = 0; i < L::kH; ++i) lds[L::slot(L::grow(i, lane), lane % L::kH)] = raw.r[r][i];
+    wave_lds_sync();
 #pragma unroll
     for (int t = 0; t < L::kH; ++t) {
-      const u32x4 raw = lds[L::slot(q, t)];
-      const vec_t rv = *reinterpret_cast<const vec_t*>(&raw);
+      const u32x4 rw = lds[L::slot(q, t)];
+      const vec_t rv = *reinterpret_cast<const vec_t*>(&rw);
       const TIO* e = reinterpret_cast<const TIO*>(&rv);
 #pragma unroll
       for (int j = 0; j < L::kVec; ++j) {
@@ -203,8 +240,16 @@ __device__ __forceinline__ void load_window32(const TIO* win, u32x4* lds, float 
         v[REV ? kW32 - 1 - idx : idx] = IOTraits<TIO>::to_f(e[j]);
       }
     }
-    __syncthreads();
+    wave_lds_sync();
   }
+}
+
+// Window -> registers, both rounds' loads in flight at once.
+template <typename TIO, bool REV>
+__device__ __forceinline__ void load_window32(const TIO* win, u32x4* lds, float (&v)[kW32]) {
+  RawWin<TIO> raw;
+  issue_window32<TIO>(win, raw);
+  stage_window32<TIO, REV>(raw, lds, v);
 }
 
 // Registers -> window, each value multiplied by `scale` (non-temporal stores).
@@ -213,7 +258,7 @@ __device__ __forceinline__ void store_window32(TIO* win, u32x4* lds, const float
                                                float scale) {
   using L = Stage32<TIO>;
   using vec_t = typename IOTraits<TIO>::vec_t;
-  const int lane = threadIdx.x & (kWave - 1);
+  const int lane = lane_now();
   u32x4* __restrict__ gp = reinterpret_cast<u32x4*>(win);
   const int q = REV ? kWave - 1 - lane : lane;
 #pragma unroll
@@ -229,12 +274,12 @@ __device__ __forceinline__ void store_window32(TIO* win, u32x4* lds, const float
       }
       lds[L::slot(q, t)] = *reinterpret_cast<const u32x4*>(&rv);
     }
-    __syncthreads();
+    wave_lds_sync();
 #pragma unroll
     for (int i = 0; i < L::kH; ++i)
       __builtin_nontemporal_store(lds[L::slot(L::grow(i, lane), lane % L::kH)],
                                   gp + L::gvec(r, i, lane));
-    __syncthreads();
+    wave_lds_sync();
   }
 }
 
@@ -258,7 +303,9 @@ struct Win32 {
   const double* tf;    // its FP64 table entries
   int64_t zi_row;
   bool has_zi, write_zf;
-  // window kg-1's inclusive state of every section, read up front
+  uint32_t flag;       // epoch of this launch
+  // chained form: the group's incoming state of every section, read up front
+  // (zi or zeros for the first group, else window kg-1's inclusive state)
   const double* prev;  // (S, 2)
   const int* prev_ok;  // (S): all words carried the current flag
   double* wagg;        // (2, NWV, 2): wave aggregates, double-buffered by section
@@ -281,7 +328,7 @@ __device__ __forceinline__ void lookback_groups(const uint64_t* status, int64_t 
                                                 uint32_t flag, const double* tb, double zi0,
                                                 double zi1, double& sw0, double& sw1) {
   constexpr int kPbase = 6 + (NWV == 4 ? 2 : (NWV == 2 ? 1 : 0));
-  const int lane = threadIdx.x & (kWave - 1);
+  const int lane = lane_now();
   double acc0 = 0.0, acc1 = 0.0;
   Mat2 P{1.0, 0.0, 0.0, 1.0};
   int64_t base = gi - 1;
@@ -347,9 +394,9 @@ __device__ __forceinline__ void lookback_groups(const uint64_t* status, int64_t 
 struct LanePow {
   float4 pm[3];
 };
-__device__ __forceinline__ LanePow lane_pow(const Fast32* __restrict__ fs) {
-  const int lane = threadIdx.x & (kWave - 1);
-  return LanePow{{fs->mq[(lane & 15) + 1], fs->mq[lane > 31 ? lane - 31 : 0], fs->mq[lane]}};
+__device__ __forceinline__ LanePow lane_pow(const float4* mq) {
+  const int lane = lane_now();
+  return LanePow{{mq[(lane & 15) + 1], mq[lane > 31 ? lane - 31 : 0], mq[lane]}};
 }
 
 // Steps 2-4 of section s after its sweep: lane scan, composition of the
@@ -361,9 +408,9 @@ template <int NWV, bool LB>
 __device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32<NWV>& w, int s,
                                               const LanePow& lp, float z0, float z1, float& pw0,
                                               float& pw1) {
-  const int lane = threadIdx.x & (kWave - 1);
+  const int lane = lane_now();
   const int S = a.S;
-  const uint32_t flag = launch_flag(a.hdr, 0);
+  const uint32_t flag = w.flag;
   const Fast32* fs = w.ff + s;
   const double* tb = w.tf + static_cast<int64_t>(s) * kTableStride;
   const float4 pa = lp.pm[0], pb = lp.pm[1], pq = lp.pm[2];
@@ -437,12 +484,6 @@ __device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32<NWV>
       sw0 = sg[0];
       sw1 = sg[1];
     }
-  } else if (w.kg == 0) {
-    if (w.has_zi) {
-      const float* zp = static_cast<const float*>(a.zi) + (w.zi_row * S + s) * 2;
-      sw0 = static_cast<double>(zp[0]);
-      sw1 = static_cast<double>(zp[1]);
-    }
   } else {
     sw0 = w.prev[2 * s];
     sw1 = w.prev[2 * s + 1];
@@ -499,7 +540,7 @@ __device__ __forceinline__ void keep_section(const FwdArgs& a, const Win32<NWV>&
   pw0 = 0.0f;
   pw1 = 0.0f;
   TIO* sr = static_cast<TIO*>(a.saved) + (static_cast<int64_t>(s) * a.B + w.c) * a.T +
-            w.t0w + static_cast<int64_t>(threadIdx.x & (kWave - 1)) * kW32;
+            w.t0w + static_cast<int64_t>(lane_now()) * kW32;
   // per-lane 16-byte stores straight from the sample registers (a staged
   // store would hold a second copy of the window in registers)
   const float scale = static_cast<float>(w.ff[s].p);
@@ -521,34 +562,51 @@ __device__ __forceinline__ void keep_section(const FwdArgs& a, const Win32<NWV>&
 
 // All sections of the window (section 0 has no pending correction, the
 // others take the previous section's); on return v holds the cascade output
-// in true units.  SAVED: the outputs of sections 0..S-2 are also stored.
+// in scaled units with the last section's correction (pw0, pw1) pending
+// (finish_cascade applies it).  SAVED: the outputs of
+// sections 0..S-2 are also stored.
 template <typename TIO, bool SAVED, bool UNIT, int NWV, bool LB>
 __device__ __forceinline__ void run_cascade(const FwdArgs& a, const Win32<NWV>& w,
-                                            float (&v)[kW32]) {
+                                            float (&v)[kW32], float& pw0, float& pw1) {
   const int S = a.S;
-  float pw0 = 0.0f, pw1 = 0.0f, z0, z1;
-  LanePow lp = lane_pow(w.ff);
+  float z0, z1;
+  pw0 = 0.0f;
+  pw1 = 0.0f;
+  LanePow lp = lane_pow(w.ff->mq);
   df1_sweep<UNIT, false>(v, w.ff, table_g(w.tf), 0.0f, 0.0f, z0, z1);
   section_carry<NWV, LB>(a, w, 0, lp, z0, z1, pw0, pw1);
   if (SAVED && S > 1) keep_section<TIO, NWV>(a, w, 0, v, pw0, pw1);
   for (int s = 1; s < S; ++s) {
-    lp = lane_pow(w.ff + s);
+    lp = lane_pow(w.ff[s].mq);
     const float* pg = table_g(w.tf + static_cast<int64_t>(s - 1) * kTableStride);
     df1_sweep<UNIT, true>(v, w.ff + s, pg, pw0, pw1, z0, z1);
     section_carry<NWV, LB>(a, w, s, lp, z0, z1, pw0, pw1);
     if (SAVED && s + 1 < S) keep_section<TIO, NWV>(a, w, s, v, pw0, pw1);
   }
-  const float* pg = table_g(w.tf + static_cast<int64_t>(S - 1) * kTableStride);
-  if constexpr (UNIT)
-    apply_corr<true>(v, pg, pw0, pw1, static_cast<float>(w.ff[S - 1].p));
-  else
-    apply_corr<false>(v, pg, pw0, pw1, 1.0f);
+}
+
+// The last section's zero-input response table, and the factor that
+// restores true units (UNIT: the running b0 product).
+template <int NWV>
+__device__ __forceinline__ const float* last_g(const FwdArgs& a, const Win32<NWV>& w) {
+  return table_g(w.tf + static_cast<int64_t>(a.S - 1) * kTableStride);
+}
+template <bool UNIT, int NWV>
+__device__ __forceinline__ float out_scale(const FwdArgs& a, const Win32<NWV>& w) {
+  return UNIT ? static_cast<float>(w.ff[a.S - 1].p) : 1.0f;
+}
+
+// v = the cascade output in true units.
+template <bool UNIT, int NWV>
+__device__ __forceinline__ void finish_cascade(const FwdArgs& a, const Win32<NWV>& w,
+                                               float (&v)[kW32], float pw0, float pw1) {
+  apply_corr<UNIT>(v, last_g(a, w), pw0, pw1, out_scale<UNIT>(a, w));
 }
 
 // One window per wave: load, cascade, store.
 template <typename TIO, bool REV, bool SAVED, bool UNIT, int NWV, bool LB>
 __device__ __forceinline__ void window32(const FwdArgs& a, Win32<NWV>& w, u32x4* stage) {
-  const int lane = threadIdx.x & (kWave - 1);
+  const int lane = lane_now();
   const TIO* xr = static_cast<const TIO*>(a.x) + (w.c % a.x_rows) * a.x_stride;
   TIO* yr = static_cast<TIO*>(a.y) + w.c * a.y_stride;
   // REV: processing index t is time T-1-t, so the window is the time-ordered
@@ -569,7 +627,9 @@ __device__ __forceinline__ void window32(const FwdArgs& a, Win32<NWV>& w, u32x4*
   } else {
     load_run<TIO, float, kW32>(xr + t0, false, kW32, v);
   }
-  run_cascade<TIO, SAVED, UNIT, NWV, LB>(a, w, v);
+  float pw0, pw1;
+  run_cascade<TIO, SAVED, UNIT, NWV, LB>(a, w, v, pw0, pw1);
+  finish_cascade<UNIT>(a, w, v, pw0, pw1);
   if (REV) {
     if (yvec)
       store_window32<TIO, true>(yr + (a.T - w.t0w - kWin32), stage, v, 1.0f);
@@ -580,6 +640,31 @@ __device__ __forceinline__ void window32(const FwdArgs& a, Win32<NWV>& w, u32x4*
   } else {
     store_lane32<TIO>(yr + t0, false, v, 1.0f);
   }
+}
+
+// Chained form, wave 0: the group's incoming state of every section into
+// LDS: zi (or zeros) for the first group, else the inclusive states window
+// kg-1 published; prev_ok = 0 where a word did not carry the current flag
+// yet (section_carry then waits for it).
+template <int NWV>
+__device__ __forceinline__ void read_group_in(const FwdArgs& a, const Win32<NWV>& w,
+                                              double* prev, int* prev_ok) {
+  const int lane = lane_now();
+  const int S = a.S;
+  if (w.wv != 0 || lane >= S) return;
+  double p0 = 0.0, p1 = 0.0;
+  int ok = 1;
+  if (w.kg > 0) {
+    const uint64_t* st = a.status + (((w.kg - 1) * a.B + w.c) * S + lane) * kStatusWords;
+    ok = read_pair(st + 4, w.flag, p0, p1) ? 1 : 0;
+  } else if (w.has_zi) {
+    const float* zp = static_cast<const float*>(a.zi) + (w.zi_row * S + lane) * 2;
+    p0 = static_cast<double>(zp[0]);
+    p1 = static_cast<double>(zp[1]);
+  }
+  prev_ok[lane] = ok;
+  prev[2 * lane] = p0;
+  prev[2 * lane + 1] = p1;
 }
 
 // Full windows of every channel, NWV consecutive windows (a group) per
@@ -608,7 +693,6 @@ __global__ __launch_bounds__(64 * NWV) void sos_fwd32_kernel(FwdArgs a, const Fa
   constexpr int kAuxV = (2 * kWave * 8 + kWave * 4 + 2 * NWV * 2 * 8 + 2 * 2 * 8) / 16;
   static_assert(NWV * kStageV + kAuxV <= kSmemV, "shared memory layout");
   __shared__ u32x4 smem[kSmemV];
-  const int lane = threadIdx.x & (kWave - 1);
   const int S = a.S;
   Win32<NWV> w;
   w.wv = threadIdx.x >> 6;
@@ -627,13 +711,8 @@ __global__ __launch_bounds__(64 * NWV) void sos_fwd32_kernel(FwdArgs a, const Fa
   w.prev_ok = prev_ok;
   w.wagg = reinterpret_cast<double*>(prev_ok + kWave);
   w.sgrp = w.wagg + 2 * NWV * 2;
-  if (!LB && w.wv == 0 && w.kg > 0 && lane < S) {
-    const uint64_t* st = a.status + (((w.kg - 1) * a.B + w.c) * S + lane) * kStatusWords;
-    double p0, p1;
-    prev_ok[lane] = read_pair(st + 4, launch_flag(a.hdr, 0), p0, p1) ? 1 : 0;
-    prev[2 * lane] = p0;
-    prev[2 * lane + 1] = p1;
-  }
+  w.flag = launch_flag(a.hdr, 0);
+  if (!LB) read_group_in<NWV>(a, w, prev, prev_ok);
   __syncthreads();
   u32x4* stage = smem + w.wv * kStageV;
   if (w.ff->unit != 0)
