@@ -3,14 +3,15 @@ against the FP64 oracle (GPU).
 
 FP32 and BF16 I/O with up to 64 sections select this kernel: with at least
 1024 channels the 16384-sample groups of a channel are chained, with fewer
-each group looks back over the groups before it.  Covers:
+the 32768-sample groups (and the leftover windows) look back over the
+groups before them.  Covers:
 filters whose b0 can be normalised (first input tap 1, running b0 product
 restored on output) and filters that keep b0 as a tap (a zero b0, or an
 overall gain far from 1), mixed in one per-channel batch; lengths around the
 window (4096 samples) boundary; zi per channel and broadcast; zf; input rows
 shared by several channels; rows that are not 16-byte aligned; time reversal;
-the saved section outputs of the training forward; few channels with more
-than 64 predecessor groups (several look-back rounds).
+the saved section outputs of the training forward; few channels with many
+predecessor groups (several look-back rounds of 4 groups each).
 
 Tolerances are per-channel normwise relative errors max|y - y64| / max|y64|
 against the FP64 oracle run on the same (rounded) coefficients and inputs:
@@ -216,10 +217,10 @@ def test_error_next_to_channel_persistent_kernel(oracle_mod):
 
 @pytest.mark.parametrize("reverse", [False, True])
 def test_few_channels_lookback(oracle_mod, reverse):
-    """5 channels of 70 groups of 16384 samples, two more full windows and a
-    partial one: the look-back combines 64 predecessors per round, so group
-    69 needs two rounds.  Per-channel filters, two of them keeping b0 as a
-    tap; zi and zf."""
+    """5 channels of 282 full windows (35 groups of 8 windows, 2 leftover
+    windows with their own look-back) and a partial one; a look-back round
+    inspects 4 predecessors, so late groups can need several rounds.
+    Per-channel filters, two of them keeping b0 as a tap; zi and zf."""
     dev = _dev()
     B, S = 5, 4
     T = 70 * 16384 + 2 * 4096 + 37

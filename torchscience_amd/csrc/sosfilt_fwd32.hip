@@ -312,22 +312,35 @@ struct Win32 {
   double* sgrp;        // (2, 2): the group's incoming state (look-back form)
 };
 
+// log2 of the windows per group (1, 2, 4 or 8): a group spans A^(4096 NWV),
+// table power 6 + log2(NWV).
+template <int NWV>
+constexpr int kLog2Nwv = NWV == 8 ? 3 : (NWV == 4 ? 2 : (NWV == 2 ? 1 : 0));
+// Predecessor groups inspected per look-back round.  The status words are
+// read with coherent loads that go past the XCD's L2; their number, more than
+// their latency, bounds the look-back form (long signals, few channels):
+// 4 per round measured faster than 8, 16 or 64.
+constexpr int kLbLanes = 4;
+constexpr int kLbLog2 = 2;
+static_assert((1 << kLbLog2) == kLbLanes, "look-back lanes");
+
 // Decoupled look-back over the groups before group gi of channel c (NWV
 // windows each; a group's status words sit at the index of its last window):
-// the incoming state of group gi, the same value in every lane.  Lane l
-// inspects group gi-1-l; the nearest group with an inclusive state (or the
-// virtual group -1 holding zi) ends the search, groups in between contribute
-// their aggregates,
+// the incoming state of group gi, the same value in every lane.  Per round,
+// lane l < kLbLanes inspects group gi-1-l; the nearest group with an
+// inclusive state (or the virtual group -1 holding zi) ends the search,
+// groups in between contribute their aggregates,
 //   in = sum_{l < f} Mg^l agg_(gi-1-l) + Mg^f incl_(gi-1-f),
-// Mg = A^(4096 NWV) = table power 6 + log2(NWV), combined by a shuffle-down
-// reduction with the powers Mg^(2^i).  A wait longer than kSpinLimit polls
-// sets the error word and gives up, so every wave terminates.
+// Mg = A^(4096 NWV), combined by a shuffle-down reduction with the powers
+// Mg^(2^i); without an inclusive state in the round the next kLbLanes groups
+// follow, scaled by Mg^kLbLanes.  A wait longer than kSpinLimit polls sets
+// the error word and gives up, so every wave terminates.
 template <int NWV>
 __device__ __forceinline__ void lookback_groups(const uint64_t* status, int64_t B, int S,
                                                 WsHeader* hdr, int64_t gi, int64_t c, int s,
                                                 uint32_t flag, const double* tb, double zi0,
                                                 double zi1, double& sw0, double& sw1) {
-  constexpr int kPbase = 6 + (NWV == 4 ? 2 : (NWV == 2 ? 1 : 0));
+  constexpr int kPbase = 6 + kLog2Nwv<NWV>;
   const int lane = lane_now();
   double acc0 = 0.0, acc1 = 0.0;
   Mat2 P{1.0, 0.0, 0.0, 1.0};
@@ -337,7 +350,9 @@ __device__ __forceinline__ void lookback_groups(const uint64_t* status, int64_t 
     const int64_t j = base - lane;
     int st = 2;
     double v0 = 0.0, v1 = 0.0;
-    if (j >= 0) {
+    if (lane >= kLbLanes) {
+      st = 1;  // outside this round: ready, contributes nothing
+    } else if (j >= 0) {
       const uint64_t* e = status + (((j * NWV + NWV - 1) * B + c) * S + s) * kStatusWords;
       if (read_pair(e + 4, flag, v0, v1))
         st = 2;
@@ -348,6 +363,8 @@ __device__ __forceinline__ void lookback_groups(const uint64_t* status, int64_t 
     } else if (j == -1) {
       v0 = zi0;
       v1 = zi1;
+    } else {
+      st = 1;  // before the first group: nothing
     }
     const uint64_t incl = __ballot(st == 2);
     const uint64_t ready = __ballot(st != 0);
@@ -366,7 +383,7 @@ __device__ __forceinline__ void lookback_groups(const uint64_t* status, int64_t 
       v1 = 0.0;
     }
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
+    for (int i = 0; i < kLbLog2; ++i) {
       const int d = 1 << i;
       const double n0 = __shfl_down(v0, d);
       const double n1 = __shfl_down(v1, d);
@@ -381,8 +398,8 @@ __device__ __forceinline__ void lookback_groups(const uint64_t* status, int64_t 
     acc0 += P.a * r0 + P.b * r1;
     acc1 += P.c * r0 + P.d * r1;
     if (f < 64) break;
-    P = mat_mul(P, load_mat(table_pow(tb, kPbase + 6)));
-    base -= kWave;
+    P = mat_mul(P, load_mat(table_pow(tb, kPbase + kLbLog2)));
+    base -= kLbLanes;
   }
   sw0 = acc0;
   sw1 = acc1;
@@ -465,7 +482,7 @@ __device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32<NWV>
       if (w.kg > 0)
         lookback_groups<NWV>(a.status, a.B, S, a.hdr, w.kg / NWV, w.c, s, flag, tb, zi0, zi1,
                              sw0, sw1);
-      const Mat2 Mg = load_mat(table_pow(tb, 6 + (NWV == 4 ? 2 : (NWV == 2 ? 1 : 0))));
+      const Mat2 Mg = load_mat(table_pow(tb, 6 + kLog2Nwv<NWV>));
       const double i0 = fma(Mg.a, sw0, fma(Mg.b, sw1, g0));
       const double i1 = fma(Mg.c, sw0, fma(Mg.d, sw1, g1));
       publish_pair(st + 4, i0, i1, flag, lane);
@@ -853,22 +870,27 @@ static void launch_fwd32_io(const FwdArgs& a, const Fast32* fast, int64_t groups
                        fast, a.tables, k0);
 }
 
-// Groups of four windows, chained (many channels) or with the look-back
-// (fewer than kChainedChannels channels: a group's predecessor runs at the
-// same time), then the leftover full windows one per workgroup, chained to
-// the last group (their predecessors in the same launch are few).
+// Many channels: groups of four windows, chained, then the leftover full
+// windows one per workgroup, chained to the last group.  Fewer than
+// kChainedChannels channels (a group's predecessor runs at the same time):
+// groups of eight windows with the look-back (half the look-backs per
+// sample of four-window groups; 0.32 against 0.43 ms on 8 x 2^24 samples),
+// then the leftover full windows one per workgroup, also with the look-back
+// (a chain of them would run one after another).
 template <typename TIO, bool REV>
 static void launch_fwd32_windows(const FwdArgs& a, const Fast32* fast, hipStream_t stream) {
   const int64_t full = a.T / kWin32;
-  const int64_t n4 = full / 4;
-  if (n4 > 0) {
-    if (a.B >= kChainedChannels)
-      launch_fwd32_io<TIO, REV, 4, false>(a, fast, n4 * a.B, 0, stream);
-    else
-      launch_fwd32_io<TIO, REV, 4, true>(a, fast, n4 * a.B, 0, stream);
+  if (a.B >= kChainedChannels) {
+    const int64_t n4 = full / 4;
+    if (n4 > 0) launch_fwd32_io<TIO, REV, 4, false>(a, fast, n4 * a.B, 0, stream);
+    if (full > 4 * n4)
+      launch_fwd32_io<TIO, REV, 1, false>(a, fast, (full - 4 * n4) * a.B, 4 * n4, stream);
+  } else {
+    const int64_t n8 = full / 8;
+    if (n8 > 0) launch_fwd32_io<TIO, REV, 8, true>(a, fast, n8 * a.B, 0, stream);
+    if (full > 8 * n8)
+      launch_fwd32_io<TIO, REV, 1, true>(a, fast, (full - 8 * n8) * a.B, 8 * n8, stream);
   }
-  if (full > 4 * n4)
-    launch_fwd32_io<TIO, REV, 1, false>(a, fast, (full - 4 * n4) * a.B, 4 * n4, stream);
 }
 
 tsiir_status launch_fwd32_full(const FwdArgs& a, const void* sos, int coef_dtype,
