@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs scaled by the same counters over a 1 GiB copy
 # (profiles/sos_fwd32_c2a_pmc.txt)
-TRAFFIC_BYTES = {"c2a": 2155991426}
+TRAFFIC_BYTES = {"c2a": 2156240896}
 
 
 def parse():
