@@ -455,32 +455,45 @@ __global__ __launch_bounds__(64) void sos_adjoint_fused_kernel(AdjFArgs a, int64
     store_rev<T, W>(static_cast<T*>(a.dx) + c * a.T, n_hi, false, m, G);
 }
 
-// dsos[f, s, :] from the per-(window, channel) partial sums.
-template <typename TC>
-__global__ __launch_bounds__(256) void dsos_reduce_kernel(const double* partial,
-                                                          const double* tables, TC* dsos,
-                                                          int S, int64_t B, int64_t cpf,
-                                                          int64_t n_windows) {
+// dsos[f, s, :] from the per-(window, channel) partial sums: one block of NT
+// threads per (filter, section).  Thread t sums channels t, t + NT, ... of
+// every window (no index division), four windows' loads in flight at a
+// time, then a shared-memory tree; the order of the sums is fixed, so the
+// result is deterministic.
+template <typename TC, int NT>
+__global__ __launch_bounds__(NT) void dsos_reduce_kernel(const double* partial,
+                                                         const double* tables, TC* dsos,
+                                                         int S, int64_t B, int64_t cpf,
+                                                         int64_t n_windows) {
   const int64_t fs = blockIdx.x;  // f * S + s
   const int64_t f = fs / S;
   const int s = static_cast<int>(fs % S);
   const int64_t c_lo = f * cpf;
   const int64_t c_hi = (f + 1) * cpf < B ? (f + 1) * cpf : B;
-  const int64_t nc = c_hi - c_lo;
   const double* ps = partial + static_cast<int64_t>(s) * n_windows * B * 5;
   double acc[5] = {0, 0, 0, 0, 0};
-  for (int64_t it = threadIdx.x; it < nc * n_windows; it += blockDim.x) {
-    const int64_t kk = it / nc;
-    const int64_t cc = c_lo + it % nc;
-    const double* p = ps + (kk * B + cc) * 5;
+  for (int64_t c = c_lo + threadIdx.x; c < c_hi; c += NT) {
+    int64_t kk = 0;
+    for (; kk + 4 <= n_windows; kk += 4) {
+      double v[4][5];
 #pragma unroll
-    for (int i = 0; i < 5; ++i) acc[i] += p[i];
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int i = 0; i < 5; ++i) v[u][i] = ps[((kk + u) * B + c) * 5 + i];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int i = 0; i < 5; ++i) acc[i] += v[u][i];
+    }
+    for (; kk < n_windows; ++kk)
+#pragma unroll
+      for (int i = 0; i < 5; ++i) acc[i] += ps[(kk * B + c) * 5 + i];
   }
-  __shared__ double sh[5][256];
+  __shared__ double sh[5][NT];
 #pragma unroll
   for (int i = 0; i < 5; ++i) sh[i][threadIdx.x] = acc[i];
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
+  for (int w = NT / 2; w > 0; w >>= 1) {
     if (static_cast<int>(threadIdx.x) < w) {
 #pragma unroll
       for (int i = 0; i < 5; ++i) sh[i][threadIdx.x] += sh[i][threadIdx.x + w];
@@ -510,6 +523,24 @@ __global__ __launch_bounds__(256) void dsos_reduce_kernel(const double* partial,
     }
     for (int i = 0; i < 6; ++i) dsos[fs * 6 + i] = static_cast<TC>(out[i]);
   }
+}
+
+// Block size by channels per filter: wide blocks for shared filters, narrow
+// ones for per-channel filters (many rows, one channel each).
+template <typename TC>
+static void launch_dsos_reduce(const double* partial, const double* tables, TC* dsos, int S,
+                               int64_t B, int64_t cpf, int64_t n_windows, int64_t n_rows,
+                               hipStream_t st) {
+  const dim3 grid(static_cast<unsigned>(n_rows));
+  if (cpf >= 1024)
+    hipLaunchKernelGGL((dsos_reduce_kernel<TC, 1024>), grid, dim3(1024), 0, st, partial, tables,
+                       dsos, S, B, cpf, n_windows);
+  else if (cpf >= 128)
+    hipLaunchKernelGGL((dsos_reduce_kernel<TC, 256>), grid, dim3(256), 0, st, partial, tables,
+                       dsos, S, B, cpf, n_windows);
+  else
+    hipLaunchKernelGGL((dsos_reduce_kernel<TC, 64>), grid, dim3(64), 0, st, partial, tables, dsos,
+                       S, B, cpf, n_windows);
 }
 
 template <typename T>
@@ -682,13 +713,11 @@ static tsiir_status backward_typed(const tsiir_sosfilt_desc* d, const BwdLayout&
   // 4. coefficient gradients
   if (dsos != nullptr) {
     if (d->coef_dtype == TSIIR_F64)
-      hipLaunchKernelGGL(dsos_reduce_kernel<double>, dim3(static_cast<unsigned>(n_rows)),
-                         dim3(256), 0, st, partial, adj_tables, static_cast<double*>(dsos), S, B,
-                         d->chans_per_filter, L.n_windows);
+      launch_dsos_reduce<double>(partial, adj_tables, static_cast<double*>(dsos), S, B,
+                                 d->chans_per_filter, L.n_windows, n_rows, st);
     else
-      hipLaunchKernelGGL(dsos_reduce_kernel<float>, dim3(static_cast<unsigned>(n_rows)),
-                         dim3(256), 0, st, partial, adj_tables, static_cast<float*>(dsos), S, B,
-                         d->chans_per_filter, L.n_windows);
+      launch_dsos_reduce<float>(partial, adj_tables, static_cast<float*>(dsos), S, B,
+                                d->chans_per_filter, L.n_windows, n_rows, st);
   }
   return hipGetLastError() == hipSuccess ? TSIIR_OK : TSIIR_ELAUNCH;
 }
