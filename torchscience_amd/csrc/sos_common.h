@@ -506,7 +506,7 @@ __device__ __forceinline__ void prefetch_window(const TIO* win,
     for (int i = 0; i < L::kH; ++i) pf[r * L::kH + i] = g[L::gvec(r, i, lane)];
 }
 
-template <typename TIO, typename TACC, int W>
+template <typename TIO, typename TACC, int W, bool REV = false>
 __device__ __forceinline__ void transpose_window_lds(const u32x4 (&pf)[LdsStage<TIO, W>::kNV],
                                                      TIO* lds, TACC (&v)[W]) {
   using L = LdsStage<TIO, W>;
@@ -517,7 +517,7 @@ __device__ __forceinline__ void transpose_window_lds(const u32x4 (&pf)[LdsStage<
 #pragma unroll
     for (int i = 0; i < L::kH; ++i) lv[L::lvec(i, lane)] = pf[r * L::kH + i];
     __syncthreads();
-    lds_rows_to_regs<TIO, TACC, W>(lds, r, v);
+    lds_rows_to_regs<TIO, TACC, W, REV>(lds, r, v);
     __syncthreads();
   }
 }

@@ -308,11 +308,21 @@ __global__ __launch_bounds__(64) void sos_adjoint_fused_kernel(AdjFArgs a, int64
     pok = read_pair(st + 4, flag, pv0, pv1);
   }
 
+  // vector path: y (the output of the last section) is loaded with dy, and
+  // the input window of the next pending section is loaded while the current
+  // section sweeps, so a window waits for HBM once instead of once per section
   T G[W], Up[W], Vp[W];
-  if (vec)
+  u32x4 pf[LdsStage<T, W>::kNV];
+  if (vec) {
+    if (S >= 2) prefetch_window<T, W>(sig_row<T>(a, S - 1, c) + t_lo, pf);
     load_window_lds<T, T, W, true>(static_cast<const T*>(a.dy) + c * a.T + t_lo, lds, G);
-  else
+    if (S >= 2) {
+      transpose_window_lds<T, T, W, true>(pf, lds, Vp);
+      prefetch_window<T, W>(sig_row<T>(a, S - 2, c) + t_lo, pf);
+    }
+  } else {
     load_rev<T, W>(static_cast<const T*>(a.dy) + c * a.T, n_hi, false, m, G);
+  }
 
   // pending section p = s + 1: lane incoming state, coefficients, table
   T q1 = 0, q2 = 0;
@@ -323,18 +333,17 @@ __global__ __launch_bounds__(64) void sos_adjoint_fused_kernel(AdjFArgs a, int64
     if (pending) {
       // u_p = output of section s, v_p = output of section p = s + 1
       if (s == S - 2) {
-        if (vec)
-          load_window_lds<T, T, W, true>(sig_row<T>(a, S - 1, c) + t_lo, lds, Vp);
-        else
-          load_rev<T, W>(sig_row<T>(a, S - 1, c), n_hi, false, m, Vp);
+        if (!vec) load_rev<T, W>(sig_row<T>(a, S - 1, c), n_hi, false, m, Vp);
       } else {
 #pragma unroll
         for (int j = 0; j < W; ++j) Vp[j] = Up[j];
       }
-      if (vec)
-        load_window_lds<T, T, W, true>(sig_row<T>(a, s, c) + t_lo, lds, Up);
-      else
+      if (vec) {
+        transpose_window_lds<T, T, W, true>(pf, lds, Up);
+        if (s >= 0) prefetch_window<T, W>(sig_row<T>(a, s - 1, c) + t_lo, pf);
+      } else {
         load_rev<T, W>(sig_row<T>(a, s, c), n_hi, false, m, Up);
+      }
     }
     const double* tb = tf + static_cast<int64_t>(s < 0 ? 0 : s) * kTableStride;
     const T a1 = static_cast<T>(tb[3]), a2 = static_cast<T>(tb[4]);
