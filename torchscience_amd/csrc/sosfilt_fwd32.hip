@@ -729,23 +729,30 @@ __global__ __launch_bounds__(64 * NWV) void sos_fwd32_kernel(FwdArgs a, const Fa
     window32<TIO, REV, SAVED, false, NWV, LB>(a, w, stage);
 }
 
-// One 64-lane group (one wave) per (filter, section) row: both the FP64
+// Sixteen lanes per (filter, section) row, four rows per wave: both the FP64
 // table entry the kernels of sosfilt_fwd.hip read for 64-sample lanes (the
-// values sos_prep_kernel writes) and the Fast32 entry (see sos_common.h).
-// Lane j computes its own matrix powers, by repeated squaring (A^(64 2^j),
-// the same operation sequence as the one-thread prep) or binary powering
-// (M^j and the zero-input response pair (g[j+1], g[j]) = C^j (1, 0), C the
-// companion matrix), so no lane walks a long product chain.  Coefficients are
+// layout sos_prep_kernel writes) and the Fast32 entry (see sos_common.h).
+// Lane l < 15 forms the table power A^(64 2^l) by repeated squaring (the
+// operation sequence of the one-thread prep).  Lane l covers entries
+// j = l + 16k (k < 4) of the other two tables, so each store instruction
+// writes one contiguous run per row: the zero-input response from the pair
+// (g[l+1], g[l]) = C^l (1, 0) (C the companion matrix, binary powering),
+// stepped by C^16; M^l by binary powering of M = A^64, stepped by M^16.
+// Coefficients are
 // normalised by a0 in the coefficient type, then promoted.  The first thread
 // advances the workspace epoch (one launch publishes status words after it).
+constexpr int kPrepLanes = 16;
+constexpr int kPrepRowsPerBlock = 256 / kPrepLanes;
+
 template <typename TC>
 __global__ __launch_bounds__(256) void sos_prep32_kernel(const TC* __restrict__ sos,
                                                          int64_t n_rows, int S,
                                                          double* __restrict__ tables,
                                                          Fast32* __restrict__ fast,
                                                          WsHeader* hdr) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
-  const int j = threadIdx.x & (kWave - 1);
+  const int64_t i =
+      static_cast<int64_t>(blockIdx.x) * kPrepRowsPerBlock + (threadIdx.x / kPrepLanes);
+  const int l = threadIdx.x % kPrepLanes;
   if (blockIdx.x == 0 && threadIdx.x == 0 && hdr != nullptr) hdr->epoch += 1;
   if (i >= n_rows) return;
   const int64_t f = i / S;
@@ -776,7 +783,7 @@ __global__ __launch_bounds__(256) void sos_prep32_kernel(const TC* __restrict__ 
   // FP64 table entry
   double* t = tables + i * kTableStride;
   const Mat2 A{-da1, 1.0, -da2, 0.0};
-  if (j == 0) {
+  if (l == 0) {
     t[0] = db0;
     t[1] = db1;
     t[2] = db2;
@@ -785,10 +792,11 @@ __global__ __launch_bounds__(256) void sos_prep32_kernel(const TC* __restrict__ 
     t[5] = static_cast<double>(a0);
     t[6] = t[7] = 0.0;
   }
-  if (j < kNPow) {  // A^(64 * 2^j)
+  static_assert(kNPow <= kPrepLanes, "one table power per lane");
+  if (l < kNPow) {  // A^(64 * 2^l)
     Mat2 Q = A;
-    for (int l = 0; l < 6 + j; ++l) Q = mat_mul(Q, Q);
-    double* q = t + 8 + 4 * j;
+    for (int k = 0; k < 6 + l; ++k) Q = mat_mul(Q, Q);
+    double* q = t + 8 + 4 * l;
     q[0] = Q.a;
     q[1] = Q.b;
     q[2] = Q.c;
@@ -798,23 +806,33 @@ __global__ __launch_bounds__(256) void sos_prep32_kernel(const TC* __restrict__ 
   const Mat2 C{-da1, -da2, 1.0, 0.0};
   Mat2 R{1.0, 0.0, 0.0, 1.0}, Pw = C;
 #pragma unroll
-  for (int bit = 0; bit < 6; ++bit) {
-    if ((j >> bit) & 1) R = mat_mul(R, Pw);
+  for (int bit = 0; bit < 4; ++bit) {
+    if ((l >> bit) & 1) R = mat_mul(R, Pw);
     Pw = mat_mul(Pw, Pw);
   }
+  // Pw = C^16; (gn, gm) = (g[j+1], g[j]) for j = l + 16k
   float* g = reinterpret_cast<float*>(t + kGOff);
-  g[j] = static_cast<float>(R.c);
-  if (j == kWave - 1) {
-    g[kWave] = static_cast<float>(R.a);
+  double gn = R.a, gm = R.c;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    g[l + kPrepLanes * k] = static_cast<float>(gm);
+    const double n1 = Pw.a * gn + Pw.b * gm;
+    const double m1 = Pw.c * gn + Pw.d * gm;
+    gn = n1;
+    gm = m1;
+  }
+  if (l == 0) {  // g[64] (now in gm of lane 0), then the table's pad
+    g[kWave] = static_cast<float>(gm);
     g[kWave + 1] = 0.0f;
   }
-  if (j < kGOff - (8 + 4 * kNPow)) t[8 + 4 * kNPow + j] = 0.0;
+  if (l < kGOff - (8 + 4 * kNPow)) t[8 + 4 * kNPow + l] = 0.0;
   constexpr int kGEnd = kGOff + (kGLen + 1) / 2;  // first double after g
-  if (kGEnd + j < kTableStride) t[kGEnd + j] = 0.0;
+  static_assert(kTableStride - kGEnd <= kPrepLanes, "tail zeros");
+  if (kGEnd + l < kTableStride) t[kGEnd + l] = 0.0;
 
   // Fast32 entry
   Fast32* o = fast + i;
-  if (j == 0) {
+  if (l == 0) {
     if (unit) {
       o->c[0] = 1.0f;
       o->c[1] = static_cast<float>(db1 / db0);
@@ -834,15 +852,20 @@ __global__ __launch_bounds__(256) void sos_prep32_kernel(const TC* __restrict__ 
     for (int p = 0; p < 6; ++p) o->pad[p] = 0.0f;
   }
   Mat2 M = A;
-  for (int l = 0; l < 6; ++l) M = mat_mul(M, M);  // A^64
+  for (int k = 0; k < 6; ++k) M = mat_mul(M, M);  // A^64
   Mat2 Q{1.0, 0.0, 0.0, 1.0};
 #pragma unroll
-  for (int bit = 0; bit < 6; ++bit) {
-    if ((j >> bit) & 1) Q = mat_mul(Q, M);
+  for (int bit = 0; bit < 4; ++bit) {
+    if ((l >> bit) & 1) Q = mat_mul(Q, M);
     M = mat_mul(M, M);
   }
-  o->mq[j] = make_float4(static_cast<float>(Q.a), static_cast<float>(Q.b),
-                         static_cast<float>(Q.c), static_cast<float>(Q.d));
+  // M = (A^64)^16
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    o->mq[l + kPrepLanes * k] = make_float4(static_cast<float>(Q.a), static_cast<float>(Q.b),
+                                            static_cast<float>(Q.c), static_cast<float>(Q.d));
+    if (k < 3) Q = mat_mul(Q, M);
+  }
 }
 
 }  // namespace
@@ -887,7 +910,8 @@ static void launch_fwd32_windows(const FwdArgs& a, const Fast32* fast, hipStream
 tsiir_status launch_fwd32_full(const FwdArgs& a, const void* sos, int coef_dtype,
                                int64_t n_filters, Fast32* fast, hipStream_t stream) {
   const int64_t n_rows = n_filters * a.S;
-  const unsigned pblocks = static_cast<unsigned>((n_rows + 3) / 4);
+  const unsigned pblocks =
+      static_cast<unsigned>((n_rows + kPrepRowsPerBlock - 1) / kPrepRowsPerBlock);
   double* tables = const_cast<double*>(a.tables);
   if (coef_dtype == TSIIR_F64)
     hipLaunchKernelGGL(sos_prep32_kernel<double>, dim3(pblocks), dim3(256), 0, stream,
