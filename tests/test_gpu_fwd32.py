@@ -242,3 +242,29 @@ def test_few_channels_lookback(oracle_mod, reverse):
     print("lookback relerr", err)
     assert err < F32_TOL
     np.testing.assert_allclose(zf.cpu().numpy(), zf64, rtol=1e-4, atol=1e-4 * np.abs(zf64).max())
+
+
+@pytest.mark.parametrize("B", [1030, 6])
+@pytest.mark.parametrize("S", [1, 17, 64])
+def test_many_sections(oracle_mod, S, B):
+    """Shared filter of S biquads (cutoffs U[0.3, 0.45]) on both forms
+    (chained groups at 1030 channels, look-back groups at 6), over full
+    groups, leftover windows and a partial window; zi and zf."""
+    from torchscience_amd.signal_processing.filter._design import butterworth_sos
+    dev = _dev()
+    rng = np.random.default_rng(100 * S + B)
+    sos = np.concatenate([butterworth_sos(2, float(c)) for c in rng.uniform(0.3, 0.45, S)])
+    sos = sos.astype(np.float32)
+    T = 2 * 32768 + 3 * 4096 + 100 if B < 1024 else 16384 + 3 * 4096 + 100
+    x = rng.standard_normal((B, T)).astype(np.float32)
+    zi = rng.standard_normal((B, S, 2)).astype(np.float32)
+    print("many sections S", S, "B", B, flush=True)
+    y, zf = _fwd(torch.tensor(sos[None], device=dev), torch.tensor(x, device=dev),
+                 torch.tensor(zi, device=dev), channels=B, chans_per_filter=B, want_zf=True)
+    y64, zf64 = oracle_mod.sosfilt_rows(sos.astype(np.float64), x.astype(np.float64),
+                                        zi.astype(np.float64), n_threads=8)
+    err = relerr(y.cpu().numpy(), y64)
+    print("relerr", err)
+    assert err < F32_TOL
+    np.testing.assert_allclose(zf.cpu().numpy(), zf64, rtol=1e-3,
+                               atol=1e-4 * max(1.0, np.abs(zf64).max()))
