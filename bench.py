@@ -122,14 +122,21 @@ def cpu_baseline(w, budget_s):
     n = max(probe, n // threads * threads)
     xs = w["x"][:n].float().cpu().numpy()
     sub = sos if sos.ndim == 2 else sos[:n]
+    # when the whole workload takes less than the budget, repeat it until the
+    # budget is spent (at least once) and report the mean rate
+    reps = 0
     t0 = time.perf_counter()
-    oracle.sosfilt_rows_f32io(sub, xs, chans_per_filter=1 if sos.ndim == 3 else None,
-                              n_threads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": n * T * S / dt / 1e9, "unit": "G sample·sections/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{n} of {B} channels x full T={T}, {S} sections, FP64 DF2T C oracle, "
-                      f"fp32 I/O, OpenMP {threads} threads, {dt:.1f} s"}
+    while True:
+        oracle.sosfilt_rows_f32io(sub, xs, chans_per_filter=1 if sos.ndim == 3 else None,
+                                  n_threads=threads)
+        reps += 1
+        dt = time.perf_counter() - t0
+        if n < B or dt >= budget_s:
+            break
+    return {"value": reps * n * T * S / dt / 1e9, "unit": "G sample·sections/s",
+            "cores": threads, "kind": "port",
+            "sample": f"{n} of {B} channels x full T={T}, {S} sections, {reps} pass(es), "
+                      f"FP64 DF2T C oracle, fp32 I/O, OpenMP {threads} threads, {dt:.1f} s"}
 
 
 def accuracy(y, x, sos_np, coef_dtype, n):
