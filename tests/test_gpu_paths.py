@@ -129,3 +129,29 @@ def test_conflicting_path_flags_rejected():
     rc = L.tsiir_sosfilt_forward(d, x.data_ptr(), x.data_ptr(), sos.data_ptr(), None, None,
                                  ws.data_ptr(), n, st)
     assert rc == 1  # TSIIR_EINVAL
+
+
+@pytest.mark.parametrize("io", [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("T", [4096, 4097, 3 * 4096 + 5, 20000])
+def test_compensated_many_channels(oracle_mod, io, T):
+    """The compensated form (FP32 or BF16 data, FP64 updates) with enough
+    channels for chained single-wave windows of 64 samples per lane; zi and
+    zf; lengths around the 4096-sample window."""
+    from torchscience_amd.signal_processing.filter._design import batched_butterworth_sos
+    from torchscience_amd.signal_processing.filter._ops import sosfilt_rows_forward
+    dev = _dev()
+    rng = np.random.default_rng(T + (io == torch.bfloat16))
+    B, S = 1030, 4
+    sos = batched_butterworth_sos(2 * S, np.array([rng.uniform(0.05, 0.45)]))
+    x = rng.standard_normal((B, T))
+    x_used = torch.tensor(x).to(io).double().numpy()
+    zi = rng.standard_normal((B, S, 2))
+    zi_used = torch.tensor(zi).float().double().numpy()
+    print("compensated many channels", io, "T", T, flush=True)
+    y, zf = sosfilt_rows_forward(torch.tensor(sos, device=dev), torch.tensor(x, device=dev).to(io),
+                                 torch.tensor(zi, device=dev).float(), channels=B,
+                                 chans_per_filter=B, want_zf=True, io_dtype=io, mode=1)
+    y64, zf64 = oracle_mod.sosfilt_rows(sos[0], x_used, zi_used, n_threads=8)
+    tol = 1e-5 if io == torch.float32 else 1e-2
+    assert relerr(y.double().cpu().numpy(), y64) < tol
+    np.testing.assert_allclose(zf.double().cpu().numpy(), zf64, rtol=1e-4, atol=1e-4)

@@ -636,9 +636,15 @@ __global__ void fill_u32_kernel(uint32_t* p, size_t n) {
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 // Samples per lane: 64 for FP32 recurrences, 32 when the recurrence (and so
-// the register footprint of its temporaries) is FP64.
+// the register footprint of its temporaries) is FP64.  The compensated form
+// (FP32 samples, FP64 updates) takes 64 in its chained single-wave windows
+// (many channels: 227 VGPRs, two waves per SIMD, half the scans per sample;
+// 7.39 against 8.28 ms on 2048 x 2^20 samples x 8 sections) and 32 in the
+// four-wave look-back windows.  The launch takes W from this geometry.
 static inline int samples_per_lane(const tsiir_sosfilt_desc* d) {
-  return (d->io_dtype == TSIIR_F64 || d->mode == TSIIR_MODE_COMPENSATED) ? 32 : 64;
+  if (d->io_dtype == TSIIR_F64) return 32;
+  if (d->mode == TSIIR_MODE_COMPENSATED) return fwd_policy(d).chained ? 64 : 32;
+  return 64;
 }
 
 
@@ -865,11 +871,16 @@ tsiir_status forward_impl(const tsiir_sosfilt_desc* d, const void* x, void* y, c
   }
   switch (d->io_dtype) {
     case TSIIR_F32:
-      return comp ? launch_fwd<float, float, double, 32>(d, g, x, y, sos, zi, zf, ws, st, saved)
-                  : launch_fwd<float, float, float, 64>(d, g, x, y, sos, zi, zf, ws, st, saved);
+      if (!comp) return launch_fwd<float, float, float, 64>(d, g, x, y, sos, zi, zf, ws, st, saved);
+      return g.W == 64
+                 ? launch_fwd<float, float, double, 64>(d, g, x, y, sos, zi, zf, ws, st, saved)
+                 : launch_fwd<float, float, double, 32>(d, g, x, y, sos, zi, zf, ws, st, saved);
     case TSIIR_BF16:
-      return comp ? launch_fwd<bf16_t, float, double, 32>(d, g, x, y, sos, zi, zf, ws, st, saved)
-                  : launch_fwd<bf16_t, float, float, 64>(d, g, x, y, sos, zi, zf, ws, st, saved);
+      if (!comp)
+        return launch_fwd<bf16_t, float, float, 64>(d, g, x, y, sos, zi, zf, ws, st, saved);
+      return g.W == 64
+                 ? launch_fwd<bf16_t, float, double, 64>(d, g, x, y, sos, zi, zf, ws, st, saved)
+                 : launch_fwd<bf16_t, float, double, 32>(d, g, x, y, sos, zi, zf, ws, st, saved);
     default:
       return launch_fwd<double, double, double, 32>(d, g, x, y, sos, zi, zf, ws, st, saved);
   }
