@@ -7,9 +7,11 @@
 // FP64 kernel in sosfilt_fwd.hip; this kernel computes the same linear
 // recurrence in direct form I, which takes one operation fewer per sample.
 //
-// Work decomposition (as in sosfilt_fwd.hip): one wavefront = one window of
-// 64 lanes x 64 consecutive samples of one channel, held in registers for the
-// whole cascade; workgroup i = k * B + c (window-major).  Per section:
+// Work decomposition: one wavefront = one window of 64 lanes x 64
+// consecutive samples of one channel, held in registers for the whole
+// cascade; a workgroup of NWV waves takes NWV consecutive windows of one
+// channel (a group: 4 windows with at least kChainedChannels channels, 8
+// with fewer), workgroup i = g * B + c (group-major).  Per section:
 //   1. zero-state sweep of the lane's samples, which first adds the pending
 //      zero-input response of the previous section (2 FMAs per sample, from
 //      the section's float table g) and then runs direct form I with the
@@ -25,15 +27,17 @@
 //      lane's zero-state end state in DF2T coordinates, M = A^64):
 //      Kogge-Stone inside each 16-lane row with DPP row shifts, then two
 //      DPP row broadcasts that carry the row totals across the wave.
-//   3. the window's incoming state, in FP64: zi, or the inclusive state of
-//      the window before (many channels: read once for all sections up
-//      front; few channels: decoupled look-back), composed with the window
-//      aggregate in FP64 and published for the windows after it.
-//   4. each lane's incoming state M^q s_window + exclusive_q (M^q per lane
+//   3. the group's incoming state, in FP64: zi, or the inclusive state of
+//      the group before (many channels: read once for all sections up
+//      front; few channels: decoupled look-back), composed through LDS with
+//      the wave aggregates of the group; the group's inclusive state is
+//      published for the groups after it.
+//   4. each lane's incoming state M^q s_wave + exclusive_q (M^q per lane
 //      from the table) becomes the pending zero-input response.
 // x is read from HBM once and y written once, through LDS with non-temporal
 // 16-byte accesses (an unpadded, XOR-swizzled 8 KB staging buffer per wave,
-// all 16 loads of a lane in flight at once).  Partial last windows run in the FP64-carry kernel of
+// all 16 loads of a lane in flight at once).  Leftover full windows run one
+// per workgroup; partial last windows run in the FP64-carry kernel of
 // sosfilt_fwd.hip, which reads the inclusive states published here (true,
 // unscaled DF2T states).
 
@@ -211,7 +215,8 @@ __device__ __forceinline__ void issue_window32(const TIO* win, RawWin<TIO>& raw)
 #pragma unroll
   for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int i = 0; i < L::kH; ++i) raw.r[r][i] = __builtin_nontemporal_load(gp + L::gvec(r, i, lane));
+    for (int i = 0; i < L::kH; ++i)
+      raw.r[r][i] = __builtin_nontemporal_load(gp + L::gvec(r, i, lane));
 }
 
 // Loaded vectors -> registers through the wave's staging buffer: lane q gets
@@ -692,7 +697,8 @@ __device__ __forceinline__ void read_group_in(const FwdArgs& a, const Win32<NWV>
 // along decreasing time (TSIIR_REVERSE); SAVED: keep the section outputs
 // (forward_saved).
 template <typename TIO, bool REV, bool SAVED, int NWV, bool LB>
-__global__ __launch_bounds__(64 * NWV) void sos_fwd32_kernel(FwdArgs a, const Fast32* __restrict__ fast,
+__global__ __launch_bounds__(64 * NWV) void sos_fwd32_kernel(FwdArgs a,
+                                                             const Fast32* __restrict__ fast,
                                                              const double* __restrict__ tables,
                                                              int64_t k0) {
   constexpr int kStageV = Stage32<TIO>::kBytes / 16;
