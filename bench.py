@@ -15,7 +15,9 @@ cpu_baseline  the FP64 C oracle (FP32 I/O, OpenMP over channels, all host
 
 Other workloads (--workload): c2b (per-channel order-8 lowpass filters),
 c2c (x [8, 2^24]), c5 (order-16 Chebyshev I, x [2048, 2^20], fp32 / bf16io),
-c3 (forward+backward of the differentiation workload).
+c3 (forward+backward of the differentiation workload), fir / fir64 (lfilter
+with a = [1]: 101-tap lowpass over x [4096, 65536], FP32 / FP64; its own
+metric, G sample*taps/s, with VALU and HBM rooflines).
 """
 
 from __future__ import annotations
@@ -45,7 +47,8 @@ def parse():
     p.add_argument("--steps", type=int, default=200)
     p.add_argument("--warmup", type=int, default=50)
     p.add_argument("--workload", default="c2a",
-                   choices=["c2a", "c2b", "c2c", "c5", "c5c", "c5bf16", "c3", "c4"])
+                   choices=["c2a", "c2b", "c2c", "c5", "c5c", "c5bf16", "c3", "c4", "fir",
+                            "fir64"])
     p.add_argument("--requests", type=int, default=10000)
     p.add_argument("--streams", type=int, default=2000)
     p.add_argument("--rate", type=float, default=20000.0, help="c4 Poisson arrivals per second")
@@ -169,6 +172,93 @@ def accuracy(y, x, sos_np, coef_dtype, n):
 ACCURACY_CHANNELS = {"c2a": 256, "c2b": 256, "c2c": 8, "c5": 64, "c5c": 64, "c5bf16": 64}
 
 
+# FP32 / FP64 vector FMA rate of MI355X without packed FP32 (all HIP code
+# here is built without packed-FP32 instruction selection): 256 CUs x 4 SIMD
+# x 16 lanes x 2 FLOP x 2.4 GHz.  The published vector FP32 peak (157.3)
+# counts packed FP32.
+VALU_PEAK_TFLOPS = 78.6
+
+
+def fir_bench(args, dev):
+    """lfilter(b, [1], x) for a 101-tap lowpass over x [4096, 65536]: one
+    step is one call over the whole batch (the direct-form FIR kernel)."""
+    import numpy as np
+    import torch
+    from torchscience_amd.signal_processing.filter import lfilter
+    from scipy.signal import firwin
+    dt = torch.float64 if args.workload == "fir64" else torch.float32
+    B, T, K = 4096, 65536, 101
+    g = torch.Generator(device=dev)
+    g.manual_seed(7)
+    x = torch.randn(B, T, device=dev, generator=g, dtype=dt)
+    h_np = firwin(K, 0.2)
+    h = torch.tensor(h_np, dtype=dt, device=dev)
+    one = torch.ones(1, dtype=dt, device=dev)
+
+    def step():
+        return lfilter(h, one, x)
+    y = None
+    for _ in range(args.warmup):
+        y = step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream(dev)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        y = step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = ev0.elapsed_time(ev1) / args.steps
+    ms = wall * 1e3 / args.steps
+    esz = x.element_size()
+    algo_bytes = B * T * esz * 2 + K * esz
+    flops = 2.0 * B * T * K
+    from oracle import oracle
+    rows = [0, 1, B // 2, B - 1]
+    errs = []
+    for c in rows:
+        ref, _ = oracle.fir_rows(h_np.astype(np.float32 if dt == torch.float32 else np.float64),
+                                 x[c:c + 1].cpu().numpy())
+        errs.append(float(np.max(np.abs(y[c:c + 1].double().cpu().numpy() - ref))
+                          / np.max(np.abs(ref))))
+    out = {
+        "metric": "FIR G sample·taps/s (1 GPU)",
+        "value": round(B * T * K / (ms * 1e-3) / 1e9, 2),
+        "unit": "G sample·taps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if dt == torch.float64 else "f32",
+        "data": "synthetic: x ~ N(0,1) from a seeded device generator; 101-tap Hamming "
+                "lowpass (cutoff 0.2 of Nyquist)",
+        "config": {"workload": f"lfilter FIR: x[{B},{T}], {K} taps, a=[1]", "channels": B,
+                   "samples": T, "taps": K, "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "achieved": round(flops / (ev_ms * 1e-3) / 1e12, 2),
+                     "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(flops / (ev_ms * 1e-3) / 1e12 / VALU_PEAK_TFLOPS, 4),
+                     "traffic": None, "event_ms_per_step": round(ev_ms, 4),
+                     "hbm_achieved_GBs": round(algo_bytes / (ev_ms * 1e-3) / 1e9, 1),
+                     "hbm_frac": round(algo_bytes / (ev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "accuracy": {"max_rel_vs_fp64_oracle": max(errs), "rows_checked": len(rows)},
+    }
+    if not args.no_cpu_baseline:
+        # single-threaded numpy convolution (the oracle) over whole rows
+        xs = x[:64].double().cpu().numpy()
+        n = 0
+        t0 = time.perf_counter()
+        while n < xs.shape[0] and time.perf_counter() - t0 < args.cpu_seconds:
+            oracle.fir_rows(h_np, xs[n:n + 1])
+            n += 1
+        dtc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n * T * K / dtc / 1e9, "unit": "G sample·taps/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"{n} of {B} rows x full T={T}, numpy.convolve "
+                                         f"(FP64), 1 thread, {dtc:.1f} s"}
+    return out
+
+
 def serving_bench(args, dev):
     """Serving workload: Poisson arrivals over persistent streams; the engine
     (continuous batching, device-resident state) against per-request plain
@@ -290,6 +380,9 @@ def main():
     dev = torch.device("cuda:0")
     if args.workload == "c4":
         print(json.dumps(serving_bench(args, dev)), flush=True)
+        return
+    if args.workload in ("fir", "fir64"):
+        print(json.dumps(fir_bench(args, dev)), flush=True)
         return
     w = make_workload(args.workload, dev)
     B, T, sos_np = w["B"], w["T"], w["sos"]

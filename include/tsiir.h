@@ -169,6 +169,52 @@ tsiir_status tsiir_sosfilt_backward_saved(const tsiir_sosfilt_desc* desc,
                                           void* dzi, void* workspace,
                                           size_t workspace_bytes, void* stream);
 
+/* ---- FIR filtering ----------------------------------------------------------
+ * y[n] = sum_{j<K} h[j] x[n + shift - j] for n in [0, n_samples), with x
+ * taken as zero outside [0, x_len).  Channel c reads input row c % x_rows and
+ * tap row c / chans_per_filter of `taps` (n_filters, n_taps) contiguous; all
+ * data (taps, x, y, zi, zf) is in io_dtype (F32 or F64). */
+#define TSIIR_FIR_REVERSE_TAPS 0x40u  /* use h[K-1-j] in place of h[j]       */
+
+typedef struct tsiir_fir_desc {
+  int64_t channels;          /* output channels B                          */
+  int64_t n_samples;         /* output samples per channel T               */
+  int64_t x_len;             /* input samples per row                      */
+  int32_t n_taps;            /* K (1 .. 2^20)                              */
+  int32_t io_dtype;          /* tsiir_dtype: F32 or F64                    */
+  int32_t shift;             /* output n reads inputs n + shift - j        */
+  uint32_t flags;            /* HAS_ZI, WRITE_ZF, ZI_BROADCAST, FIR_REVERSE_TAPS */
+  int64_t x_rows;            /* distinct input rows                        */
+  int64_t x_row_stride;      /* elements between input rows               */
+  int64_t y_row_stride;      /* elements between output rows              */
+  int64_t chans_per_filter;  /* channels that share one tap row (>= 1)     */
+} tsiir_fir_desc;
+
+/* FIR filtering.  With shift = 0, x_len = n_samples and the lfilter state
+ * flags it is lfilter(b, [a0], x, zi) for taps b / a0: zi / zf are
+ * (channels, K - 1), zi is added to the first K - 1 outputs and zf is the
+ * final DF2T state (the outputs K - 1 samples past the end).  Replaces the
+ * per-sample DF2T loop _lfilter.py:153-166 for a = [a0] (with b normalised as
+ * _lfilter.py:96-98 does), the per-filter FIR loop of batched_filter_apply
+ * _batched.py:533-561 (via x_rows / chans_per_filter), and the FFT
+ * convolution of fftfilt _fftfilt.py:60-92 (shift = (K - 1) / 2, no state).
+ * The state flags require shift == 0. */
+tsiir_status tsiir_fir_forward(const tsiir_fir_desc* desc, const void* taps, const void* x,
+                               void* y, const void* zi, void* zf, void* stream);
+
+/* Gradient of the taps: dtaps[f][j] = sum over the channels c of filter f of
+ * sum_{n < n_samples} g[c][n] x[c % x_rows][n + shift - j], x zero outside
+ * [0, x_len); g has row stride y_row_stride.  The input gradient needs no
+ * entry of its own: it is tsiir_fir_forward of g with TSIIR_FIR_REVERSE_TAPS
+ * and shift K - 1 - shift.  Together they replace the autograd tape the
+ * reference records through _lfilter.py:153-166 (a = [a0]) and through
+ * fftfilt's FFTs (_fftfilt.py:81-85).  Requires channels % chans_per_filter
+ * == 0 and fewer than 65536 filters; sums are accumulated in FP64. */
+size_t tsiir_fir_backward_taps_workspace_bytes(const tsiir_fir_desc* desc);
+tsiir_status tsiir_fir_backward_taps(const tsiir_fir_desc* desc, const void* x, const void* g,
+                                     void* dtaps, void* workspace, size_t workspace_bytes,
+                                     void* stream);
+
 /* ---- serving: ragged batches of streaming chunks --------------------------
  * The reference has no serving layer; these entry points serve the streaming
  * use of sosfilt with zi/zf (_sosfilt.py:98-133): a chunk of a persistent

@@ -20,7 +20,13 @@ It restates, in numpy over the C DF2T core in ``oracle.c``:
   crop);
 * ``filtfilt`` - _filtfilt.py:86-165 (a0 normalisation, padlen
   3 * max(len(a), len(b)), the same two passes with lfilter_zi, order-0
-  gain b0^2).
+  gain b0^2);
+* ``fir_rows`` - lfilter with a = [a0] (_lfilter.py:96-166: the DF2T loop
+  with a zero denominator tail is a convolution plus the state terms) and
+  the 'same' slice of fftfilt (_fftfilt.py:60-92), as full convolutions
+  (numpy.convolve) sliced at an offset;
+* ``fftfilt`` - _fftfilt.py:9-99 (axis move, 'same' slice from
+  (len(b) - 1) // 2).
 
 Deliberate difference from the reference: zi with ``axis != -1`` follows the
 documented meaning (zi batch dims are x's batch dims in order), not the
@@ -420,3 +426,56 @@ def impulse_state_response(sos_row: np.ndarray, n: int) -> np.ndarray:
         h[k] = C @ P @ Bv
         P = A @ P
     return h
+
+
+def fir_rows(taps: np.ndarray, x: np.ndarray, zi: np.ndarray | None = None, shift: int = 0,
+             want_zf: bool = False):
+    """FIR over rows of x (B, T) with taps (K,) or (B, K) per row:
+    y[n] = sum_j h[j] x[n + shift - j] (x zero outside the row), plus zi[n]
+    for n < K - 1; zf[k] = the full convolution at T + k plus zi[T + k]
+    (the DF2T final state of _lfilter.py:153-166 with a = [1]).
+    Returns (y, zf or None)."""
+    x = np.asarray(x, dtype=np.float64)
+    B, T = x.shape
+    h = np.asarray(taps, dtype=np.float64)
+    if h.ndim == 1:
+        h = np.broadcast_to(h, (B, h.size))
+    K = h.shape[1]
+    N = K - 1
+    if shift < 0:
+        raise ValueError("shift >= 0 only")
+    ext = np.zeros((B, T + N + shift))         # full convolution, zero-extended
+    if T > 0:
+        for i in range(B):
+            ext[i, :T + N] = np.convolve(x[i], h[i])
+    y = ext[:, shift:shift + T].copy()
+    zf = None
+    if zi is not None or want_zf:
+        if shift != 0:
+            raise ValueError("state terms need shift == 0")
+        tail = ext[:, T:T + N].copy()
+        if zi is not None:
+            z = np.broadcast_to(np.asarray(zi, dtype=np.float64), (B, N))
+            m = min(N, T)
+            y[:, :m] += z[:, :m]
+            if T < N:
+                tail[:, :N - T] += z[:, T:]
+        zf = tail
+    return y, zf
+
+
+def fftfilt(b, x, axis: int = -1):
+    """fftfilt with the reference's semantics (_fftfilt.py:9-99): the full
+    convolution of each row with b, sliced from (len(b) - 1) // 2."""
+    b = np.asarray(b)
+    x = np.asarray(x)
+    xm = np.moveaxis(x, axis, -1)
+    T = xm.shape[-1]
+    rows = xm.reshape(-1, T)
+    if np.iscomplexobj(rows):
+        yr, _ = fir_rows(b, np.real(rows), shift=(b.size - 1) // 2)
+        yi, _ = fir_rows(b, np.imag(rows), shift=(b.size - 1) // 2)
+        y = yr + 1j * yi
+    else:
+        y, _ = fir_rows(b, rows, shift=(b.size - 1) // 2)
+    return np.moveaxis(y.reshape(xm.shape), -1, axis)

@@ -6,6 +6,9 @@ _sosfilt.py:84-133 op for op: a0 normalisation only when a[0] != 1, the
 per-sample DF2T update as separate tensor ops vectorised over the batch,
 sections in order.  Run on the CPU, its autograd tape is the reference's
 backward (gradients reach x, b0..b2, a1, a2, zi; a0 gets 0 when a0 == 1).
+``fftfilt_fft`` restates _fftfilt.py:60-92 (real FFTs of the next power of
+two at or above the full length, product, inverse, 'same' slice) for the
+gradients of fftfilt.
 """
 
 from __future__ import annotations
@@ -52,3 +55,16 @@ def sosfilt_loop_timed(sos, x, n_samples):
     """The reference loop over the first n_samples of x (CPU baseline)."""
     with torch.no_grad():
         return sosfilt_loop(sos, x[:, :n_samples])[0]
+
+
+def fftfilt_fft(b, x):
+    """x (B, T) real, b (K,) real: the reference's FFT convolution, sliced."""
+    T = x.shape[-1]
+    K = b.shape[0]
+    n_fft = 1
+    while n_fft < T + K - 1:
+        n_fft *= 2
+    y_full = torch.fft.irfft(torch.fft.rfft(b, n=n_fft).unsqueeze(0)
+                             * torch.fft.rfft(x, n=n_fft), n=n_fft)
+    start = (K - 1) // 2
+    return y_full[:, start:start + T]

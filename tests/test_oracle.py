@@ -213,3 +213,81 @@ def test_oracle_zero_phase_properties(oracle_mod):
         oracle_mod.sosfiltfilt(sos, np.ones(15))
     with pytest.raises(ValueError, match="Unknown padtype"):
         oracle_mod.sosfiltfilt(sos, np.ones(200), padtype="wrap")
+
+
+@pytest.mark.parametrize("K,T", [(4, 1), (9, 5), (9, 300), (40, 39), (63, 700)])
+def test_fir_rows_matches_the_df2t_loop(oracle_mod, K, T):
+    """fir_rows (convolution plus state terms) equals the DF2T lfilter of
+    a = [1] (itself pinned by the scipy goldens), zi per row and shared,
+    including T < K where zi reaches zf."""
+    rng = np.random.default_rng(K * 1000 + T)
+    b = rng.standard_normal(K)
+    x = rng.standard_normal((3, T))
+    for zi in (rng.standard_normal((3, K - 1)), rng.standard_normal((1, K - 1))):
+        y, zf = oracle_mod.fir_rows(b, x, zi, want_zf=True)
+        y2, zf2 = oracle_mod.lfilter(b, np.ones(1), x, zi=np.broadcast_to(zi, (3, K - 1)))
+        np.testing.assert_allclose(y, y2, rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(zf, zf2, rtol=1e-12, atol=1e-12)
+    y, zf = oracle_mod.fir_rows(b, x, None, want_zf=True)
+    y2, zf2 = oracle_mod.lfilter(b, np.ones(1), x, zi=np.zeros((3, K - 1)))
+    np.testing.assert_allclose(zf, zf2, rtol=1e-12, atol=1e-12)
+
+
+def test_fftfilt_matches_scipy_fftconvolve(oracle_mod):
+    """The reference's own check (test__fftfilt.py:12-60, 102-160): fftfilt
+    equals scipy.signal.fftconvolve(x, b, mode='same'), odd and even taps,
+    batched rows and axis 0."""
+    import scipy.signal as ss
+    rng = np.random.default_rng(7)
+    for K, T in [(51, 1000), (11, 100), (201, 500), (20, 100), (2, 7), (1, 5)]:
+        b = ss.firwin(K, 0.3) if K > 2 else rng.standard_normal(K)
+        x = rng.standard_normal(T)
+        np.testing.assert_allclose(oracle_mod.fftfilt(b, x), ss.fftconvolve(x, b, mode="same"),
+                                   rtol=1e-10, atol=1e-10)
+    b = ss.firwin(21, 0.3)
+    x = rng.standard_normal((5, 8, 100))
+    ref = np.stack([ss.fftconvolve(r, b, mode="same") for r in x.reshape(-1, 100)])
+    np.testing.assert_allclose(oracle_mod.fftfilt(b, x).reshape(-1, 100), ref, rtol=1e-10,
+                               atol=1e-10)
+    x = rng.standard_normal((100, 50))
+    ref = np.stack([ss.fftconvolve(x[:, j], b, mode="same") for j in range(50)], axis=1)
+    np.testing.assert_allclose(oracle_mod.fftfilt(b, x, axis=0), ref, rtol=1e-10, atol=1e-10)
+
+
+def test_fftfilt_fft_restatement(oracle_mod):
+    """The torch FFT restatement of fftfilt (used for its gradients) equals
+    the oracle, and its autograd passes a finite-difference gradcheck."""
+    import torch
+    from oracle import torch_ref
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(21)
+    x = rng.standard_normal((2, 100))
+    y = torch_ref.fftfilt_fft(torch.tensor(b), torch.tensor(x)).numpy()
+    np.testing.assert_allclose(y, oracle_mod.fftfilt(b, x), rtol=1e-10, atol=1e-10)
+    bt = torch.tensor(b, requires_grad=True)
+    xt = torch.tensor(x, requires_grad=True)
+    assert torch.autograd.gradcheck(torch_ref.fftfilt_fft, (bt, xt), eps=1e-6, atol=1e-4,
+                                    rtol=1e-4)
+
+
+@pytest.mark.parametrize("mutation", ["shift_off_by_one", "drop_zi_tail", "reversed_taps"])
+def test_fir_planted_mistakes_are_caught(oracle_mod, mutation):
+    """A wrong FIR restatement (planted here, on the CPU) fails the pins."""
+    import scipy.signal as ss
+    rng = np.random.default_rng(11)
+    b = rng.standard_normal(8)
+    x = rng.standard_normal((2, 5))
+    zi = rng.standard_normal((2, 7))
+    if mutation == "shift_off_by_one":
+        y, _ = oracle_mod.fir_rows(b, x, shift=(b.size - 1) // 2 + 1)
+        ref = np.stack([ss.fftconvolve(r, b, mode="same") for r in x])
+        assert not np.allclose(y, ref, rtol=RTOL, atol=ATOL)
+    elif mutation == "drop_zi_tail":
+        y, zf = oracle_mod.fir_rows(b, x, np.zeros_like(zi), want_zf=True)
+        zf = zf + np.pad(zi[:, :x.shape[1]], ((0, 0), (0, 7 - x.shape[1])))  # wrong indices
+        _, zf2 = oracle_mod.lfilter(b, np.ones(1), x, zi=zi)
+        assert not np.allclose(zf, zf2, rtol=RTOL, atol=ATOL)
+    else:
+        y, _ = oracle_mod.fir_rows(b[::-1].copy(), x)
+        y2 = oracle_mod.lfilter(b, np.ones(1), x)
+        assert not np.allclose(y, y2, rtol=RTOL, atol=ATOL)

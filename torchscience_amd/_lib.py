@@ -21,6 +21,7 @@ F32, F64, BF16 = 0, 1, 2
 MODE_PLAIN, MODE_COMPENSATED = 0, 1
 HAS_ZI, WRITE_ZF, ZI_BROADCAST, REVERSE = 0x1, 0x2, 0x4, 0x8
 PATH_WINDOW, PATH_CHANNEL = 0x10, 0x20
+FIR_REVERSE_TAPS = 0x40
 
 # Every symbol include/tsiir.h declares.
 EXPORTED = (
@@ -35,6 +36,9 @@ EXPORTED = (
     "tsiir_sosfilt_forward_saved",
     "tsiir_sosfilt_backward_saved_workspace_bytes",
     "tsiir_sosfilt_backward_saved",
+    "tsiir_fir_forward",
+    "tsiir_fir_backward_taps_workspace_bytes",
+    "tsiir_fir_backward_taps",
     "tsiir_serve_table_bytes",
     "tsiir_serve_prepare_filter",
     "tsiir_serve_batch",
@@ -51,6 +55,22 @@ class SosfiltDesc(ctypes.Structure):
         ("mode", ctypes.c_int32),
         ("flags", ctypes.c_uint32),
         ("reserved", ctypes.c_int32),
+        ("x_rows", ctypes.c_int64),
+        ("x_row_stride", ctypes.c_int64),
+        ("y_row_stride", ctypes.c_int64),
+        ("chans_per_filter", ctypes.c_int64),
+    ]
+
+
+class FirDesc(ctypes.Structure):
+    _fields_ = [
+        ("channels", ctypes.c_int64),
+        ("n_samples", ctypes.c_int64),
+        ("x_len", ctypes.c_int64),
+        ("n_taps", ctypes.c_int32),
+        ("io_dtype", ctypes.c_int32),
+        ("shift", ctypes.c_int32),
+        ("flags", ctypes.c_uint32),
         ("x_rows", ctypes.c_int64),
         ("x_row_stride", ctypes.c_int64),
         ("y_row_stride", ctypes.c_int64),
@@ -111,6 +131,13 @@ def load(path: str | None = None) -> ctypes.CDLL:
     L.tsiir_sosfilt_backward_saved_workspace_bytes.argtypes = [dp]
     L.tsiir_sosfilt_backward_saved.restype = ctypes.c_int
     L.tsiir_sosfilt_backward_saved.argtypes = [dp] + [vp] * 11 + [sz, vp]
+    fp = ctypes.POINTER(FirDesc)
+    L.tsiir_fir_forward.restype = ctypes.c_int
+    L.tsiir_fir_forward.argtypes = [fp, vp, vp, vp, vp, vp, vp]
+    L.tsiir_fir_backward_taps_workspace_bytes.restype = sz
+    L.tsiir_fir_backward_taps_workspace_bytes.argtypes = [fp]
+    L.tsiir_fir_backward_taps.restype = ctypes.c_int
+    L.tsiir_fir_backward_taps.argtypes = [fp, vp, vp, vp, vp, sz, vp]
     L.tsiir_serve_table_bytes.restype = sz
     L.tsiir_serve_table_bytes.argtypes = [ctypes.c_int32]
     L.tsiir_serve_prepare_filter.restype = ctypes.c_int

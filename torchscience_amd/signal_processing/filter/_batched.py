@@ -11,18 +11,20 @@ c // chans_per_filter, so
 * broadcast=False (output (F, *x.shape[1:])): channel c = f * B' + b reads
   row c of x with filter f (chans_per_filter = B', x_rows = F * B').
 
-The FIR format (a = [1]) goes through lfilter per filter, as the reference
-does (_batched.py:533-561).
+The FIR format (filters (F, taps), a = [1], _batched.py:533-561) maps the
+same way onto one launch of the direct-form FIR kernel.
 """
 
 from __future__ import annotations
+
+import math
 
 from typing import Literal
 
 import torch
 from torch import Tensor
 
-from ._lfilter import lfilter
+from ._fir import fir_rows
 from ._ops import SosfiltRows, sosfilt_rows_forward
 
 
@@ -49,22 +51,11 @@ def batched_filter_apply(
         if filters.ndim != 2:
             raise ValueError(
                 f"FIR filters must have shape (batch, num_taps), got {filters.shape}")
-        outs = []
-        if broadcast:
-            for i in range(n_filters):
-                b = filters[i]
-                outs.append(lfilter(b, torch.ones(1, dtype=b.dtype, device=b.device), x,
-                                    axis=axis))
-        else:
-            if x.shape[0] != n_filters:
-                raise ValueError(
-                    f"When broadcast=False, x batch dim must match filters: "
-                    f"{x.shape[0]} != {n_filters}")
-            for i in range(n_filters):
-                b = filters[i]
-                outs.append(lfilter(b, torch.ones(1, dtype=b.dtype, device=b.device), x[i],
-                                    axis=axis))
-        return torch.stack(outs, dim=0)
+        if not broadcast and x.shape[0] != n_filters:
+            raise ValueError(
+                f"When broadcast=False, x batch dim must match filters: "
+                f"{x.shape[0]} != {n_filters}")
+        return _apply_fir(filters, x, axis, broadcast)
     raise ValueError(f"filter_format must be 'sos' or 'fir', got '{filter_format}'")
 
 
@@ -84,13 +75,13 @@ def _apply_sos(filters: Tensor, x: Tensor, axis: int, broadcast: bool) -> Tensor
     T = sig.shape[ax]
     if broadcast:
         xm = torch.movedim(x, ax, -1)
-        rows = xm.reshape(-1, T).to(comp).contiguous()
+        rows = xm.reshape(math.prod(xm.shape[:-1]), T).to(comp).contiguous()
         Bsig = rows.shape[0]
         channels, cpf = F * Bsig, Bsig
         out_shape = (F,) + tuple(xm.shape)
     else:
         xm = torch.movedim(x, ax + 1, -1)
-        rows = xm.reshape(-1, T).to(comp).contiguous()
+        rows = xm.reshape(math.prod(xm.shape[:-1]), T).to(comp).contiguous()
         Bsig = rows.shape[0] // F
         channels, cpf = F * Bsig, Bsig
         out_shape = tuple(xm.shape)
@@ -104,4 +95,39 @@ def _apply_sos(filters: Tensor, x: Tensor, axis: int, broadcast: bool) -> Tensor
                                     want_zf=False)
     y = y.reshape(out_shape)
     y = torch.movedim(y, -1, ax + 1)
+    return y.to(out)
+
+
+def _rows_of(x: Tensor, axis: int, broadcast: bool, F: int, comp: torch.dtype):
+    """(rows, T) contiguous signal rows, channels per filter and output shape."""
+    sig = x if broadcast else x[0]
+    ax = axis % sig.dim()
+    T = sig.shape[ax]
+    if broadcast:
+        xm = torch.movedim(x, ax, -1)
+        rows = xm.reshape(math.prod(xm.shape[:-1]), T).to(comp).contiguous()
+        return rows, rows.shape[0], (F,) + tuple(xm.shape), ax
+    xm = torch.movedim(x, ax + 1, -1)
+    rows = xm.reshape(math.prod(xm.shape[:-1]), T).to(comp).contiguous()
+    return rows, rows.shape[0] // F, tuple(xm.shape), ax
+
+
+def _apply_fir(filters: Tensor, x: Tensor, axis: int, broadcast: bool) -> Tensor:
+    F = filters.shape[0]
+    out = torch.promote_types(filters.dtype, x.dtype)
+    if out.is_complex or not out.is_floating_point:
+        from ._lfilter import lfilter
+        one = torch.ones(1, dtype=filters.dtype, device=filters.device)
+        if broadcast:
+            return torch.stack([lfilter(filters[i], one, x, axis=axis) for i in range(F)])
+        return torch.stack([lfilter(filters[i], one, x[i], axis=axis) for i in range(F)])
+    comp = torch.float64 if out == torch.float64 else torch.float32
+    rows, cpf, out_shape, ax = _rows_of(x, axis, broadcast, F, comp)
+    taps = filters.to(device=x.device, dtype=comp).contiguous()
+    channels = F * cpf
+    if channels == 0 or rows.shape[-1] == 0:
+        y = rows.new_zeros((channels, rows.shape[-1]))
+    else:
+        y, _ = fir_rows(taps, rows, None, channels=channels, chans_per_filter=cpf)
+    y = torch.movedim(y.reshape(out_shape), -1, ax + 1)
     return y.to(out)
