@@ -15,11 +15,12 @@
 //
 // Work decomposition: one workgroup of 256 lanes produces a tile of 2048
 // consecutive outputs of one channel.  The tile's input window (tile + taps
-// of one chunk) and the chunk's taps are staged in LDS once; each lane owns 8
-// consecutive outputs and walks the taps 8 at a time, reading one aligned
-// 8-sample vector of the window and one broadcast 8-tap vector per step
-// (both prefetched a step ahead) and doing 64 FMAs on them (a sliding
-// register window).  Filters longer than one chunk (1024 taps) restage.
+// of one chunk) is staged in LDS once; each lane owns 8 consecutive outputs
+// and walks the taps 8 at a time, reading one aligned 8-sample vector of the
+// window from LDS and 8 taps (FP32: a broadcast LDS vector; FP64: through
+// the scalar cache into SGPRs) per step, both prefetched a step ahead, and
+// doing 64 FMAs on them (a sliding register window).  Filters longer than
+// one chunk (1024 taps) restage.
 // The kernel is HBM-bound for short filters and VALU-bound from about 10
 // FLOP/byte (40 FP32 taps) on.
 //
@@ -85,19 +86,60 @@ struct FirArgs {
   uint32_t flags;
 };
 
+// 64 FMAs of one 8-tap step: acc[r] += h[q] * w[r - q + 7], w = lo ++ hi.
 template <typename T>
+__device__ __forceinline__ void fir_step(T (&acc)[kFirR], const T (&hq)[kFirR],
+                                         const T (&lo)[kFirR], const T (&hi)[kFirR]) {
+#pragma unroll
+  for (int q = 0; q < kFirR; ++q) {
+#pragma unroll
+    for (int r = 0; r < kFirR; ++r) {
+      const int u = r - q + kFirR - 1;
+      acc[r] = fma(hq[q], u < kFirR ? lo[u] : hi[u - kFirR], acc[r]);
+    }
+  }
+}
+
+// Taps jg .. jg + 7 of the (possibly reversed) filter; needs jg + 8 <= K.
+// The address is uniform, so these are contiguous scalar loads.
+template <typename T, bool REV>
+__device__ __forceinline__ void load_taps(const T* __restrict__ h, int K, int jg,
+                                          T (&hq)[kFirR]) {
+  const T* p = REV ? h + (K - kFirR - jg) : h + jg;
+#pragma unroll
+  for (int q = 0; q < kFirR; ++q) hq[q] = p[REV ? kFirR - 1 - q : q];
+}
+
+// Taps jg .. jg + 7, zero from K on (the last, partial block).
+template <typename T, bool REV>
+__device__ __forceinline__ void load_taps_tail(const T* __restrict__ h, int K, int jg,
+                                               T (&hq)[kFirR]) {
+#pragma unroll
+  for (int q = 0; q < kFirR; ++q) {
+    const int j = jg + q;
+    hq[q] = j < K ? h[REV ? K - 1 - j : j] : T(0);
+  }
+}
+
+// Taps through LDS (broadcast reads) for FP32, through the scalar cache into
+// SGPRs for FP64: FP64 window reads already take half of the LDS bandwidth
+// at the FMA rate, while FP32 runs faster with both operands from LDS.
+template <typename T>
+constexpr bool kTapsInLds = sizeof(T) == 4;
+
+template <typename T, bool REV>
 __global__ __launch_bounds__(kFirThreads) void fir_kernel(FirArgs<T> a) {
   constexpr int KC = kFirChunk;
-  // xs[kFirR + i] holds x[mbase + i] (the first kFirR slots absorb the
-  // prefetch past the last step); hs holds the chunk's taps, zero-padded.
+  constexpr bool TL = kTapsInLds<T>;
+  // xs[kFirR + i] holds x[mbase + i]; the first kFirR slots absorb the
+  // window prefetch past the last step.  hs: the chunk's taps, zero-padded.
   __shared__ __align__(16) T xs[kFirR + kFirTile + KC];
-  __shared__ __align__(16) T hs[KC + kFirR];
+  __shared__ __align__(16) T hs[TL ? KC + kFirR : 1];
   const int64_t blk = blockIdx.x;
   const int64_t c = blk / a.tiles;
   const int64_t n0 = (blk - c * a.tiles) * kFirTile;
   const T* __restrict__ h = a.taps + (c / a.cpf) * a.K;
   const T* __restrict__ xr = a.x + (c % a.x_rows) * a.x_stride;
-  const bool rev = a.flags & TSIIR_FIR_REVERSE_TAPS;
   const int t = threadIdx.x;
   T acc[kFirR];
 #pragma unroll
@@ -125,36 +167,59 @@ __global__ __launch_bounds__(kFirThreads) void fir_kernel(FirArgs<T> a) {
         xs[kFirR + i] = (m >= 0 && m < a.x_len) ? xr[m] : T(0);
       }
     }
-    for (int i = t; i < kcp + kFirR; i += kFirThreads) {
-      const int j = jc + i;
-      hs[i] = (i < kc) ? h[rev ? a.K - 1 - j : j] : T(0);
+    if (TL) {
+      for (int i = t; i < kcp + kFirR; i += kFirThreads) {
+        const int j = jc + i;
+        hs[i] = (i < kc) ? h[REV ? a.K - 1 - j : j] : T(0);
+      }
     }
     __syncthreads();
-    const T* xw = xs + kFirR + t * kFirR + kcp - kFirR;   // window start, step -8
-    T hi[kFirR], lo[kFirR], hq[kFirR];
-    lds_read8(xw + kFirR, hi);
-    lds_read8(xw, lo);
-    lds_read8(hs, hq);
-    for (int jl = 0; jl < kcp; jl += kFirR) {
-      T nlo[kFirR], nhq[kFirR];
-      lds_read8(xw - kFirR, nlo);
-      lds_read8(hs + jl + kFirR, nhq);
-      // window w[u] = xw[u], u = r - q + 7
-#pragma unroll
-      for (int q = 0; q < kFirR; ++q) {
-#pragma unroll
-        for (int r = 0; r < kFirR; ++r) {
-          const int u = r - q + kFirR - 1;
-          acc[r] = fma(hq[q], u < kFirR ? lo[u] : hi[u - kFirR], acc[r]);
+    // Step k uses the window xw - 8k (hi = 8 samples above it, lo = at it)
+    // and taps jc + 8k ..; three window buffers and two tap buffers rotate
+    // through a 6-step unrolled body, so no register moves are needed, and
+    // each step prefetches the next step's window and taps.  From LDS every
+    // step is a full (zero-padded) block.  From SGPRs full 8-tap blocks run
+    // there (the tap prefetch index is clamped into the filter; what it
+    // fetches past the last full block is not used) and a partial last block
+    // runs as one more step that reads its window afresh.
+    const T* xw = xs + kFirR + t * kFirR + kcp - kFirR;
+    const int nsteps = TL ? kcp / kFirR : kc / kFirR;
+    const int jmax = a.K - kFirR;   // >= 0 whenever a full block exists
+    if (nsteps > 0) {
+      T W0[kFirR], W1[kFirR], W2[kFirR], H0[kFirR], H1[kFirR];
+      lds_read8(xw + kFirR, W0);
+      lds_read8(xw, W1);
+      if (TL)
+        lds_read8(hs, H0);
+      else
+        load_taps<T, REV>(h, a.K, jc, H0);
+      for (int k = 0; k < nsteps; k += 6) {
+#define FIR_STEP(HI, LO, NEXT, HQ, HN, i)                                          \
+        if (k + (i) < nsteps) {                                                    \
+          lds_read8(xw - kFirR * ((i) + 1), NEXT);                                 \
+          if (TL)                                                                  \
+            lds_read8(hs + kFirR * (k + (i) + 1), HN);                             \
+          else                                                                     \
+            load_taps<T, REV>(h, a.K, min(jc + kFirR * (k + (i) + 1), jmax), HN);  \
+          fir_step(acc, HQ, LO, HI);                                               \
         }
+        FIR_STEP(W0, W1, W2, H0, H1, 0)
+        FIR_STEP(W1, W2, W0, H1, H0, 1)
+        FIR_STEP(W2, W0, W1, H0, H1, 2)
+        FIR_STEP(W0, W1, W2, H1, H0, 3)
+        FIR_STEP(W1, W2, W0, H0, H1, 4)
+        FIR_STEP(W2, W0, W1, H1, H0, 5)
+#undef FIR_STEP
+        xw -= 6 * kFirR;
       }
-#pragma unroll
-      for (int i = 0; i < kFirR; ++i) {
-        hi[i] = lo[i];
-        lo[i] = nlo[i];
-        hq[i] = nhq[i];
-      }
-      xw -= kFirR;
+    }
+    if (!TL && kc > nsteps * kFirR) {
+      const T* xt = xs + kFirR + t * kFirR + kcp - kFirR * (nsteps + 1);
+      T lo[kFirR], hi[kFirR], hq[kFirR];
+      lds_read8(xt + kFirR, hi);
+      lds_read8(xt, lo);
+      load_taps_tail<T, REV>(h, a.K, jc + kFirR * nsteps, hq);
+      fir_step(acc, hq, lo, hi);
     }
   }
 
@@ -338,8 +403,12 @@ tsiir_status launch_fir(const tsiir_fir_desc* d, const void* taps, const void* x
   const int64_t blocks = d->channels * a.tiles;
   if (blocks == 0) return TSIIR_OK;
   if (blocks > 0x7fffffff) return TSIIR_EUNSUPPORTED;
-  hipLaunchKernelGGL(fir_kernel<T>, dim3(static_cast<unsigned>(blocks)), dim3(kFirThreads), 0,
-                     st, a);
+  if (d->flags & TSIIR_FIR_REVERSE_TAPS)
+    hipLaunchKernelGGL((fir_kernel<T, true>), dim3(static_cast<unsigned>(blocks)),
+                       dim3(kFirThreads), 0, st, a);
+  else
+    hipLaunchKernelGGL((fir_kernel<T, false>), dim3(static_cast<unsigned>(blocks)),
+                       dim3(kFirThreads), 0, st, a);
   return hipGetLastError() == hipSuccess ? TSIIR_OK : TSIIR_ELAUNCH;
 }
 
