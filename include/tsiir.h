@@ -215,6 +215,19 @@ tsiir_status tsiir_fir_backward_taps(const tsiir_fir_desc* desc, const void* x, 
                                      void* dtaps, void* workspace, size_t workspace_bytes,
                                      void* stream);
 
+/* ---- analysis ---------------------------------------------------------------
+ * Frequency response of n_filters cascades of n_sections second-order
+ * sections: out[f][k] = prod_s (b0 + b1 z^-1 + b2 z^-2) / (a0 + a1 z^-1 +
+ * a2 z^-2) at z = exp(j w[k]), as interleaved FP64 (re, im) pairs,
+ * (n_filters, n_freq, 2).  sos is (n_filters, n_sections, 6) contiguous in
+ * coef_dtype (F32 or F64); w holds n_freq angular frequencies (radians per
+ * sample, FP64).  a0 is used as given.  Replaces the section loop of
+ * frequency_response_sos _frequency_response_sos.py:128-137. */
+tsiir_status tsiir_sos_frequency_response(const void* sos, int32_t coef_dtype,
+                                          int64_t n_filters, int32_t n_sections,
+                                          const double* w, int64_t n_freq, double* out,
+                                          void* stream);
+
 /* ---- serving: ragged batches of streaming chunks --------------------------
  * The reference has no serving layer; these entry points serve the streaming
  * use of sosfilt with zi/zf (_sosfilt.py:98-133): a chunk of a persistent

@@ -26,7 +26,11 @@ It restates, in numpy over the C DF2T core in ``oracle.c``:
   the 'same' slice of fftfilt (_fftfilt.py:60-92), as full convolutions
   (numpy.convolve) sliced at an offset;
 * ``fftfilt`` - _fftfilt.py:9-99 (axis move, 'same' slice from
-  (len(b) - 1) // 2).
+  (len(b) - 1) // 2);
+* ``impulse_response_sos`` - _impulse_response.py:173-271 (sections filtered
+  in order with a0 taken as 1);
+* ``sos_frequency_response`` - _frequency_response_sos.py:128-137 (product
+  of the section ratios at z = exp(jw)).
 
 Deliberate difference from the reference: zi with ``axis != -1`` follows the
 documented meaning (zi batch dims are x's batch dims in order), not the
@@ -479,3 +483,29 @@ def fftfilt(b, x, axis: int = -1):
     else:
         y, _ = fir_rows(b, rows, shift=(b.size - 1) // 2)
     return np.moveaxis(y.reshape(xm.shape), -1, axis)
+
+
+def impulse_response_sos(sos, n: int) -> np.ndarray:
+    """The cascade's impulse response with each section's a0 taken as 1, as
+    the reference's per-section helper filters (_impulse_response.py:215-220,
+    226-271)."""
+    s = np.array(sos, dtype=np.float64).reshape(-1, 6)
+    s[:, 3] = 1.0
+    x = np.zeros((1, n))
+    if n:
+        x[0, 0] = 1.0
+    if s.shape[0] == 0 or n == 0:
+        return x[0]
+    y, _ = sosfilt_rows(s, x)
+    return y[0]
+
+
+def sos_frequency_response(sos, w) -> np.ndarray:
+    """H(exp(jw)) = prod over sections of (b0 + b1 z^-1 + b2 z^-2) /
+    (a0 + a1 z^-1 + a2 z^-2), a0 as given."""
+    s = np.asarray(sos, dtype=np.float64).reshape(-1, 6)
+    zi1 = np.exp(-1j * np.asarray(w, dtype=np.float64))
+    h = np.ones_like(zi1)
+    for b0, b1, b2, a0, a1, a2 in s:
+        h = h * (b0 + b1 * zi1 + b2 * zi1 ** 2) / (a0 + a1 * zi1 + a2 * zi1 ** 2)
+    return h

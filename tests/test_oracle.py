@@ -291,3 +291,29 @@ def test_fir_planted_mistakes_are_caught(oracle_mod, mutation):
         y, _ = oracle_mod.fir_rows(b[::-1].copy(), x)
         y2 = oracle_mod.lfilter(b, np.ones(1), x)
         assert not np.allclose(y, y2, rtol=RTOL, atol=ATOL)
+
+
+def test_responses_match_scipy(oracle_mod):
+    """The frequency response restatement equals scipy.signal.sosfreqz (the
+    reference's own check, test__frequency_response.py:25-40) and a
+    hand-normalised section's DC gain; the impulse response of a normalised
+    cascade equals scipy's sosfilt of an impulse and its cumulative sum the
+    step response (test__analysis_functions.py:272-330)."""
+    import scipy.signal as ss
+    for order, cut in [(1, 0.1), (4, 0.3), (6, 0.7)]:
+        sos = ss.butter(order, cut, output="sos")
+        w, h = ss.sosfreqz(sos, worN=256)
+        np.testing.assert_allclose(oracle_mod.sos_frequency_response(sos, w), h, rtol=1e-12,
+                                   atol=1e-14)
+        imp = np.zeros(60)
+        imp[0] = 1.0
+        hi = oracle_mod.impulse_response_sos(sos, 60)
+        np.testing.assert_allclose(hi, ss.sosfilt(sos, imp), rtol=1e-12, atol=1e-14)
+    sec = np.array([[0.5, 0.2, 0.1, 2.0, 0.3, 0.1]])
+    g = oracle_mod.sos_frequency_response(sec, np.array([0.0]))
+    np.testing.assert_allclose(g, [(0.5 + 0.2 + 0.1) / (2.0 + 0.3 + 0.1)], rtol=1e-14)
+    # a0 is ignored by the impulse response: the same as the section with a0 = 1
+    sec1 = sec.copy()
+    sec1[0, 3] = 1.0
+    np.testing.assert_allclose(oracle_mod.impulse_response_sos(sec, 20),
+                               ss.sosfilt(sec1, np.eye(1, 20)[0]), rtol=1e-12, atol=1e-14)

@@ -39,6 +39,7 @@ EXPORTED = (
     "tsiir_fir_forward",
     "tsiir_fir_backward_taps_workspace_bytes",
     "tsiir_fir_backward_taps",
+    "tsiir_sos_frequency_response",
     "tsiir_serve_table_bytes",
     "tsiir_serve_prepare_filter",
     "tsiir_serve_batch",
@@ -138,6 +139,9 @@ def load(path: str | None = None) -> ctypes.CDLL:
     L.tsiir_fir_backward_taps_workspace_bytes.argtypes = [fp]
     L.tsiir_fir_backward_taps.restype = ctypes.c_int
     L.tsiir_fir_backward_taps.argtypes = [fp, vp, vp, vp, vp, sz, vp]
+    L.tsiir_sos_frequency_response.restype = ctypes.c_int
+    L.tsiir_sos_frequency_response.argtypes = [vp, ctypes.c_int32, ctypes.c_int64,
+                                               ctypes.c_int32, vp, ctypes.c_int64, vp, vp]
     L.tsiir_serve_table_bytes.restype = sz
     L.tsiir_serve_table_bytes.argtypes = [ctypes.c_int32]
     L.tsiir_serve_prepare_filter.restype = ctypes.c_int

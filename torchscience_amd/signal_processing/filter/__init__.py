@@ -1,14 +1,17 @@
 """IIR filtering on MI355X: drop-ins for torchscience.signal_processing.filter
 (src/torchscience/signal_processing/filter/__init__.py exports
-batched_filter_apply, fftfilt, filtfilt, lfilter, sosfilt, sosfiltfilt and
-the zi helpers)."""
+batched_filter_apply, fftfilt, filtfilt, lfilter, sosfilt, sosfiltfilt, the
+zi helpers and the impulse / step / frequency responses)."""
 
 from ._batched import batched_filter_apply
 from ._fir import fftfilt
 from ._filtfilt import filtfilt, sosfiltfilt
 from ._lfilter import lfilter
+from ._response import (FilterDesignError, SOSNormalizationError, frequency_response_sos,
+                        impulse_response, impulse_response_sos, step_response)
 from ._sosfilt import sosfilt
 from ._zi import lfilter_zi, sosfilt_zi
 
-__all__ = ["batched_filter_apply", "fftfilt", "filtfilt", "lfilter", "lfilter_zi", "sosfilt",
-           "sosfilt_zi", "sosfiltfilt"]
+__all__ = ["FilterDesignError", "SOSNormalizationError", "batched_filter_apply", "fftfilt",
+           "filtfilt", "frequency_response_sos", "impulse_response", "impulse_response_sos",
+           "lfilter", "lfilter_zi", "sosfilt", "sosfilt_zi", "sosfiltfilt", "step_response"]
