@@ -228,6 +228,42 @@ tsiir_status tsiir_sos_frequency_response(const void* sos, int32_t coef_dtype,
                                           const double* w, int64_t n_freq, double* out,
                                           void* stream);
 
+/* ---- batched IIR design -------------------------------------------------------
+ * FP64 design of n_filters digital filters of one family, band type and
+ * order into second-order sections, one cutoff (lowpass / highpass) or one
+ * (low, high) pair (bandpass / bandstop) per filter, as fractions of the
+ * Nyquist frequency.  Follows the reference's design chain (prototype,
+ * pre-warp 4 tan(pi f / 2), band transform, bilinear map at fs = 2, its
+ * pole/zero pairing and even gain split).  Replaces the per-filter loops of
+ * batched_butterworth_design / batched_chebyshev_type_1_design /
+ * batched_chebyshev_type_2_design _batched.py:22-344. */
+#define TSIIR_DESIGN_BUTTERWORTH 0
+#define TSIIR_DESIGN_CHEBYSHEV1 1   /* param: passband ripple in dB (> 0)     */
+#define TSIIR_DESIGN_CHEBYSHEV2 2   /* param: stopband attenuation in dB (> 0) */
+#define TSIIR_BAND_LOWPASS 0
+#define TSIIR_BAND_HIGHPASS 1
+#define TSIIR_BAND_BANDPASS 2
+#define TSIIR_BAND_BANDSTOP 3
+
+typedef struct tsiir_iir_design_desc {
+  int64_t n_filters;
+  int32_t family;       /* TSIIR_DESIGN_*                               */
+  int32_t band;         /* TSIIR_BAND_*                                 */
+  int32_t order;        /* 1 .. 32                                      */
+  int32_t out_dtype;    /* F32 or F64                                   */
+  double param;         /* ripple / attenuation in dB (Chebyshev)       */
+} tsiir_iir_design_desc;
+
+/* Sections per filter: ceil(order / 2), or order for band filters (0 for
+ * an unsupported order). */
+int32_t tsiir_iir_design_sections(const tsiir_iir_design_desc* desc);
+
+/* cutoffs: (n_filters) or (n_filters, 2) FP64 device array, each in (0, 1)
+ * (checked by the caller); sos: (n_filters, sections, 6) device output in
+ * out_dtype, rows [b0 b1 b2 1 a1 a2]. */
+tsiir_status tsiir_iir_design_sos(const tsiir_iir_design_desc* desc, const double* cutoffs,
+                                  void* sos, void* stream);
+
 /* ---- serving: ragged batches of streaming chunks --------------------------
  * The reference has no serving layer; these entry points serve the streaming
  * use of sosfilt with zi/zf (_sosfilt.py:98-133): a chunk of a persistent

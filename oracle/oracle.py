@@ -30,7 +30,15 @@ It restates, in numpy over the C DF2T core in ``oracle.c``:
 * ``impulse_response_sos`` - _impulse_response.py:173-271 (sections filtered
   in order with a0 taken as 1);
 * ``sos_frequency_response`` - _frequency_response_sos.py:128-137 (product
-  of the section ratios at z = exp(jw)).
+  of the section ratios at z = exp(jw));
+* ``iir_design_sos`` - one filter of batched_butterworth_design /
+  batched_chebyshev_type_1_design / batched_chebyshev_type_2_design
+  (_batched.py:22-344): prototypes (_butterworth_prototype.py:84-91,
+  _chebyshev_type_1_prototype.py:103-124, _chebyshev_type_2_prototype.py:
+  108-142), pre-warp (_butterworth_design.py:118), band maps
+  (_lowpass_to_*_zpk.py, bandstop keeping the gain), bilinear map at fs = 2
+  (_bilinear_transform_zpk.py:57-87) and zpk_to_sos (_zpk_to_sos.py:44-197),
+  all in float64.
 
 Deliberate difference from the reference: zi with ``axis != -1`` follows the
 documented meaning (zi batch dims are x's batch dims in order), not the
@@ -509,3 +517,110 @@ def sos_frequency_response(sos, w) -> np.ndarray:
     for b0, b1, b2, a0, a1, a2 in s:
         h = h * (b0 + b1 * zi1 + b2 * zi1 ** 2) / (a0 + a1 * zi1 + a2 * zi1 ** 2)
     return h
+
+
+def _design_separate(x):
+    """Real values and positive-imaginary complex values, in order
+    (_zpk_to_sos.py:176-197)."""
+    x = np.asarray(x, dtype=np.complex128)
+    if x.size == 0:
+        return np.zeros(0), np.zeros(0, np.complex128)
+    is_real = (np.abs(x.imag) < 1e-6 * np.abs(x.real) + 1e-10) | (np.abs(x) < 1e-6)
+    return x[is_real].real, x[~is_real & (x.imag > 0)]
+
+
+def _design_zpk_to_sos(z, p, k):
+    n_sec = (p.size + 1) // 2
+    if z.size < p.size:
+        z = np.concatenate([z, -np.ones(p.size - z.size, np.complex128)])
+    rp, cp = _design_separate(p)
+    rz, cz = _design_separate(z)
+    rp, cp, rz, cz = list(rp), list(cp), list(rz), list(cz)
+    rows = []
+    for p1 in cp:
+        if cz:
+            z1 = cz.pop(0)
+            num = [1.0, -2 * z1.real, z1.real ** 2 + z1.imag ** 2]
+        elif len(rz) >= 2:
+            z1, z2 = rz.pop(0), rz.pop(0)
+            num = [1.0, -(z1 + z2), z1 * z2]
+        else:
+            num = [1.0, 2.0, 1.0]
+        rows.append(num + [1.0, -2 * p1.real, p1.real ** 2 + p1.imag ** 2])
+    while len(rp) >= 2:
+        p1, p2 = rp.pop(0), rp.pop(0)
+        z1, z2 = (rz.pop(0), rz.pop(0)) if len(rz) >= 2 else (-1.0, -1.0)
+        rows.append([1.0, -(z1 + z2), z1 * z2, 1.0, -(p1 + p2), p1 * p2])
+    if len(rp) == 1:
+        z1 = rz[0] if rz else -1.0
+        rows.append([1.0, -z1, 0.0, 1.0, -rp[0], 0.0])
+    sos = np.array(rows, dtype=np.float64).reshape(-1, 6)
+    sos[:, :3] *= abs(k) ** (1.0 / n_sec)
+    if k < 0:
+        sos[0, 0] = -sos[0, 0]
+    return sos
+
+
+def iir_design_sos(family: str, order: int, cutoff, band: str = "lowpass",
+                   param: float = 0.0) -> np.ndarray:
+    """One digital IIR design, family "butterworth" | "chebyshev1" (param:
+    passband ripple dB) | "chebyshev2" (param: stopband attenuation dB),
+    cutoff(s) as fractions of Nyquist, into the reference's SOS layout."""
+    N = int(order)
+    kk = np.arange(N, dtype=np.float64)
+    z = np.zeros(0, np.complex128)
+    if family == "butterworth":
+        ang = np.pi * (2 * kk + N + 1) / (2 * N)
+        p = np.cos(ang) + 1j * np.sin(ang)
+        k = 1.0
+    else:
+        e = np.sqrt(10 ** (param / 10) - 1)
+        eps = e if family == "chebyshev1" else 1.0 / e
+        a = np.arcsinh(1.0 / eps) / N
+        th = np.pi * (2 * kk + 1) / (2 * N)
+        p1 = -np.sinh(a) * np.sin(th) + 1j * np.cosh(a) * np.cos(th)
+        if family == "chebyshev1":
+            p = p1
+            k = float(np.real(np.prod(-p)))
+            if N % 2 == 0:
+                k = k / np.sqrt(1 + eps ** 2)
+        else:
+            p = 1.0 / p1
+            ct = np.cos(th)
+            if N % 2 == 1:
+                ct = np.concatenate([ct[:N // 2], ct[N // 2 + 1:]])
+            z = 1j * (1.0 / ct)
+            k = float(np.real(np.prod(-p) / np.prod(-z))) if z.size else float(np.real(np.prod(-p)))
+    c = np.atleast_1d(np.asarray(cutoff, dtype=np.float64))
+    w = 4.0 * np.tan(np.pi * c / 2.0)
+    deg = p.size - z.size
+    if band == "lowpass":
+        z, p, k = z * w[0], p * w[0], k * w[0] ** deg
+    elif band == "highpass":
+        pz = np.prod(-z) if z.size else 1.0
+        k = k * float(np.real(pz / np.prod(-p)))
+        z = np.concatenate([w[0] / z, np.zeros(deg, np.complex128)])
+        p = w[0] / p
+    else:
+        bw = w[1] - w[0]
+        wc = np.sqrt(w[0] * w[1])
+
+        def roots(v, stop):
+            h = bw / (2 * v) if stop else bw * v / 2
+            d = np.sqrt(h * h - wc * wc + 0j)
+            return np.concatenate([h + d, h - d])
+        stop = band == "bandstop"
+        p = roots(p, stop)
+        zt = roots(z, stop) if z.size else np.zeros(0, np.complex128)
+        if stop:
+            z = np.concatenate([zt, np.tile(np.array([1j * wc, -1j * wc]), deg)])
+        else:
+            z = np.concatenate([zt, np.zeros(deg, np.complex128)])
+            k = k * bw ** deg
+    deg = p.size - z.size
+    num = np.prod(4.0 - z) if z.size else 1.0
+    k = k * float(np.real(num / np.prod(4.0 - p)))
+    zd = (1 + z / 4.0) / (1 - z / 4.0)
+    zd = np.concatenate([zd, -np.ones(deg, np.complex128)])
+    pd = (1 + p / 4.0) / (1 - p / 4.0)
+    return _design_zpk_to_sos(zd, pd, k)

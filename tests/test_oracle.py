@@ -317,3 +317,62 @@ def test_responses_match_scipy(oracle_mod):
     sec1[0, 3] = 1.0
     np.testing.assert_allclose(oracle_mod.impulse_response_sos(sec, 20),
                                ss.sosfilt(sec1, np.eye(1, 20)[0]), rtol=1e-12, atol=1e-14)
+
+
+def _zpk_of(sos):
+    import scipy.signal as ss
+    return ss.sos2zpk(sos)
+
+
+@pytest.mark.parametrize("family,param", [("butterworth", 0.0), ("chebyshev1", 1.0),
+                                          ("chebyshev2", 40.0)])
+def test_iir_design_matches_scipy(oracle_mod, family, param):
+    """The design restatement against scipy.signal's designs (the
+    reference's own checks compare responses with scipy, test__batched.py,
+    test__zpk_to_sos.py:18-43): the same response at 1e-10 for lowpass,
+    highpass and bandpass.  Bandstop differs from scipy in two places the
+    reference fixes: its band map keeps the prototype gain
+    (_lowpass_to_bandstop_zpk.py:103), so the gain is scipy's divided by
+    Re(prod(-z) / prod(-p)) of the analog prototype; and when the
+    prototype's real pole maps to two real poles, its pairing gives them
+    the zeros -1, -1 and leaves one complex zero pair unused
+    (_zpk_to_sos.py:126-140).  Poles always equal scipy's."""
+    import scipy.signal as ss
+    design = {"butterworth": lambda N, c, bt: ss.butter(N, c, bt, output="zpk"),
+              "chebyshev1": lambda N, c, bt: ss.cheby1(N, param, c, bt, output="zpk"),
+              "chebyshev2": lambda N, c, bt: ss.cheby2(N, param, c, bt, output="zpk")}[family]
+    proto = {"butterworth": lambda N: ss.buttap(N), "chebyshev1": lambda N: ss.cheb1ap(N, param),
+             "chebyshev2": lambda N: ss.cheb2ap(N, param)}[family]
+    w = np.linspace(0.01, np.pi - 0.01, 400)
+    for bt, c in [("lowpass", 0.3), ("highpass", 0.15), ("bandpass", [0.2, 0.5]),
+                  ("bandstop", [0.1, 0.6]), ("bandstop", [0.2, 0.5])]:
+        for N in (1, 2, 3, 4, 5, 8):
+            sos = oracle_mod.iir_design_sos(family, N, c, bt, param)
+            assert sos.shape == (N if bt in ("bandpass", "bandstop") else (N + 1) // 2, 6)
+            zs, ps, ks = design(N, c, bt)
+            if bt != "bandstop":
+                h = oracle_mod.sos_frequency_response(sos, w)
+                hs = oracle_mod.sos_frequency_response(ss.zpk2sos(zs, ps, ks), w)
+                assert np.max(np.abs(h - hs)) < 1e-10 * np.max(np.abs(hs)), (bt, N)
+                continue
+            z, p, k = _zpk_of(sos)
+            np.testing.assert_allclose(np.sort_complex(p), np.sort_complex(ps), rtol=1e-9,
+                                       atol=1e-12)
+            za, pa, _ = proto(N)
+            rule = np.real(np.prod(-za) / np.prod(-pa)) if za.size else np.real(1 / np.prod(-pa))
+            assert abs(k - ks / rule) < 1e-9 * abs(ks / rule), (N, c)
+            n_real_p = int(np.sum(np.abs(ps.imag) < 1e-9))
+            if n_real_p < 2:
+                np.testing.assert_allclose(np.sort_complex(z), np.sort_complex(zs), rtol=1e-9,
+                                           atol=1e-9)
+                continue
+            zr = np.sort_complex(z)
+            assert np.sum(np.abs(zr + 1) < 1e-9) == 2, (N, c)
+            rest = zr[np.abs(zr + 1) >= 1e-9]
+            assert np.allclose(np.abs(rest), 1.0, atol=1e-9)
+            # every remaining zero is one of scipy's, with multiplicity
+            pool = list(zs)
+            for zz in rest:
+                i = int(np.argmin(np.abs(np.array(pool) - zz)))
+                assert abs(pool[i] - zz) < 1e-9
+                pool.pop(i)

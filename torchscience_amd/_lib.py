@@ -40,6 +40,8 @@ EXPORTED = (
     "tsiir_fir_backward_taps_workspace_bytes",
     "tsiir_fir_backward_taps",
     "tsiir_sos_frequency_response",
+    "tsiir_iir_design_sections",
+    "tsiir_iir_design_sos",
     "tsiir_serve_table_bytes",
     "tsiir_serve_prepare_filter",
     "tsiir_serve_batch",
@@ -76,6 +78,17 @@ class FirDesc(ctypes.Structure):
         ("x_row_stride", ctypes.c_int64),
         ("y_row_stride", ctypes.c_int64),
         ("chans_per_filter", ctypes.c_int64),
+    ]
+
+
+class IirDesignDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_filters", ctypes.c_int64),
+        ("family", ctypes.c_int32),
+        ("band", ctypes.c_int32),
+        ("order", ctypes.c_int32),
+        ("out_dtype", ctypes.c_int32),
+        ("param", ctypes.c_double),
     ]
 
 
@@ -142,6 +155,11 @@ def load(path: str | None = None) -> ctypes.CDLL:
     L.tsiir_sos_frequency_response.restype = ctypes.c_int
     L.tsiir_sos_frequency_response.argtypes = [vp, ctypes.c_int32, ctypes.c_int64,
                                                ctypes.c_int32, vp, ctypes.c_int64, vp, vp]
+    gp = ctypes.POINTER(IirDesignDesc)
+    L.tsiir_iir_design_sections.restype = ctypes.c_int32
+    L.tsiir_iir_design_sections.argtypes = [gp]
+    L.tsiir_iir_design_sos.restype = ctypes.c_int
+    L.tsiir_iir_design_sos.argtypes = [gp, vp, vp, vp]
     L.tsiir_serve_table_bytes.restype = sz
     L.tsiir_serve_table_bytes.argtypes = [ctypes.c_int32]
     L.tsiir_serve_prepare_filter.restype = ctypes.c_int

@@ -1,9 +1,12 @@
 """IIR filtering on MI355X: drop-ins for torchscience.signal_processing.filter
 (src/torchscience/signal_processing/filter/__init__.py exports
 batched_filter_apply, fftfilt, filtfilt, lfilter, sosfilt, sosfiltfilt, the
-zi helpers and the impulse / step / frequency responses)."""
+zi helpers, the impulse / step / frequency responses and the batched IIR
+designs)."""
 
 from ._batched import batched_filter_apply
+from ._batched_design import (batched_butterworth_design, batched_chebyshev_type_1_design,
+                              batched_chebyshev_type_2_design)
 from ._fir import fftfilt
 from ._filtfilt import filtfilt, sosfiltfilt
 from ._lfilter import lfilter
@@ -12,6 +15,8 @@ from ._response import (FilterDesignError, SOSNormalizationError, frequency_resp
 from ._sosfilt import sosfilt
 from ._zi import lfilter_zi, sosfilt_zi
 
-__all__ = ["FilterDesignError", "SOSNormalizationError", "batched_filter_apply", "fftfilt",
+__all__ = ["FilterDesignError", "SOSNormalizationError", "batched_butterworth_design",
+           "batched_chebyshev_type_1_design", "batched_chebyshev_type_2_design",
+           "batched_filter_apply", "fftfilt",
            "filtfilt", "frequency_response_sos", "impulse_response", "impulse_response_sos",
            "lfilter", "lfilter_zi", "sosfilt", "sosfilt_zi", "sosfiltfilt", "step_response"]
