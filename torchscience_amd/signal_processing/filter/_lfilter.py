@@ -6,8 +6,9 @@ Filters of order <= 2 (max(len(b), len(a)) <= 3) are one second-order
 section and run through the single-pass SOS kernel, with autograd.  Higher
 orders with a single denominator coefficient (FIR, a = [a0]) run the
 direct-form FIR kernel, with autograd.  Other higher orders run the
-general-order DF2T kernel (one lane per channel, serial in time), which has
-no backward.
+general-order DF2T kernel (one lane per channel, serial in time), with an
+autograd whose backward reuses that kernel (reverse-time all-pole pass), the
+FIR kernel and the tap-gradient kernel.
 
 zi is (batch..., N) with x's batch dimensions in order (the documented
 meaning; the reference's moveaxis of zi, _lfilter.py:137, is not followed),
@@ -139,11 +140,128 @@ def _lfilter_as_section(b, a, x, axis, zi, order):
     return y, zf[..., :order].reshape(zi.shape).to(out)
 
 
+def _general_rows(coef, rows, zi_r, want_zf, zi_broadcast):
+    """DF2T of order N = len(coef) // 2 - 1 (coef = b then a, padded) over
+    rows (B, T) through tsiir_lfilter_forward.  Returns (y, zf or None)."""
+    B, T = rows.shape
+    N = coef.numel() // 2 - 1
+    dev = rows.device
+    y = torch.empty_like(rows)
+    flags = 0
+    if zi_r is not None:
+        flags |= _lib.HAS_ZI | (_lib.ZI_BROADCAST if zi_broadcast else 0)
+    if want_zf:
+        flags |= _lib.WRITE_ZF
+    zf = torch.empty((B, N), dtype=rows.dtype, device=dev) if want_zf else None
+    if B == 0:
+        return y, zf
+    d = _desc(channels=B, n_samples=T, n_sections=N, io_dtype=rows.dtype,
+              coef_dtype=rows.dtype, flags=flags, x_rows=B, x_row_stride=T, y_row_stride=T,
+              chans_per_filter=B)
+    L = _lib.load()
+    _lib.check(L.tsiir_lfilter_forward(d, _lib.ptr(coef), _lib.ptr(rows), _lib.ptr(y),
+                                       _lib.ptr(zi_r), _lib.ptr(zf), _lib.stream_handle(dev)),
+               "tsiir_lfilter_forward")
+    return y, zf
+
+
+def _tail_toeplitz(c, N):
+    """M[k, l] = c[N + k - l] for k <= l, else 0 (N x N): the weights with
+    which zf[k] reads the sample T - N + l of a signal through c."""
+    k = torch.arange(N, device=c.device)[:, None]
+    l_ = torch.arange(N, device=c.device)[None, :]
+    return torch.where(k <= l_, c[(N + k - l_).clamp(0, N)], torch.zeros((), dtype=c.dtype,
+                                                                         device=c.device))
+
+
+def _tail(v, N):
+    """The last N samples of rows v (B, T), left-padded with zeros if T < N."""
+    T = v.shape[-1]
+    return v[:, T - N:] if T >= N else F.pad(v, (N - T, 0))
+
+
+def _direct_taps(dz, v, N):
+    """d[m] = sum_c sum_{k < m} dz[c, k] v[c, T + k - m] for m = 1..N (d[0] = 0):
+    the part of a coefficient gradient that reaches it through zf directly."""
+    G = dz.transpose(0, 1) @ _tail(v, N)              # G[k, l] = sum_c dz[c,k] vtail[c,l]
+    m = torch.arange(1, N + 1, device=dz.device)[:, None]
+    k = torch.arange(N, device=dz.device)[None, :]
+    # entries with k >= m are masked below; clamp their indices into G
+    vals = G[k.expand(N, N), (N + k - m).clamp(0, N - 1)]
+    vals = torch.where(k < m, vals, torch.zeros((), dtype=G.dtype, device=G.device))
+    return F.pad(vals.sum(1), (1, 0))
+
+
+class LfilterRows(torch.autograd.Function):
+    """Autograd of the general-order DF2T over rows, with normalised
+    coefficients (a[0] = 1).  With g the output gradient run backwards in
+    time through the all-pole part 1/A (plus what zf feeds back through
+    y), dx is g through the taps b reversed (the FIR kernel), db and da are
+    the correlations of g with x and with -y (the tap-gradient kernel), dzi
+    is g's first N samples; zf's gradient also reaches x, y, b and a
+    directly through the last N samples (_lfilter.py:153-166 recorded by
+    the reference's autograd)."""
+
+    @staticmethod
+    def forward(ctx, rows, bn, an, zi_r, want_zf, zi_broadcast):
+        y, zf = _general_rows(torch.cat([bn, an]).contiguous(), rows, zi_r, want_zf,
+                              zi_broadcast)
+        ctx.save_for_backward(rows, bn, an, y)
+        ctx.meta = (want_zf, zi_broadcast, zi_r is not None)
+        return y, zf
+
+    @staticmethod
+    def backward(ctx, dy, dzf):
+        from ._fir import fir_rows_backward_taps, fir_rows_forward
+        x, bn, an, y = ctx.saved_tensors
+        want_zf, zi_b, has_zi = ctx.meta
+        B, T = x.shape
+        N = bn.numel() - 1
+        dt = x.dtype
+        dyh = torch.zeros_like(x) if dy is None else dy.to(dt).clone()
+        dz = dzf.to(dt) if (want_zf and dzf is not None) else None
+        lo = max(N - T, 0)                 # tail columns l >= lo exist in the signal
+        if dz is not None and T > 0:
+            dyh[:, T - N + lo:] -= (dz @ _tail_toeplitz(an, N))[:, lo:]
+        # g[n] = dyh[n] - sum_k a_k g[n + k]: the all-pole filter in reverse time
+        one = F.pad(torch.ones(1, dtype=dt, device=x.device), (0, N))
+        g = dyh
+        if T > 0 and B > 0:
+            gr, _ = _general_rows(torch.cat([one, an]).contiguous(),
+                                  dyh.flip(-1).contiguous(), None, False, False)
+            g = gr.flip(-1).contiguous()
+        gx = gb = ga = gzi = None
+        if ctx.needs_input_grad[0]:
+            gx = torch.zeros_like(x)
+            if T > 0 and B > 0:
+                gx, _ = fir_rows_forward(bn.reshape(1, -1).contiguous(), g, None, channels=B,
+                                         chans_per_filter=B, shift=N, reverse_taps=True)
+            if dz is not None and T > 0:
+                gx[:, T - N + lo:] += (dz @ _tail_toeplitz(bn, N))[:, lo:]
+        if ctx.needs_input_grad[1] or ctx.needs_input_grad[2]:
+            db = torch.zeros(N + 1, dtype=dt, device=x.device)
+            da = torch.zeros(N + 1, dtype=dt, device=x.device)
+            if T > 0 and B > 0:
+                db = fir_rows_backward_taps((1, N + 1), x, g, channels=B,
+                                            chans_per_filter=B)[0]
+                da = -fir_rows_backward_taps((1, N + 1), y, g, channels=B,
+                                             chans_per_filter=B)[0]
+            if dz is not None:
+                db = db + _direct_taps(dz, x, N)
+                da = da - _direct_taps(dz, y, N)
+            da = torch.cat([torch.zeros(1, dtype=dt, device=x.device), da[1:]])
+            gb, ga = db, da
+        if has_zi and ctx.needs_input_grad[3]:
+            dzi = torch.zeros((B, N), dtype=dt, device=x.device)
+            m = min(N, T)
+            dzi[:, :m] = g[:, :m]
+            if dz is not None and T < N:
+                dzi[:, T:] += dz[:, :N - T]
+            gzi = dzi.sum(0, keepdim=True) if zi_b else dzi
+        return gx, gb, ga, gzi, None, None
+
+
 def _lfilter_general(b, a, x, axis, zi, order):
-    if torch.is_grad_enabled() and (b.requires_grad or a.requires_grad or x.requires_grad
-                                    or (zi is not None and zi.requires_grad)):
-        raise NotImplementedError(
-            "gradients of lfilter are available for filters of order <= 2")
     dev = _require_device(x)
     nd = x.dim()
     ax = axis % nd
@@ -157,23 +275,18 @@ def _lfilter_general(b, a, x, axis, zi, order):
     rows = xm.reshape(math.prod(xm.shape[:-1]), T).to(comp).contiguous()
     B = rows.shape[0]
     n = order + 1
-    coef = torch.cat([F.pad(b.to(comp), (0, n - b.shape[0])),
-                      F.pad(a.to(comp), (0, n - a.shape[0]))]).to(dev).contiguous()
-    y = torch.empty_like(rows)
+    bb = F.pad(b.to(device=dev, dtype=comp), (0, n - b.shape[0]))
+    aa = F.pad(a.to(device=dev, dtype=comp), (0, n - a.shape[0]))
+    a0 = aa[0]
+    # normalise by a0 only when a0 != 1 (_lfilter.py:96-98), so that a0 gets
+    # no gradient then
+    bn = torch.where(a0 != 1, bb / a0, bb)
+    an = torch.where(a0 != 1, aa / a0, aa)
     zi_r = None
-    flags = 0
     if zi is not None:
         zi_r = _state_rows(zi, order, dev, comp)
-        if zi_r.shape[0] == 1 and B > 1:
-            flags |= _lib.ZI_BROADCAST
-        flags |= _lib.HAS_ZI | _lib.WRITE_ZF
-    zf = torch.empty((B, order), dtype=comp, device=dev) if zi is not None else None
-    d = _desc(channels=B, n_samples=T, n_sections=order, io_dtype=comp, coef_dtype=comp,
-              flags=flags, x_rows=B, x_row_stride=T, y_row_stride=T, chans_per_filter=B)
-    L = _lib.load()
-    _lib.check(L.tsiir_lfilter_forward(d, _lib.ptr(coef), _lib.ptr(rows), _lib.ptr(y),
-                                       _lib.ptr(zi_r), _lib.ptr(zf), _lib.stream_handle(dev)),
-               "tsiir_lfilter_forward")
+    zb = zi_r is not None and zi_r.shape[0] == 1 and B > 1
+    y, zf = LfilterRows.apply(rows, bn, an, zi_r, zi is not None, zb)
     y = torch.movedim(y.reshape(batch + (T,)), -1, ax).to(out)
     if zi is None:
         return y
