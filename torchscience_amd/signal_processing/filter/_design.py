@@ -20,7 +20,7 @@ The reference builds its prototype poles at torch's default dtype (float32
 unless changed, _butterworth_prototype.py:66-77), which leaves float32-accurate
 poles inside a float64 SOS; this module does not reproduce that and designs
 every step in float64.  Its magnitude responses are pinned against
-scipy.signal's designs in tests/test_design.py.
+scipy.signal's designs in tests/test_host.py.
 """
 
 from __future__ import annotations
