@@ -264,6 +264,15 @@ int32_t tsiir_iir_design_sections(const tsiir_iir_design_desc* desc);
 tsiir_status tsiir_iir_design_sos(const tsiir_iir_design_desc* desc, const double* cutoffs,
                                   void* sos, void* stream);
 
+/* tsiir_iir_design_sos that also writes the Jacobian of every coefficient
+ * with respect to the filter's cutoffs: jacobian (n_filters, sections, 6, 2)
+ * FP64, the last index the low (or only) and the high cutoff (0 for a
+ * single cutoff).  Replaces the autograd the reference records through its
+ * design ops (the cutoff gradients of _batched.py:22-344). */
+tsiir_status tsiir_iir_design_sos_jacobian(const tsiir_iir_design_desc* desc,
+                                           const double* cutoffs, void* sos,
+                                           double* jacobian, void* stream);
+
 /* ---- serving: ragged batches of streaming chunks --------------------------
  * The reference has no serving layer; these entry points serve the streaming
  * use of sosfilt with zi/zf (_sosfilt.py:98-133): a chunk of a persistent

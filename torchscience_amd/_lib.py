@@ -42,6 +42,7 @@ EXPORTED = (
     "tsiir_sos_frequency_response",
     "tsiir_iir_design_sections",
     "tsiir_iir_design_sos",
+    "tsiir_iir_design_sos_jacobian",
     "tsiir_serve_table_bytes",
     "tsiir_serve_prepare_filter",
     "tsiir_serve_batch",
@@ -160,6 +161,8 @@ def load(path: str | None = None) -> ctypes.CDLL:
     L.tsiir_iir_design_sections.argtypes = [gp]
     L.tsiir_iir_design_sos.restype = ctypes.c_int
     L.tsiir_iir_design_sos.argtypes = [gp, vp, vp, vp]
+    L.tsiir_iir_design_sos_jacobian.restype = ctypes.c_int
+    L.tsiir_iir_design_sos_jacobian.argtypes = [gp, vp, vp, vp, vp]
     L.tsiir_serve_table_bytes.restype = sz
     L.tsiir_serve_table_bytes.argtypes = [ctypes.c_int32]
     L.tsiir_serve_prepare_filter.restype = ctypes.c_int

@@ -10,10 +10,14 @@ argument checks and their messages follow the reference
 (_batched.py:84-117, _butterworth_design.py:97-113, the prototypes' order /
 ripple / attenuation checks, the "Invalid filter_type" check).
 
-Differences: every step is FP64 (the reference builds its prototypes at
+Cutoffs that require a gradient get one: the same launch writes the
+Jacobian of every coefficient with respect to the filter's cutoffs
+(forward-mode differentiation in the kernel), which the backward contracts
+with the incoming gradient.
+
+Difference: every step is FP64 (the reference builds its prototypes at
 torch's default dtype unless dtype is given, which leaves float32-accurate
-poles in the float64 result), and the design is not differentiable with
-respect to the cutoffs (NotImplementedError when they require a gradient).
+poles in the float64 result).
 """
 
 from __future__ import annotations
@@ -72,17 +76,13 @@ def _design(family: str, order: int, param: float, cutoffs: Tensor, filter_type:
     if output != "sos":
         raise ValueError(f"Batched design only supports 'sos' output, got '{output}'")
     c = _cutoff_rows(cutoffs, filter_type)
-    if c.requires_grad and torch.is_grad_enabled():
-        raise NotImplementedError(
-            "batched IIR design on the MI355X path is not differentiable with respect to "
-            "the cutoffs")
     if device is None:
         device = cutoffs.device
     band = filter_type in ("bandpass", "bandstop")
-    c = c.detach().to(torch.float64)
+    c = c.to(torch.float64)
     if sampling_frequency is not None:
         c = c / (sampling_frequency / 2.0)
-    _check_range(c, band)
+    _check_range(c.detach(), band)
     if order < 1:
         raise ValueError(f"Filter order must be positive, got {order}")
     if family == "chebyshev1" and param <= 0:
@@ -109,10 +109,40 @@ def _design(family: str, order: int, param: float, cutoffs: Tensor, filter_type:
     if n_sec == 0:
         raise NotImplementedError(
             f"filter order {order} is above the largest order of the batched design (32)")
-    sos = torch.empty((c.shape[0], n_sec, 6), dtype=kdt, device=dev)
-    _lib.check(L.tsiir_iir_design_sos(d, _lib.ptr(c), _lib.ptr(sos), _lib.stream_handle(dev)),
-               "tsiir_iir_design_sos")
+    if c.requires_grad and torch.is_grad_enabled():
+        sos = _DesignWithGrad.apply(c, d, n_sec, band)
+    else:
+        sos = torch.empty((c.shape[0], n_sec, 6), dtype=kdt, device=dev)
+        _lib.check(L.tsiir_iir_design_sos(d, _lib.ptr(c), _lib.ptr(sos),
+                                          _lib.stream_handle(dev)), "tsiir_iir_design_sos")
     return sos.to(dtype=out_dtype, device=device)
+
+
+class _DesignWithGrad(torch.autograd.Function):
+    """FP64 design whose backward contracts the kernel's Jacobian
+    d sos / d cutoffs with the incoming gradient."""
+
+    @staticmethod
+    def forward(ctx, c, d, n_sec, band):
+        dev = c.device
+        d64 = _lib.IirDesignDesc()
+        for f in ("n_filters", "family", "band", "order", "param"):
+            setattr(d64, f, getattr(d, f))
+        d64.out_dtype = _lib.F64
+        sos = torch.empty((c.shape[0], n_sec, 6), dtype=torch.float64, device=dev)
+        jac = torch.empty((c.shape[0], n_sec, 6, 2), dtype=torch.float64, device=dev)
+        _lib.check(_lib.load().tsiir_iir_design_sos_jacobian(
+            d64, _lib.ptr(c.detach().contiguous()), _lib.ptr(sos), _lib.ptr(jac),
+            _lib.stream_handle(dev)), "tsiir_iir_design_sos_jacobian")
+        ctx.save_for_backward(jac)
+        ctx.band = band
+        return sos
+
+    @staticmethod
+    def backward(ctx, g):
+        (jac,) = ctx.saved_tensors
+        gc = (g.to(jac.dtype).unsqueeze(-1) * jac).sum(dim=(1, 2))     # (n, 2)
+        return (gc if ctx.band else gc[:, 0]), None, None, None
 
 
 def batched_butterworth_design(order: int, cutoffs: Tensor, filter_type: str = "lowpass",
