@@ -166,3 +166,44 @@ def test_zero_phase_validation_matches_reference():
         filtfilt(torch.ones(3), torch.ones(4), torch.randn(12))
     with pytest.raises(RuntimeError, match="HIP"):
         sosfiltfilt(sos, torch.randn(100, dtype=torch.float64))
+
+
+def test_batched_design_argument_checks():
+    """The reference's ValueErrors of the batched designs (_batched.py:84-117,
+    _butterworth_design.py:102-113, the prototypes' checks), raised before
+    any device work."""
+    from torchscience_amd.signal_processing.filter import (
+        batched_butterworth_design, batched_chebyshev_type_1_design,
+        batched_chebyshev_type_2_design)
+    with pytest.raises(ValueError, match="cutoffs tensor cannot be empty"):
+        batched_butterworth_design(4, torch.tensor([]))
+    with pytest.raises(ValueError, match="Batched design only supports 'sos' output"):
+        batched_butterworth_design(4, torch.tensor([0.2]), output="zpk")
+    with pytest.raises(ValueError, match=r"cutoffs must have shape \(batch, 2\)"):
+        batched_butterworth_design(4, torch.tensor([0.2, 0.3]), filter_type="bandstop")
+    with pytest.raises(ValueError, match="must be 1D or"):
+        batched_butterworth_design(4, torch.ones(3, 2) * 0.2)
+    with pytest.raises(ValueError, match="between 0 and 1"):
+        batched_butterworth_design(4, torch.tensor([0.0]))
+    with pytest.raises(ValueError, match="0 < low < high < 1"):
+        batched_butterworth_design(4, torch.tensor([[0.3, 0.2]]), filter_type="bandpass")
+    with pytest.raises(ValueError, match="Filter order must be positive"):
+        batched_butterworth_design(-1, torch.tensor([0.2]))
+    with pytest.raises(ValueError, match="Passband ripple must be positive"):
+        batched_chebyshev_type_1_design(4, -0.5, torch.tensor([0.2]))
+    with pytest.raises(ValueError, match="Stopband attenuation must be positive"):
+        batched_chebyshev_type_2_design(4, 0.0, torch.tensor([0.2]))
+    with pytest.raises(ValueError, match="Invalid filter_type"):
+        batched_butterworth_design(4, torch.tensor([0.2]), filter_type="allpass")
+    with pytest.raises(ValueError, match="between 0 and 1"):
+        batched_butterworth_design(4, torch.tensor([9000.0]), sampling_frequency=16000.0)
+
+
+def test_frequency_response_validation_before_device_work():
+    """a0 != 1 raises the reference's SOSNormalizationError
+    (_frequency_response_sos.py:84-89), a FilterDesignError."""
+    from torchscience_amd.signal_processing.filter import (
+        FilterDesignError, SOSNormalizationError, frequency_response_sos)
+    assert issubclass(SOSNormalizationError, FilterDesignError)
+    with pytest.raises(SOSNormalizationError, match="a0 = 1"):
+        frequency_response_sos(torch.tensor([[1.0, 0.5, 0.25, 2.0, 0.1, 0.05]]))
