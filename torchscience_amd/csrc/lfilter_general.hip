@@ -5,12 +5,32 @@
 // direct-form IIR polynomials).  Coefficients are normalised by a[0] when
 // a[0] != 1 (_lfilter.py:96-98) and were zero-padded to N+1 by the caller
 // (_lfilter.py:114-117).  The state lives in registers for N <= 16 and in
-// per-lane scratch above that.
+// per-lane scratch above that.  Each lane streams its row in blocks of 16
+// samples (8 for orders 9..16), the next block's loads issued before the
+// current block's recurrence, so the serial chain does not wait on memory
+// every sample.
 
 #include "sos_common.h"
 #include "tsiir.h"
 
 namespace tsiir {
+
+// samples per lane per block: fewer for deep states, whose unrolled
+// recurrence would not fit the registers
+template <int NMAX>
+constexpr int kGenBlk = NMAX <= 8 ? 16 : (NMAX <= 16 ? 8 : 1);
+
+template <typename TACC, int NMAX>
+__device__ inline void df2t_step(const TACC (&bn)[NMAX + 1], const TACC (&an)[NMAX + 1],
+                                 int N, TACC (&z)[NMAX], TACC xn, TACC& yn) {
+  yn = bn[0] * xn + z[0];
+#pragma unroll
+  for (int k = 0; k < NMAX - 1; ++k)
+    if (k < N - 1) z[k] = (bn[k + 1] * xn - an[k + 1] * yn) + z[k + 1];
+#pragma unroll
+  for (int k = 0; k < NMAX; ++k)
+    if (k == N - 1) z[k] = bn[N] * xn - an[N] * yn;
+}
 
 template <typename TACC, int NMAX>
 __global__ __launch_bounds__(64) void lfilter_general_kernel(
@@ -38,7 +58,34 @@ __global__ __launch_bounds__(64) void lfilter_general_kernel(
     z[k] = (k < N && (flags & TSIIR_HAS_ZI)) ? zi[zrow * N + k] : TACC(0);
   const TACC* xr = x + (c % x_rows) * x_stride;
   TACC* yr = y + c * y_stride;
-  for (int64_t n = 0; n < T; ++n) {
+  // Whole blocks of BLK samples: the next block's loads are in flight while
+  // the current block runs through the recurrence.
+  constexpr int BLK = kGenBlk<NMAX>;
+  const int64_t Tb = BLK > 1 ? T / BLK * BLK : 0;
+  if constexpr (BLK > 1) {
+    TACC cur[BLK], nxt[BLK];
+    if (Tb > 0) {
+#pragma unroll
+      for (int i = 0; i < BLK; ++i) cur[i] = xr[i];
+    }
+    for (int64_t n0 = 0; n0 < Tb; n0 += BLK) {
+      if (n0 + BLK < Tb) {
+#pragma unroll
+        for (int i = 0; i < BLK; ++i) nxt[i] = xr[n0 + BLK + i];
+      }
+#pragma unroll
+      for (int i = 0; i < BLK; ++i) {
+        TACC yn;
+        df2t_step<TACC, NMAX>(bn, an, N, z, cur[i], yn);
+        yr[n0 + i] = yn;
+      }
+#pragma unroll
+      for (int i = 0; i < BLK; ++i) cur[i] = nxt[i];
+    }
+  }
+  // the remaining samples one at a time (the same update as df2t_step,
+  // written out: through the helper the order-64 form spills)
+  for (int64_t n = Tb; n < T; ++n) {
     const TACC xn = xr[n];
     const TACC yn = bn[0] * xn + z[0];
 #pragma unroll
