@@ -130,6 +130,18 @@ tsiir_status tsiir_lfilter_forward(const tsiir_sosfilt_desc* desc, const void* c
                                    const void* x, void* y, const void* zi, void* zf,
                                    void* stream);
 
+/* tsiir_lfilter_forward with a workspace, which lets orders up to 8 run in
+ * time-parallel form: chunks of 256 samples filtered from zero state, the
+ * chunk start states scanned in FP64 across the chunks, then every chunk
+ * filtered again from its start state.  Workspace: channels x chunks x order
+ * FP64 values (tsiir_lfilter_workspace_bytes; 0 when the serial form is
+ * used, for orders above 8 or fewer than 512 samples).  Same results to
+ * rounding; same reference loop replaced. */
+size_t tsiir_lfilter_workspace_bytes(const tsiir_sosfilt_desc* desc);
+tsiir_status tsiir_lfilter_forward_ws(const tsiir_sosfilt_desc* desc, const void* coef,
+                                      const void* x, void* y, const void* zi, void* zf,
+                                      void* workspace, size_t workspace_bytes, void* stream);
+
 /* Gradients of sosfilt (FP32 or FP64 compute).  Given dy (and dzf when zf
  * was an output), writes dx, dsos (n_filters, n_sections, 6) and dzi
  * (channels, n_sections, 2).  Replaces the autograd tape the reference

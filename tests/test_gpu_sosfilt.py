@@ -263,3 +263,36 @@ def test_lfilter_general_order_and_complex(golden):
     yc = F.lfilter(bb.to(dev), aa.to(dev), xc.to(dev))
     np.testing.assert_allclose(yc.cpu().numpy(), ss.lfilter(bb.numpy(), aa.numpy(), xc.numpy()),
                                rtol=1e-10, atol=1e-10)
+
+
+@pytest.mark.parametrize("order", [3, 8, 16])
+def test_lfilter_general_chunked_matches_serial_oracle(oracle_mod, order):
+    """Orders up to 8 over at least 512 samples run in time-parallel form
+    (chunks of 256 from zero state, FP64 start-state scan, second pass):
+    equal to the oracle's serial DF2T, zi per row and shared, zf, including
+    a partial last chunk.  Order 16 takes the serial form."""
+    F, dev = _filt(), _dev()
+    import scipy.signal as ss
+    b, a = ss.butter(order, 0.23)
+    a = a * 1.5      # a0 != 1: normalised
+    b = b * 1.5
+    rng = np.random.default_rng(order)
+    for T in (512, 256 * 9 + 37, 65536 + 5):
+        print("case lfilter chunked", order, T, flush=True)
+        x = rng.standard_normal((3, T))
+        for zi in (rng.standard_normal((3, order)), rng.standard_normal(order)):
+            y, zf = F.lfilter(torch.tensor(b, device=dev), torch.tensor(a, device=dev),
+                              torch.tensor(x, device=dev), zi=torch.tensor(zi, device=dev))
+            yr, zfr = oracle_mod.lfilter(b, a, x, zi=np.broadcast_to(zi, (3, order)))
+            scale = np.max(np.abs(yr))
+            assert np.max(np.abs(y.cpu().numpy() - yr)) < 1e-9 * scale, (order, T)
+            assert np.max(np.abs(zf.cpu().numpy() - zfr)) < 1e-9 * max(np.max(np.abs(zfr)), 1)
+        if order <= 8:
+            # (an order-16 direct form is beyond FP32 arithmetic in any order
+            # of evaluation)
+            y32 = F.lfilter(torch.tensor(b, dtype=torch.float32, device=dev),
+                            torch.tensor(a, dtype=torch.float32, device=dev),
+                            torch.tensor(x, dtype=torch.float32, device=dev))
+            y64 = oracle_mod.lfilter(b.astype(np.float32), a.astype(np.float32),
+                                     x.astype(np.float32))
+            assert relerr(y32.cpu().numpy(), y64) < 1e-4, (order, T)

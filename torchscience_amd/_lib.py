@@ -31,6 +31,8 @@ EXPORTED = (
     "tsiir_workspace_init",
     "tsiir_sosfilt_forward",
     "tsiir_lfilter_forward",
+    "tsiir_lfilter_workspace_bytes",
+    "tsiir_lfilter_forward_ws",
     "tsiir_sosfilt_backward_workspace_bytes",
     "tsiir_sosfilt_backward",
     "tsiir_sosfilt_forward_saved",
@@ -136,6 +138,10 @@ def load(path: str | None = None) -> ctypes.CDLL:
     L.tsiir_sosfilt_forward.argtypes = [dp, vp, vp, vp, vp, vp, vp, sz, vp]
     L.tsiir_lfilter_forward.restype = ctypes.c_int
     L.tsiir_lfilter_forward.argtypes = [dp, vp, vp, vp, vp, vp, vp]
+    L.tsiir_lfilter_workspace_bytes.restype = sz
+    L.tsiir_lfilter_workspace_bytes.argtypes = [dp]
+    L.tsiir_lfilter_forward_ws.restype = ctypes.c_int
+    L.tsiir_lfilter_forward_ws.argtypes = [dp, vp, vp, vp, vp, vp, vp, sz, vp]
     L.tsiir_sosfilt_backward_workspace_bytes.restype = sz
     L.tsiir_sosfilt_backward_workspace_bytes.argtypes = [dp]
     L.tsiir_sosfilt_backward.restype = ctypes.c_int

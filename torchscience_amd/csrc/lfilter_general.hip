@@ -33,13 +33,8 @@ __device__ inline void df2t_step(const TACC (&bn)[NMAX + 1], const TACC (&an)[NM
 }
 
 template <typename TACC, int NMAX>
-__global__ __launch_bounds__(64) void lfilter_general_kernel(
-    const TACC* __restrict__ coef, int N, const TACC* __restrict__ x, TACC* __restrict__ y,
-    int64_t B, int64_t T, int64_t x_rows, int64_t x_stride, int64_t y_stride,
-    const TACC* __restrict__ zi, TACC* __restrict__ zf, uint32_t flags) {
-  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= B) return;
-  TACC bn[NMAX + 1], an[NMAX + 1], z[NMAX];
+__device__ inline void load_coef(const TACC* __restrict__ coef, int N, TACC (&bn)[NMAX + 1],
+                                 TACC (&an)[NMAX + 1]) {
   const TACC a0 = coef[N + 1];
   const bool norm = a0 != TACC(1);
 #pragma unroll
@@ -52,49 +47,76 @@ __global__ __launch_bounds__(64) void lfilter_general_kernel(
       an[i] = 0;
     }
   }
+}
+
+// DF2T over xr[n0, n1) from state z (updated in place); outputs to yr when
+// OUT.  Whole blocks of BLK samples have the next block's loads in flight
+// while the current block runs through the recurrence.
+template <typename TACC, int NMAX, bool OUT>
+__device__ inline void run_span(const TACC (&bn)[NMAX + 1], const TACC (&an)[NMAX + 1], int N,
+                                TACC (&z)[NMAX], const TACC* __restrict__ xr,
+                                TACC* __restrict__ yr, int64_t n0, int64_t n1) {
+  constexpr int BLK = kGenBlk<NMAX>;
+  const int64_t nb = BLK > 1 ? n0 + (n1 - n0) / BLK * BLK : n0;
+  if constexpr (BLK > 1) {
+    TACC cur[BLK], nxt[BLK];
+    if (nb > n0) {
+#pragma unroll
+      for (int i = 0; i < BLK; ++i) cur[i] = xr[n0 + i];
+    }
+    for (int64_t m0 = n0; m0 < nb; m0 += BLK) {
+      if (m0 + BLK < nb) {
+#pragma unroll
+        for (int i = 0; i < BLK; ++i) nxt[i] = xr[m0 + BLK + i];
+      }
+#pragma unroll
+      for (int i = 0; i < BLK; ++i) {
+        TACC yn;
+        df2t_step<TACC, NMAX>(bn, an, N, z, cur[i], yn);
+        if (OUT) yr[m0 + i] = yn;
+      }
+#pragma unroll
+      for (int i = 0; i < BLK; ++i) cur[i] = nxt[i];
+    }
+  }
+  for (int64_t n = nb; n < n1; ++n) {
+    TACC yn;
+    df2t_step<TACC, NMAX>(bn, an, N, z, xr[n], yn);
+    if (OUT) yr[n] = yn;
+  }
+}
+
+// Serial form: one lane per channel walks the whole row.
+template <typename TACC, int NMAX>
+__global__ __launch_bounds__(64) void lfilter_general_kernel(
+    const TACC* __restrict__ coef, int N, const TACC* __restrict__ x, TACC* __restrict__ y,
+    int64_t B, int64_t T, int64_t x_rows, int64_t x_stride, int64_t y_stride,
+    const TACC* __restrict__ zi, TACC* __restrict__ zf, uint32_t flags) {
+  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= B) return;
+  TACC bn[NMAX + 1], an[NMAX + 1], z[NMAX];
+  load_coef<TACC, NMAX>(coef, N, bn, an);
   const int64_t zrow = (flags & TSIIR_ZI_BROADCAST) ? 0 : c;
 #pragma unroll
   for (int k = 0; k < NMAX; ++k)
     z[k] = (k < N && (flags & TSIIR_HAS_ZI)) ? zi[zrow * N + k] : TACC(0);
   const TACC* xr = x + (c % x_rows) * x_stride;
   TACC* yr = y + c * y_stride;
-  // Whole blocks of BLK samples: the next block's loads are in flight while
-  // the current block runs through the recurrence.
-  constexpr int BLK = kGenBlk<NMAX>;
-  const int64_t Tb = BLK > 1 ? T / BLK * BLK : 0;
-  if constexpr (BLK > 1) {
-    TACC cur[BLK], nxt[BLK];
-    if (Tb > 0) {
+  if constexpr (kGenBlk<NMAX> > 1) {
+    run_span<TACC, NMAX, true>(bn, an, N, z, xr, yr, 0, T);
+  } else {
+    // deep states: the loop written out here (through run_span it spills)
+    for (int64_t n = 0; n < T; ++n) {
+      const TACC xn = xr[n];
+      const TACC yn = bn[0] * xn + z[0];
 #pragma unroll
-      for (int i = 0; i < BLK; ++i) cur[i] = xr[i];
+      for (int k = 0; k < NMAX - 1; ++k)
+        if (k < N - 1) z[k] = (bn[k + 1] * xn - an[k + 1] * yn) + z[k + 1];
+#pragma unroll
+      for (int k = 0; k < NMAX; ++k)
+        if (k == N - 1) z[k] = bn[N] * xn - an[N] * yn;
+      yr[n] = yn;
     }
-    for (int64_t n0 = 0; n0 < Tb; n0 += BLK) {
-      if (n0 + BLK < Tb) {
-#pragma unroll
-        for (int i = 0; i < BLK; ++i) nxt[i] = xr[n0 + BLK + i];
-      }
-#pragma unroll
-      for (int i = 0; i < BLK; ++i) {
-        TACC yn;
-        df2t_step<TACC, NMAX>(bn, an, N, z, cur[i], yn);
-        yr[n0 + i] = yn;
-      }
-#pragma unroll
-      for (int i = 0; i < BLK; ++i) cur[i] = nxt[i];
-    }
-  }
-  // the remaining samples one at a time (the same update as df2t_step,
-  // written out: through the helper the order-64 form spills)
-  for (int64_t n = Tb; n < T; ++n) {
-    const TACC xn = xr[n];
-    const TACC yn = bn[0] * xn + z[0];
-#pragma unroll
-    for (int k = 0; k < NMAX - 1; ++k)
-      if (k < N - 1) z[k] = (bn[k + 1] * xn - an[k + 1] * yn) + z[k + 1];
-#pragma unroll
-    for (int k = 0; k < NMAX; ++k)
-      if (k == N - 1) z[k] = bn[N] * xn - an[N] * yn;
-    yr[n] = yn;
   }
   if (flags & TSIIR_WRITE_ZF) {
 #pragma unroll
@@ -103,10 +125,156 @@ __global__ __launch_bounds__(64) void lfilter_general_kernel(
   }
 }
 
+// Chunked form.  Lane (c, j) owns samples [j L, min((j + 1) L, T)) of
+// channel c, L = kGenChunk.  First pass (SECOND false): filter the chunk from
+// zero state and store the final state (FP64) in ws[(c M + j) N ..].  The
+// scan kernel replaces those with the chunk start states.  Second pass:
+// filter the chunk from its start state, writing y (and zf from the last
+// chunk).
+constexpr int kGenChunk = 256;
+constexpr int kGenChunkLog2 = 8;
+
+template <typename TACC, int NMAX, bool SECOND>
+__global__ __launch_bounds__(64) void lfilter_chunk_kernel(
+    const TACC* __restrict__ coef, int N, const TACC* __restrict__ x, TACC* __restrict__ y,
+    int64_t B, int64_t T, int64_t M, int64_t x_rows, int64_t x_stride, int64_t y_stride,
+    double* __restrict__ ws, TACC* __restrict__ zf, uint32_t flags) {
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= B * M) return;
+  const int64_t c = idx / M, j = idx - c * M;
+  TACC bn[NMAX + 1], an[NMAX + 1], z[NMAX];
+  load_coef<TACC, NMAX>(coef, N, bn, an);
+  double* st = ws + idx * N;
+#pragma unroll
+  for (int k = 0; k < NMAX; ++k) z[k] = (SECOND && k < N) ? static_cast<TACC>(st[k]) : TACC(0);
+  const int64_t n0 = j * kGenChunk;
+  const int64_t n1 = n0 + kGenChunk < T ? n0 + kGenChunk : T;
+  run_span<TACC, NMAX, SECOND>(bn, an, N, z, x + (c % x_rows) * x_stride, y + c * y_stride,
+                               n0, n1);
+  if (!SECOND) {
+#pragma unroll
+    for (int k = 0; k < NMAX; ++k)
+      if (k < N) st[k] = static_cast<double>(z[k]);
+  } else if ((flags & TSIIR_WRITE_ZF) && j == M - 1) {
+#pragma unroll
+    for (int k = 0; k < NMAX; ++k)
+      if (k < N) zf[c * N + k] = z[k];
+  }
+}
+
+// Chunk start states in FP64: S_0 = zi (or 0), S_{j+1} = A^L S_j + s_j with
+// s_j the zero-state final state of chunk j and A the DF2T state matrix
+// (A[r][0] = -a_{r+1}, A[r][r+1] = 1, a normalised by a0).  The workgroup
+// forms P = A^L by repeated squaring in LDS; then one lane per channel
+// walks its chunks.
+template <typename TACC, int NMAX>
+__global__ __launch_bounds__(64) void lfilter_scan_kernel(const TACC* __restrict__ coef, int N,
+                                                          int64_t B, int64_t M,
+                                                          double* __restrict__ ws,
+                                                          const TACC* __restrict__ zi,
+                                                          uint32_t flags) {
+  __shared__ double P[NMAX * NMAX], Q[NMAX * NMAX];
+  const int l = threadIdx.x;
+  const double a0 = static_cast<double>(coef[N + 1]);
+  const bool norm = a0 != 1.0;
+  for (int e = l; e < N * N; e += 64) {
+    const int r = e / N, cc = e - r * N;
+    const double ar = static_cast<double>(coef[N + 2 + r]);
+    P[e] = (cc == 0 ? -(norm ? ar / a0 : ar) : 0.0) + (cc == r + 1 ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  for (int sq = 0; sq < kGenChunkLog2; ++sq) {
+    for (int e = l; e < N * N; e += 64) {
+      const int r = e / N, cc = e - r * N;
+      double acc = 0.0;
+      for (int m = 0; m < N; ++m) acc += P[r * N + m] * P[m * N + cc];
+      Q[e] = acc;
+    }
+    __syncthreads();
+    for (int e = l; e < N * N; e += 64) P[e] = Q[e];
+    __syncthreads();
+  }
+  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + l;
+  if (c >= B) return;
+  double S[NMAX];
+  const int64_t zrow = (flags & TSIIR_ZI_BROADCAST) ? 0 : c;
+#pragma unroll
+  for (int k = 0; k < NMAX; ++k)
+    S[k] = (k < N && (flags & TSIIR_HAS_ZI)) ? static_cast<double>(zi[zrow * N + k]) : 0.0;
+  double* w = ws + c * M * N;
+  for (int64_t j = 0; j < M; ++j) {
+    double s[NMAX];
+#pragma unroll
+    for (int k = 0; k < NMAX; ++k)
+      if (k < N) {
+        s[k] = w[j * N + k];
+        w[j * N + k] = S[k];
+      }
+    if (j + 1 < M) {
+      double Sn[NMAX];
+#pragma unroll
+      for (int r = 0; r < NMAX; ++r) {
+        if (r < N) {
+          double acc = s[r];
+#pragma unroll
+          for (int m = 0; m < NMAX; ++m)
+            if (m < N) acc += P[r * N + m] * S[m];
+          Sn[r] = acc;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < NMAX; ++r)
+        if (r < N) S[r] = Sn[r];
+    }
+  }
+}
+
+// chunks per channel of the chunked form, or 0 when the serial form is used
+// (order above 8, fewer than two chunks, or no workspace).  Above order 8 the
+// companion-form state matrix is too ill-conditioned for A^L formed by
+// repeated squaring: at order 16 (a Butterworth lowpass) the start states
+// lose all but three digits in FP64.
+constexpr int kGenChunkMaxOrder = 8;
+
+int64_t general_chunks(const tsiir_sosfilt_desc* d) {
+  if (d->n_sections > kGenChunkMaxOrder) return 0;
+  const int64_t M = (d->n_samples + kGenChunk - 1) / kGenChunk;
+  return M >= 2 ? M : 0;
+}
+
 template <typename TACC>
 static tsiir_status launch_general(const tsiir_sosfilt_desc* d, const void* coef, const void* x,
-                                   void* y, const void* zi, void* zf, hipStream_t st) {
+                                   void* y, const void* zi, void* zf, void* ws, hipStream_t st) {
   const int N = d->n_sections;
+  const int64_t M = ws ? general_chunks(d) : 0;
+  if (M > 0) {
+    const int64_t lanes = d->channels * M;
+    if ((lanes + 63) / 64 > 0x7fffffff) return TSIIR_EUNSUPPORTED;
+    const unsigned cblocks = static_cast<unsigned>((lanes + 63) / 64);
+    const unsigned sblocks = static_cast<unsigned>((d->channels + 63) / 64);
+    auto go = [&](auto first, auto scan, auto second) {
+      hipLaunchKernelGGL(first, dim3(cblocks), dim3(64), 0, st, static_cast<const TACC*>(coef),
+                         N, static_cast<const TACC*>(x), static_cast<TACC*>(y), d->channels,
+                         d->n_samples, M, d->x_rows, d->x_row_stride, d->y_row_stride,
+                         static_cast<double*>(ws), static_cast<TACC*>(zf), d->flags);
+      hipLaunchKernelGGL(scan, dim3(sblocks), dim3(64), 0, st, static_cast<const TACC*>(coef), N,
+                         d->channels, M, static_cast<double*>(ws),
+                         static_cast<const TACC*>(zi), d->flags);
+      hipLaunchKernelGGL(second, dim3(cblocks), dim3(64), 0, st,
+                         static_cast<const TACC*>(coef), N, static_cast<const TACC*>(x),
+                         static_cast<TACC*>(y), d->channels, d->n_samples, M, d->x_rows,
+                         d->x_row_stride, d->y_row_stride, static_cast<double*>(ws),
+                         static_cast<TACC*>(zf), d->flags);
+    };
+    if (N <= 4)
+      go(lfilter_chunk_kernel<TACC, 4, false>, lfilter_scan_kernel<TACC, 4>,
+         lfilter_chunk_kernel<TACC, 4, true>);
+    else
+      go(lfilter_chunk_kernel<TACC, kGenChunkMaxOrder, false>,
+         lfilter_scan_kernel<TACC, kGenChunkMaxOrder>,
+         lfilter_chunk_kernel<TACC, kGenChunkMaxOrder, true>);
+    return hipGetLastError() == hipSuccess ? TSIIR_OK : TSIIR_ELAUNCH;
+  }
   const unsigned blocks = static_cast<unsigned>((d->channels + 63) / 64);
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(64), 0, st, static_cast<const TACC*>(coef), N,
@@ -129,9 +297,8 @@ static tsiir_status launch_general(const tsiir_sosfilt_desc* d, const void* coef
 
 using namespace tsiir;
 
-extern "C" tsiir_status tsiir_lfilter_forward(const tsiir_sosfilt_desc* d, const void* coef,
-                                              const void* x, void* y, const void* zi, void* zf,
-                                              void* stream) {
+static tsiir_status check_general(const tsiir_sosfilt_desc* d, const void* coef, const void* x,
+                                  void* y, const void* zi, void* zf) {
   if (!d || d->channels < 0 || d->n_samples < 0) return TSIIR_EINVAL;
   if (d->n_sections < 1 || d->n_sections > 64) return TSIIR_EUNSUPPORTED;
   if (d->io_dtype != d->coef_dtype) return TSIIR_EINVAL;
@@ -139,9 +306,36 @@ extern "C" tsiir_status tsiir_lfilter_forward(const tsiir_sosfilt_desc* d, const
   if (d->x_rows < 1) return TSIIR_EINVAL;
   if ((d->flags & TSIIR_HAS_ZI) && !zi) return TSIIR_EINVAL;
   if ((d->flags & TSIIR_WRITE_ZF) && !zf) return TSIIR_EINVAL;
-  if (d->channels == 0) return TSIIR_OK;
-  if (!coef || (d->n_samples > 0 && (!x || !y))) return TSIIR_EINVAL;
+  if (d->channels > 0 && (!coef || (d->n_samples > 0 && (!x || !y)))) return TSIIR_EINVAL;
+  return TSIIR_OK;
+}
+
+extern "C" tsiir_status tsiir_lfilter_forward(const tsiir_sosfilt_desc* d, const void* coef,
+                                              const void* x, void* y, const void* zi, void* zf,
+                                              void* stream) {
+  const tsiir_status s = check_general(d, coef, x, y, zi, zf);
+  if (s != TSIIR_OK || d->channels == 0) return s;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (d->io_dtype == TSIIR_F64) return launch_general<double>(d, coef, x, y, zi, zf, st);
-  return launch_general<float>(d, coef, x, y, zi, zf, st);
+  if (d->io_dtype == TSIIR_F64)
+    return launch_general<double>(d, coef, x, y, zi, zf, nullptr, st);
+  return launch_general<float>(d, coef, x, y, zi, zf, nullptr, st);
+}
+
+extern "C" size_t tsiir_lfilter_workspace_bytes(const tsiir_sosfilt_desc* d) {
+  if (!d || d->channels <= 0 || d->n_samples <= 0 || d->n_sections < 1) return 0;
+  return static_cast<size_t>(d->channels) * general_chunks(d) * d->n_sections * sizeof(double);
+}
+
+extern "C" tsiir_status tsiir_lfilter_forward_ws(const tsiir_sosfilt_desc* d, const void* coef,
+                                                 const void* x, void* y, const void* zi,
+                                                 void* zf, void* workspace,
+                                                 size_t workspace_bytes, void* stream) {
+  const tsiir_status s = check_general(d, coef, x, y, zi, zf);
+  if (s != TSIIR_OK || d->channels == 0) return s;
+  const size_t need = tsiir_lfilter_workspace_bytes(d);
+  if (need > 0 && (!workspace || workspace_bytes < need)) return TSIIR_EWORKSPACE;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  void* ws = need > 0 ? workspace : nullptr;
+  if (d->io_dtype == TSIIR_F64) return launch_general<double>(d, coef, x, y, zi, zf, ws, st);
+  return launch_general<float>(d, coef, x, y, zi, zf, ws, st);
 }

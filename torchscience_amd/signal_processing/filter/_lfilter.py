@@ -142,7 +142,8 @@ def _lfilter_as_section(b, a, x, axis, zi, order):
 
 def _general_rows(coef, rows, zi_r, want_zf, zi_broadcast):
     """DF2T of order N = len(coef) // 2 - 1 (coef = b then a, padded) over
-    rows (B, T) through tsiir_lfilter_forward.  Returns (y, zf or None)."""
+    rows (B, T) through tsiir_lfilter_forward_ws (time-parallel for orders up
+    to 8).  Returns (y, zf or None)."""
     B, T = rows.shape
     N = coef.numel() // 2 - 1
     dev = rows.device
@@ -159,9 +160,11 @@ def _general_rows(coef, rows, zi_r, want_zf, zi_broadcast):
               coef_dtype=rows.dtype, flags=flags, x_rows=B, x_row_stride=T, y_row_stride=T,
               chans_per_filter=B)
     L = _lib.load()
-    _lib.check(L.tsiir_lfilter_forward(d, _lib.ptr(coef), _lib.ptr(rows), _lib.ptr(y),
-                                       _lib.ptr(zi_r), _lib.ptr(zf), _lib.stream_handle(dev)),
-               "tsiir_lfilter_forward")
+    nbytes = L.tsiir_lfilter_workspace_bytes(d)
+    ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
+    _lib.check(L.tsiir_lfilter_forward_ws(d, _lib.ptr(coef), _lib.ptr(rows), _lib.ptr(y),
+                                          _lib.ptr(zi_r), _lib.ptr(zf), _lib.ptr(ws), nbytes,
+                                          _lib.stream_handle(dev)), "tsiir_lfilter_forward_ws")
     return y, zf
 
 
