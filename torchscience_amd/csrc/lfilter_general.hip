@@ -125,32 +125,101 @@ __global__ __launch_bounds__(64) void lfilter_general_kernel(
   }
 }
 
-// Chunked form.  Lane (c, j) owns samples [j L, min((j + 1) L, T)) of
-// channel c, L = kGenChunk.  First pass (SECOND false): filter the chunk from
-// zero state and store the final state (FP64) in ws[(c M + j) N ..].  The
-// scan kernel replaces those with the chunk start states.  Second pass:
-// filter the chunk from its start state, writing y (and zf from the last
-// chunk).
+// Chunked form.  Workgroup (c, g) owns chunks 64 g .. 64 g + 63 of channel
+// c, lane l chunk j = 64 g + l, samples [j L, min((j + 1) L, T)), L =
+// kGenChunk.  First pass (SECOND false): filter the chunk from zero state
+// and store the final state (FP64) in ws[(c M + j) N ..].  The scan kernel
+// replaces those with the chunk start states.  Second pass: filter the
+// chunk from its start state, writing y (and zf from the last chunk).
+// Full chunks advance together in blocks of BLK samples; in the second pass
+// each block's outputs go through an LDS tile and are stored as contiguous
+// BLK-sample segments (a store instruction covers 64 / BLK segments rather
+// than 64 scattered addresses).  A partial last chunk runs after that,
+// sample by sample.
 constexpr int kGenChunk = 256;
 constexpr int kGenChunkLog2 = 8;
 
 template <typename TACC, int NMAX, bool SECOND>
 __global__ __launch_bounds__(64) void lfilter_chunk_kernel(
     const TACC* __restrict__ coef, int N, const TACC* __restrict__ x, TACC* __restrict__ y,
-    int64_t B, int64_t T, int64_t M, int64_t x_rows, int64_t x_stride, int64_t y_stride,
-    double* __restrict__ ws, TACC* __restrict__ zf, uint32_t flags) {
-  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (idx >= B * M) return;
-  const int64_t c = idx / M, j = idx - c * M;
+    int64_t B, int64_t T, int64_t M, int64_t G, int64_t x_rows, int64_t x_stride,
+    int64_t y_stride, double* __restrict__ ws, TACC* __restrict__ zf, uint32_t flags) {
+  constexpr int BLK = kGenBlk<NMAX>;
+  static_assert(BLK > 1 && kGenChunk % BLK == 0, "chunked form needs register blocks");
+  // tile rows of BLK samples padded to a multiple of 4 (16-byte vectors)
+  constexpr int TS = BLK + 4;
+  constexpr int V = 16 / sizeof(TACC);            // samples per 16-byte vector
+  __shared__ __align__(16) TACC tile[SECOND ? 64 * TS : 1];
+  const int l = threadIdx.x;
+  const int64_t c = blockIdx.x / G;
+  const int64_t g = blockIdx.x - c * G;
+  const int64_t j = g * 64 + l;
+  const bool valid = j < M;
+  const int64_t n0 = j * kGenChunk;
+  const int64_t n1 = valid ? (n0 + kGenChunk < T ? n0 + kGenChunk : T) : n0;
+  const bool full = valid && n1 - n0 == kGenChunk;
   TACC bn[NMAX + 1], an[NMAX + 1], z[NMAX];
   load_coef<TACC, NMAX>(coef, N, bn, an);
-  double* st = ws + idx * N;
+  double* st = ws + (c * M + j) * N;
 #pragma unroll
-  for (int k = 0; k < NMAX; ++k) z[k] = (SECOND && k < N) ? static_cast<TACC>(st[k]) : TACC(0);
-  const int64_t n0 = j * kGenChunk;
-  const int64_t n1 = n0 + kGenChunk < T ? n0 + kGenChunk : T;
-  run_span<TACC, NMAX, SECOND>(bn, an, N, z, x + (c % x_rows) * x_stride, y + c * y_stride,
-                               n0, n1);
+  for (int k = 0; k < NMAX; ++k)
+    z[k] = (SECOND && valid && k < N) ? static_cast<TACC>(st[k]) : TACC(0);
+  const TACC* xr = x + (c % x_rows) * x_stride;
+  TACC* yr = y + c * y_stride;
+  TACC cur[BLK], nxt[BLK];
+  if (full) {
+#pragma unroll
+    for (int i = 0; i < BLK; ++i) cur[i] = xr[n0 + i];
+  }
+  for (int sb = 0; sb < kGenChunk / BLK; ++sb) {
+    const int64_t m0 = n0 + sb * BLK;
+    if (full && sb + 1 < kGenChunk / BLK) {
+#pragma unroll
+      for (int i = 0; i < BLK; ++i) nxt[i] = xr[m0 + BLK + i];
+    }
+    TACC yv[BLK];
+    if (full) {
+#pragma unroll
+      for (int i = 0; i < BLK; ++i) df2t_step<TACC, NMAX>(bn, an, N, z, cur[i], yv[i]);
+    }
+    if constexpr (SECOND) {
+      if (full) {
+#pragma unroll
+        for (int i = 0; i < BLK; ++i) tile[l * TS + i] = yv[i];
+      }
+      __syncthreads();
+      // vector e = l + 64 q: source lane e / (BLK / V), its samples from
+      // V (e % (BLK / V)); stored whole when the row is 16-byte aligned
+      constexpr int VPR = BLK / V;                  // vectors per tile row
+      const bool vec_ok = ((reinterpret_cast<uintptr_t>(yr) | (y_stride * sizeof(TACC))) & 15) == 0;
+#pragma unroll
+      for (int q = 0; q < VPR; ++q) {
+        const int e = l + 64 * q;
+        const int src = e / VPR, part = e % VPR;
+        const int64_t js = g * 64 + src;
+        if ((js + 1) * kGenChunk > T) continue;
+        TACC* dst = yr + js * kGenChunk + sb * BLK + part * V;
+        const TACC* from = tile + src * TS + part * V;
+        if (vec_ok) {
+          *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(from);
+        } else {
+#pragma unroll
+          for (int v = 0; v < V; ++v) dst[v] = from[v];
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < BLK; ++i) cur[i] = nxt[i];
+  }
+  if (valid && !full) {
+    for (int64_t n = n0; n < n1; ++n) {
+      TACC yn;
+      df2t_step<TACC, NMAX>(bn, an, N, z, xr[n], yn);
+      if (SECOND) yr[n] = yn;
+    }
+  }
+  if (!valid) return;
   if (!SECOND) {
 #pragma unroll
     for (int k = 0; k < NMAX; ++k)
@@ -166,7 +235,8 @@ __global__ __launch_bounds__(64) void lfilter_chunk_kernel(
 // s_j the zero-state final state of chunk j and A the DF2T state matrix
 // (A[r][0] = -a_{r+1}, A[r][r+1] = 1, a normalised by a0).  The workgroup
 // forms P = A^L by repeated squaring in LDS; then one lane per channel
-// walks its chunks.
+// walks its chunks with P in registers, the chunk states loaded in groups
+// with the next group in flight.
 template <typename TACC, int NMAX>
 __global__ __launch_bounds__(64) void lfilter_scan_kernel(const TACC* __restrict__ coef, int N,
                                                           int64_t B, int64_t M,
@@ -196,36 +266,52 @@ __global__ __launch_bounds__(64) void lfilter_scan_kernel(const TACC* __restrict
   }
   const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + l;
   if (c >= B) return;
+  double Pr[NMAX][NMAX];
+#pragma unroll
+  for (int r = 0; r < NMAX; ++r)
+#pragma unroll
+    for (int m = 0; m < NMAX; ++m) Pr[r][m] = (r < N && m < N) ? P[r * N + m] : 0.0;
   double S[NMAX];
   const int64_t zrow = (flags & TSIIR_ZI_BROADCAST) ? 0 : c;
+  double* w = ws + c * M * N;
 #pragma unroll
   for (int k = 0; k < NMAX; ++k)
     S[k] = (k < N && (flags & TSIIR_HAS_ZI)) ? static_cast<double>(zi[zrow * N + k]) : 0.0;
-  double* w = ws + c * M * N;
-  for (int64_t j = 0; j < M; ++j) {
-    double s[NMAX];
+  // chunk states in groups of kScanGroup, the next group's loads in flight
+  constexpr int kScanGroup = 4;
+  double cur[kScanGroup][NMAX], nxt[kScanGroup][NMAX];
 #pragma unroll
-    for (int k = 0; k < NMAX; ++k)
-      if (k < N) {
-        s[k] = w[j * N + k];
-        w[j * N + k] = S[k];
-      }
-    if (j + 1 < M) {
-      double Sn[NMAX];
+  for (int q = 0; q < kScanGroup; ++q)
 #pragma unroll
-      for (int r = 0; r < NMAX; ++r) {
-        if (r < N) {
-          double acc = s[r];
+    for (int k = 0; k < NMAX; ++k) cur[q][k] = (q < M && k < N) ? w[q * N + k] : 0.0;
+  for (int64_t j0 = 0; j0 < M; j0 += kScanGroup) {
 #pragma unroll
-          for (int m = 0; m < NMAX; ++m)
-            if (m < N) acc += P[r * N + m] * S[m];
+    for (int q = 0; q < kScanGroup; ++q)
+#pragma unroll
+      for (int k = 0; k < NMAX; ++k)
+        nxt[q][k] = (j0 + kScanGroup + q < M && k < N) ? w[(j0 + kScanGroup + q) * N + k] : 0.0;
+#pragma unroll
+    for (int q = 0; q < kScanGroup; ++q) {
+      if (j0 + q < M) {
+#pragma unroll
+        for (int k = 0; k < NMAX; ++k)
+          if (k < N) w[(j0 + q) * N + k] = S[k];
+        double Sn[NMAX];
+#pragma unroll
+        for (int r = 0; r < NMAX; ++r) {
+          double acc = cur[q][r];
+#pragma unroll
+          for (int m = 0; m < NMAX; ++m) acc += Pr[r][m] * S[m];
           Sn[r] = acc;
         }
-      }
 #pragma unroll
-      for (int r = 0; r < NMAX; ++r)
-        if (r < N) S[r] = Sn[r];
+        for (int r = 0; r < NMAX; ++r) S[r] = Sn[r];
+      }
     }
+#pragma unroll
+    for (int q = 0; q < kScanGroup; ++q)
+#pragma unroll
+      for (int k = 0; k < NMAX; ++k) cur[q][k] = nxt[q][k];
   }
 }
 
@@ -248,21 +334,21 @@ static tsiir_status launch_general(const tsiir_sosfilt_desc* d, const void* coef
   const int N = d->n_sections;
   const int64_t M = ws ? general_chunks(d) : 0;
   if (M > 0) {
-    const int64_t lanes = d->channels * M;
-    if ((lanes + 63) / 64 > 0x7fffffff) return TSIIR_EUNSUPPORTED;
-    const unsigned cblocks = static_cast<unsigned>((lanes + 63) / 64);
+    const int64_t G = (M + 63) / 64;   // workgroups per channel
+    if (d->channels * G > 0x7fffffff) return TSIIR_EUNSUPPORTED;
+    const unsigned cblocks = static_cast<unsigned>(d->channels * G);
     const unsigned sblocks = static_cast<unsigned>((d->channels + 63) / 64);
     auto go = [&](auto first, auto scan, auto second) {
       hipLaunchKernelGGL(first, dim3(cblocks), dim3(64), 0, st, static_cast<const TACC*>(coef),
                          N, static_cast<const TACC*>(x), static_cast<TACC*>(y), d->channels,
-                         d->n_samples, M, d->x_rows, d->x_row_stride, d->y_row_stride,
+                         d->n_samples, M, G, d->x_rows, d->x_row_stride, d->y_row_stride,
                          static_cast<double*>(ws), static_cast<TACC*>(zf), d->flags);
       hipLaunchKernelGGL(scan, dim3(sblocks), dim3(64), 0, st, static_cast<const TACC*>(coef), N,
                          d->channels, M, static_cast<double*>(ws),
                          static_cast<const TACC*>(zi), d->flags);
       hipLaunchKernelGGL(second, dim3(cblocks), dim3(64), 0, st,
                          static_cast<const TACC*>(coef), N, static_cast<const TACC*>(x),
-                         static_cast<TACC*>(y), d->channels, d->n_samples, M, d->x_rows,
+                         static_cast<TACC*>(y), d->channels, d->n_samples, M, G, d->x_rows,
                          d->x_row_stride, d->y_row_stride, static_cast<double*>(ws),
                          static_cast<TACC*>(zf), d->flags);
     };
