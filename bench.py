@@ -17,7 +17,9 @@ Other workloads (--workload): c2b (per-channel order-8 lowpass filters),
 c2c (x [8, 2^24]), c5 (order-16 Chebyshev I, x [2048, 2^20], fp32 / bf16io),
 c3 (forward+backward of the differentiation workload), fir / fir64 (lfilter
 with a = [1]: 101-tap lowpass over x [4096, 65536], FP32 / FP64; its own
-metric, G sample*taps/s, with VALU and HBM rooflines).
+metric, G sample*taps/s, with VALU and HBM rooflines), iir8 (general-order
+lfilter: order-8 Butterworth lowpass in direct form over x [4096, 65536]
+FP32; G samples/s).
 """
 
 from __future__ import annotations
@@ -48,7 +50,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=50)
     p.add_argument("--workload", default="c2a",
                    choices=["c2a", "c2b", "c2c", "c5", "c5c", "c5bf16", "c3", "c4", "fir",
-                            "fir64"])
+                            "fir64", "iir8"])
     p.add_argument("--requests", type=int, default=10000)
     p.add_argument("--streams", type=int, default=2000)
     p.add_argument("--rate", type=float, default=20000.0, help="c4 Poisson arrivals per second")
@@ -259,6 +261,60 @@ def fir_bench(args, dev):
     return out
 
 
+def iir8_bench(args, dev):
+    """lfilter(b, a, x) for an order-8 Butterworth lowpass (cutoff 0.2) in
+    direct form over x [4096, 65536] FP32: the general-order kernel."""
+    import numpy as np
+    import torch
+    from scipy.signal import butter
+    from torchscience_amd.signal_processing.filter import lfilter
+    B, T = 4096, 65536
+    g = torch.Generator(device=dev)
+    g.manual_seed(8)
+    x = torch.randn(B, T, device=dev, generator=g)
+    b_np, a_np = butter(8, 0.2)
+    b = torch.tensor(b_np, dtype=torch.float32, device=dev)
+    a = torch.tensor(a_np, dtype=torch.float32, device=dev)
+    y = None
+    for _ in range(args.warmup):
+        y = lfilter(b, a, x)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream(dev)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        y = lfilter(b, a, x)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    ev_ms = ev0.elapsed_time(ev1) / args.steps
+    from oracle import oracle
+    errs = []
+    for c in (0, B - 1):
+        ref = oracle.lfilter(b_np.astype(np.float32), a_np.astype(np.float32),
+                             x[c].cpu().numpy())
+        errs.append(float(np.max(np.abs(y[c].double().cpu().numpy() - ref))
+                          / np.max(np.abs(ref))))
+    return {
+        "metric": "general-order lfilter G samples/s (1 GPU)",
+        "value": round(B * T / (ms * 1e-3) / 1e9, 2), "unit": "G samples/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: x ~ N(0,1) from a seeded device generator; order-8 Butterworth "
+                "lowpass 0.2 as (b, a)",
+        "config": {"workload": f"lfilter order 8 (b, a): x[{B},{T}]", "channels": B,
+                   "samples": T, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(B * T * 4 * 2 / (ev_ms * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(B * T * 8 / (ev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "event_ms_per_step": round(ev_ms, 4)},
+        "accuracy": {"max_rel_vs_fp64_oracle": max(errs), "rows_checked": 2},
+        "cpu_baseline": None,
+    }
+
+
 def serving_bench(args, dev):
     """Serving workload: Poisson arrivals over persistent streams; the engine
     (continuous batching, device-resident state) against per-request plain
@@ -383,6 +439,9 @@ def main():
         return
     if args.workload in ("fir", "fir64"):
         print(json.dumps(fir_bench(args, dev)), flush=True)
+        return
+    if args.workload == "iir8":
+        print(json.dumps(iir8_bench(args, dev)), flush=True)
         return
     w = make_workload(args.workload, dev)
     B, T, sos_np = w["B"], w["T"], w["sos"]
