@@ -142,8 +142,8 @@ constexpr int kGenChunkLog2 = 8;
 template <typename TACC, int NMAX, bool SECOND>
 __global__ __launch_bounds__(64) void lfilter_chunk_kernel(
     const TACC* __restrict__ coef, int N, const TACC* __restrict__ x, TACC* __restrict__ y,
-    int64_t B, int64_t T, int64_t M, int64_t G, int64_t x_rows, int64_t x_stride,
-    int64_t y_stride, double* __restrict__ ws, TACC* __restrict__ zf, uint32_t flags) {
+    int64_t T, int64_t M, int64_t G, int64_t x_rows, int64_t x_stride, int64_t y_stride,
+    double* __restrict__ ws, TACC* __restrict__ zf, uint32_t flags) {
   constexpr int BLK = kGenBlk<NMAX>;
   static_assert(BLK > 1 && kGenChunk % BLK == 0, "chunked form needs register blocks");
   // tile rows of BLK samples padded to a multiple of 4 (16-byte vectors)
@@ -340,7 +340,7 @@ static tsiir_status launch_general(const tsiir_sosfilt_desc* d, const void* coef
     const unsigned sblocks = static_cast<unsigned>((d->channels + 63) / 64);
     auto go = [&](auto first, auto scan, auto second) {
       hipLaunchKernelGGL(first, dim3(cblocks), dim3(64), 0, st, static_cast<const TACC*>(coef),
-                         N, static_cast<const TACC*>(x), static_cast<TACC*>(y), d->channels,
+                         N, static_cast<const TACC*>(x), static_cast<TACC*>(y),
                          d->n_samples, M, G, d->x_rows, d->x_row_stride, d->y_row_stride,
                          static_cast<double*>(ws), static_cast<TACC*>(zf), d->flags);
       hipLaunchKernelGGL(scan, dim3(sblocks), dim3(64), 0, st, static_cast<const TACC*>(coef), N,
@@ -348,7 +348,7 @@ static tsiir_status launch_general(const tsiir_sosfilt_desc* d, const void* coef
                          static_cast<const TACC*>(zi), d->flags);
       hipLaunchKernelGGL(second, dim3(cblocks), dim3(64), 0, st,
                          static_cast<const TACC*>(coef), N, static_cast<const TACC*>(x),
-                         static_cast<TACC*>(y), d->channels, d->n_samples, M, G, d->x_rows,
+                         static_cast<TACC*>(y), d->n_samples, M, G, d->x_rows,
                          d->x_row_stride, d->y_row_stride, static_cast<double*>(ws),
                          static_cast<TACC*>(zf), d->flags);
     };
