@@ -130,12 +130,12 @@ tsiir_status tsiir_lfilter_forward(const tsiir_sosfilt_desc* desc, const void* c
                                    const void* x, void* y, const void* zi, void* zf,
                                    void* stream);
 
-/* tsiir_lfilter_forward with a workspace, which lets orders up to 8 run in
+/* tsiir_lfilter_forward with a workspace, which lets orders up to 16 run in
  * time-parallel form: chunks of 256 samples filtered from zero state, the
  * chunk start states scanned in FP64 across the chunks, then every chunk
  * filtered again from its start state.  Workspace: channels x chunks x order
  * FP64 values (tsiir_lfilter_workspace_bytes; 0 when the serial form is
- * used, for orders above 8 or fewer than 512 samples).  Same results to
+ * used, for orders above 16 or fewer than 512 samples).  Same results to
  * rounding; same reference loop replaced. */
 size_t tsiir_lfilter_workspace_bytes(const tsiir_sosfilt_desc* desc);
 tsiir_status tsiir_lfilter_forward_ws(const tsiir_sosfilt_desc* desc, const void* coef,

@@ -267,10 +267,10 @@ def test_lfilter_general_order_and_complex(golden):
 
 @pytest.mark.parametrize("order", [3, 8, 16])
 def test_lfilter_general_chunked_matches_serial_oracle(oracle_mod, order):
-    """Orders up to 8 over at least 512 samples run in time-parallel form
+    """Orders up to 16 over at least 512 samples run in time-parallel form
     (chunks of 256 from zero state, FP64 start-state scan, second pass):
     equal to the oracle's serial DF2T, zi per row and shared, zf, including
-    a partial last chunk.  Order 16 takes the serial form."""
+    a partial last chunk."""
     F, dev = _filt(), _dev()
     import scipy.signal as ss
     b, a = ss.butter(order, 0.23)

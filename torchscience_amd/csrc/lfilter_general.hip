@@ -137,7 +137,6 @@ __global__ __launch_bounds__(64) void lfilter_general_kernel(
 // than 64 scattered addresses).  A partial last chunk runs after that,
 // sample by sample.
 constexpr int kGenChunk = 256;
-constexpr int kGenChunkLog2 = 8;
 
 template <typename TACC, int NMAX, bool SECOND>
 __global__ __launch_bounds__(64) void lfilter_chunk_kernel(
@@ -233,44 +232,55 @@ __global__ __launch_bounds__(64) void lfilter_chunk_kernel(
 
 // Chunk start states in FP64: S_0 = zi (or 0), S_{j+1} = A^L S_j + s_j with
 // s_j the zero-state final state of chunk j and A the DF2T state matrix
-// (A[r][0] = -a_{r+1}, A[r][r+1] = 1, a normalised by a0).  The workgroup
-// forms P = A^L by repeated squaring in LDS; then one lane per channel
-// walks its chunks with P in registers, the chunk states loaded in groups
-// with the next group in flight.
+// (A[r][0] = -a_{r+1}, A[r][r+1] = 1, a normalised by a0).  Lane c < N
+// forms column c of P = A^L by L products with the companion matrix (O(N)
+// each): repeated squaring loses the accuracy of this non-normal matrix
+// (relative error of A^256 for an order-16 Butterworth lowpass above 1,
+// against 4e-10 this way).  Then one lane per channel
+// walks its chunks, P in registers up to order 8 (read from LDS above),
+// the chunk states loaded in groups with the next group in flight.
 template <typename TACC, int NMAX>
 __global__ __launch_bounds__(64) void lfilter_scan_kernel(const TACC* __restrict__ coef, int N,
                                                           int64_t B, int64_t M,
                                                           double* __restrict__ ws,
                                                           const TACC* __restrict__ zi,
                                                           uint32_t flags) {
-  __shared__ double P[NMAX * NMAX], Q[NMAX * NMAX];
+  __shared__ double P[NMAX * NMAX], na[NMAX];
   const int l = threadIdx.x;
   const double a0 = static_cast<double>(coef[N + 1]);
   const bool norm = a0 != 1.0;
-  for (int e = l; e < N * N; e += 64) {
-    const int r = e / N, cc = e - r * N;
-    const double ar = static_cast<double>(coef[N + 2 + r]);
-    P[e] = (cc == 0 ? -(norm ? ar / a0 : ar) : 0.0) + (cc == r + 1 ? 1.0 : 0.0);
+  if (l < N) {
+    const double ar = static_cast<double>(coef[N + 2 + l]);
+    na[l] = -(norm ? ar / a0 : ar);
   }
   __syncthreads();
-  for (int sq = 0; sq < kGenChunkLog2; ++sq) {
-    for (int e = l; e < N * N; e += 64) {
-      const int r = e / N, cc = e - r * N;
-      double acc = 0.0;
-      for (int m = 0; m < N; ++m) acc += P[r * N + m] * P[m * N + cc];
-      Q[e] = acc;
+  if (l < N) {
+    // column l of A^L: e_l through L steps of v <- A v,
+    // (A v)[r] = -a_{r+1} v[0] + v[r+1]
+    double v[NMAX];
+#pragma unroll
+    for (int k = 0; k < NMAX; ++k) v[k] = k == l ? 1.0 : 0.0;
+    for (int it = 0; it < kGenChunk; ++it) {
+      const double v0 = v[0];
+#pragma unroll
+      for (int r = 0; r < NMAX; ++r)
+        if (r < N) v[r] = na[r] * v0 + (r + 1 < N ? v[r + 1] : 0.0);
     }
-    __syncthreads();
-    for (int e = l; e < N * N; e += 64) P[e] = Q[e];
-    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NMAX; ++r)
+      if (r < N) P[r * N + l] = v[r];
   }
+  __syncthreads();
   const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + l;
   if (c >= B) return;
-  double Pr[NMAX][NMAX];
+  constexpr bool kPinRegs = NMAX <= 8;
+  double Pr[kPinRegs ? NMAX : 1][kPinRegs ? NMAX : 1];
+  if constexpr (kPinRegs) {
 #pragma unroll
-  for (int r = 0; r < NMAX; ++r)
+    for (int r = 0; r < NMAX; ++r)
 #pragma unroll
-    for (int m = 0; m < NMAX; ++m) Pr[r][m] = (r < N && m < N) ? P[r * N + m] : 0.0;
+      for (int m = 0; m < NMAX; ++m) Pr[r][m] = (r < N && m < N) ? P[r * N + m] : 0.0;
+  }
   double S[NMAX];
   const int64_t zrow = (flags & TSIIR_ZI_BROADCAST) ? 0 : c;
   double* w = ws + c * M * N;
@@ -301,7 +311,12 @@ __global__ __launch_bounds__(64) void lfilter_scan_kernel(const TACC* __restrict
         for (int r = 0; r < NMAX; ++r) {
           double acc = cur[q][r];
 #pragma unroll
-          for (int m = 0; m < NMAX; ++m) acc += Pr[r][m] * S[m];
+          for (int m = 0; m < NMAX; ++m) {
+            if constexpr (kPinRegs)
+              acc += Pr[r][m] * S[m];
+            else if (r < N && m < N)
+              acc += P[r * N + m] * S[m];
+          }
           Sn[r] = acc;
         }
 #pragma unroll
@@ -316,11 +331,8 @@ __global__ __launch_bounds__(64) void lfilter_scan_kernel(const TACC* __restrict
 }
 
 // chunks per channel of the chunked form, or 0 when the serial form is used
-// (order above 8, fewer than two chunks, or no workspace).  Above order 8 the
-// companion-form state matrix is too ill-conditioned for A^L formed by
-// repeated squaring: at order 16 (a Butterworth lowpass) the start states
-// lose all but three digits in FP64.
-constexpr int kGenChunkMaxOrder = 8;
+// (order above 16, fewer than two chunks, or no workspace)
+constexpr int kGenChunkMaxOrder = 16;
 
 int64_t general_chunks(const tsiir_sosfilt_desc* d) {
   if (d->n_sections > kGenChunkMaxOrder) return 0;
@@ -355,10 +367,12 @@ static tsiir_status launch_general(const tsiir_sosfilt_desc* d, const void* coef
     if (N <= 4)
       go(lfilter_chunk_kernel<TACC, 4, false>, lfilter_scan_kernel<TACC, 4>,
          lfilter_chunk_kernel<TACC, 4, true>);
+    else if (N <= 8)
+      go(lfilter_chunk_kernel<TACC, 8, false>, lfilter_scan_kernel<TACC, 8>,
+         lfilter_chunk_kernel<TACC, 8, true>);
     else
-      go(lfilter_chunk_kernel<TACC, kGenChunkMaxOrder, false>,
-         lfilter_scan_kernel<TACC, kGenChunkMaxOrder>,
-         lfilter_chunk_kernel<TACC, kGenChunkMaxOrder, true>);
+      go(lfilter_chunk_kernel<TACC, 16, false>, lfilter_scan_kernel<TACC, 16>,
+         lfilter_chunk_kernel<TACC, 16, true>);
     return hipGetLastError() == hipSuccess ? TSIIR_OK : TSIIR_ELAUNCH;
   }
   const unsigned blocks = static_cast<unsigned>((d->channels + 63) / 64);

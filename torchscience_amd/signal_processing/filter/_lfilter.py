@@ -143,7 +143,7 @@ def _lfilter_as_section(b, a, x, axis, zi, order):
 def _general_rows(coef, rows, zi_r, want_zf, zi_broadcast):
     """DF2T of order N = len(coef) // 2 - 1 (coef = b then a, padded) over
     rows (B, T) through tsiir_lfilter_forward_ws (time-parallel for orders up
-    to 8).  Returns (y, zf or None)."""
+    to 16).  Returns (y, zf or None)."""
     B, T = rows.shape
     N = coef.numel() // 2 - 1
     dev = rows.device
