@@ -19,7 +19,8 @@ c3 (forward+backward of the differentiation workload), fir / fir64 (lfilter
 with a = [1]: 101-tap lowpass over x [4096, 65536], FP32 / FP64; its own
 metric, G sample*taps/s, with VALU and HBM rooflines), iir8 (general-order
 lfilter: order-8 Butterworth lowpass in direct form over x [4096, 65536]
-FP32; G samples/s).
+FP32; G samples/s), design (batched design of 4096 order-8 Butterworth
+lowpass filters; filters/s).
 """
 
 from __future__ import annotations
@@ -50,7 +51,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=50)
     p.add_argument("--workload", default="c2a",
                    choices=["c2a", "c2b", "c2c", "c5", "c5c", "c5bf16", "c3", "c4", "fir",
-                            "fir64", "iir8"])
+                            "fir64", "iir8", "design"])
     p.add_argument("--requests", type=int, default=10000)
     p.add_argument("--streams", type=int, default=2000)
     p.add_argument("--rate", type=float, default=20000.0, help="c4 Poisson arrivals per second")
@@ -246,14 +247,17 @@ def fir_bench(args, dev):
         "accuracy": {"max_rel_vs_fp64_oracle": max(errs), "rows_checked": len(rows)},
     }
     if not args.no_cpu_baseline:
-        # single-threaded numpy convolution (the oracle) over whole rows
-        xs = x[:64].double().cpu().numpy()
-        n = 0
-        t0 = time.perf_counter()
-        while n < xs.shape[0] and time.perf_counter() - t0 < args.cpu_seconds:
-            oracle.fir_rows(h_np, xs[n:n + 1])
-            n += 1
-        dtc = time.perf_counter() - t0
+        # single-threaded numpy convolution (the oracle) over whole rows, as
+        # many as fit the time budget (rows copied to the host 64 at a time,
+        # outside the timed part)
+        n, dtc = 0, 0.0
+        while n < B and dtc < args.cpu_seconds:
+            xs = x[n:n + 64].double().cpu().numpy()
+            t0 = time.perf_counter()
+            for r in range(xs.shape[0]):
+                oracle.fir_rows(h_np, xs[r:r + 1])
+                n += 1
+            dtc += time.perf_counter() - t0
         out["cpu_baseline"] = {"value": n * T * K / dtc / 1e9, "unit": "G sample·taps/s",
                                "cores": 1, "kind": "port",
                                "sample": f"{n} of {B} rows x full T={T}, numpy.convolve "
@@ -311,8 +315,75 @@ def iir8_bench(args, dev):
                      "frac": round(B * T * 8 / (ev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": None, "event_ms_per_step": round(ev_ms, 4)},
         "accuracy": {"max_rel_vs_fp64_oracle": max(errs), "rows_checked": 2},
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu_baseline else _iir8_cpu(x, b_np, a_np,
+                                                                    args.cpu_seconds),
     }
+
+
+def _iir8_cpu(x, b_np, a_np, budget_s):
+    """The FP64 C oracle's DF2T (OpenMP over rows, all host cores) on whole
+    rows of the same workload, as many rows as fit the time budget."""
+    import numpy as np
+    from oracle import oracle
+    threads = max(1, os.cpu_count() or 1)
+    B, T = x.shape
+    n, dt, t0 = 0, 0.0, time.perf_counter()
+    while n < B and dt < budget_s:
+        rows = x[n:n + threads].double().cpu().numpy()
+        oracle.lfilter_rows(b_np, a_np, rows, n_threads=threads)
+        n += rows.shape[0]
+        dt = time.perf_counter() - t0
+    return {"value": n * T / dt / 1e9, "unit": "G samples/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n} of {B} rows x full T={T}, FP64 DF2T C oracle, OpenMP {threads} "
+                      f"threads, {dt:.1f} s"}
+
+
+def design_bench(args, dev):
+    """Batched design of 4096 order-8 Butterworth lowpass filters (cutoffs
+    U[0.02, 0.45]) into SOS: one step is one call."""
+    import numpy as np
+    import torch
+    from torchscience_amd.signal_processing.filter import batched_butterworth_design
+    n = 4096
+    rng = np.random.default_rng(2)
+    cut_np = rng.uniform(0.02, 0.45, n)
+    cut = torch.tensor(cut_np, device=dev)
+    sos = None
+    for _ in range(args.warmup):
+        sos = batched_butterworth_design(8, cut)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sos = batched_butterworth_design(8, cut)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    from oracle import oracle
+    idx = [0, n // 2, n - 1]
+    ref = np.stack([oracle.iir_design_sos("butterworth", 8, cut_np[i]) for i in idx])
+    err = float(np.max(np.abs(sos[idx].cpu().numpy() - ref)))
+    out = {
+        "metric": "batched IIR design filters/s (1 GPU)",
+        "value": round(n / (ms * 1e-3), 1), "unit": "filters/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: 4096 cutoffs U[0.02, 0.45] (seeded)",
+        "config": {"workload": "batched_butterworth_design(8, cutoffs[4096]) -> (4096, 4, 6)",
+                   "parallelism": "single GPU"},
+        "roofline": None,
+        "accuracy": {"max_abs_vs_fp64_oracle": err, "filters_checked": len(idx)},
+    }
+    if not args.no_cpu_baseline:
+        k, t0 = 0, time.perf_counter()
+        while k < n and time.perf_counter() - t0 < min(args.cpu_seconds, 5.0):
+            oracle.iir_design_sos("butterworth", 8, cut_np[k])
+            k += 1
+        dtc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": k / dtc, "unit": "filters/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{k} of {n} filters, numpy FP64 design chain, "
+                                         f"1 thread, {dtc:.1f} s"}
+    return out
 
 
 def serving_bench(args, dev):
@@ -442,6 +513,9 @@ def main():
         return
     if args.workload == "iir8":
         print(json.dumps(iir8_bench(args, dev)), flush=True)
+        return
+    if args.workload == "design":
+        print(json.dumps(design_bench(args, dev)), flush=True)
         return
     w = make_workload(args.workload, dev)
     B, T, sos_np = w["B"], w["T"], w["sos"]
