@@ -3,8 +3,9 @@
 Checked against the autograd of the reference's DF2T loop restated op for op
 (oracle/torch_ref.py, run on the CPU), which is the reference's backward:
 gradients of x, b, a (a0 gets 0 when a0 == 1) and zi, with zf in the loss,
-zi per row and shared, T shorter than the order; and torch.autograd.gradcheck
-with the reference's settings.
+zi per row and shared, T shorter than the order and long enough (700) for
+the time-parallel form of the forward and of the reverse-time pass; and
+torch.autograd.gradcheck with the reference's settings.
 """
 
 import numpy as np
@@ -20,7 +21,7 @@ def _dev():
     return torch.device("cuda:0")
 
 
-@pytest.mark.parametrize("T", [3, 1, 257])
+@pytest.mark.parametrize("T", [3, 1, 257, 700])
 @pytest.mark.parametrize("a0", [1.0, 2.0])
 def test_lfilter_iir_gradients_match_reference_loop(T, a0):
     from torchscience_amd.signal_processing import filter as F
