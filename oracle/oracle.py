@@ -14,6 +14,8 @@ It restates, in numpy over the C DF2T core in ``oracle.c``:
   zi reshape, DF2T loop);
 * ``batched_filter_apply`` - src/torchscience/signal_processing/filter/_batched.py:449-566;
 * ``sosfilt_zi`` / ``lfilter_zi`` - _sosfilt_zi.py:54-86 / _lfilter_zi.py:70-128;
+* ``lfiltic`` - _lfilter.py:181-305 (a0 normalisation, zero padding of the
+  histories, the state sums worked backwards from the last delay);
 * ``sosfiltfilt`` - _sosfiltfilt.py:96-173 (default padlen from the count of
   zeros at the origin, padding _pad_signal :176-217, forward pass from
   sosfilt_zi * first sample, reversed pass from sosfilt_zi * last output,
@@ -624,3 +626,30 @@ def iir_design_sos(family: str, order: int, cutoff, band: str = "lowpass",
     zd = np.concatenate([zd, -np.ones(deg, np.complex128)])
     pd = (1 + p / 4.0) / (1 - p / 4.0)
     return _design_zpk_to_sos(zd, pd, k)
+
+
+def lfiltic(b, a, y, x=None) -> np.ndarray:
+    """DF2T initial state from past outputs / inputs, most recent first
+    (_lfilter.py:181-305): zi[k] = sum_{j=k}^{N-1} b[j+1] x[j-k] -
+    a[j+1] y[j-k], histories zero-padded to N."""
+    b = np.atleast_1d(np.asarray(b, dtype=np.float64))
+    a = np.atleast_1d(np.asarray(a, dtype=np.float64))
+    y = np.atleast_1d(np.asarray(y, dtype=np.float64))
+    if a[0] != 1.0:
+        b, a = b / a[0], a / a[0]
+    n = max(b.size, a.size)
+    N = n - 1
+    if N == 0:
+        return np.zeros(0)
+    b = np.concatenate([b, np.zeros(n - b.size)])
+    a = np.concatenate([a, np.zeros(n - a.size)])
+    x = np.zeros(N) if x is None else np.atleast_1d(np.asarray(x, dtype=np.float64))
+    y = np.concatenate([y, np.zeros(max(N - y.size, 0))])
+    x = np.concatenate([x, np.zeros(max(N - x.size, 0))])
+    zi = np.zeros(N)
+    for k in range(N - 1, -1, -1):
+        acc = 0.0
+        for j in range(k, N):
+            acc += b[j + 1] * x[j - k] - a[j + 1] * y[j - k]
+        zi[k] = acc
+    return zi

@@ -296,3 +296,27 @@ def test_lfilter_general_chunked_matches_serial_oracle(oracle_mod, order):
             y64 = oracle_mod.lfilter(b.astype(np.float32), a.astype(np.float32),
                                      x.astype(np.float32))
             assert relerr(y32.cpu().numpy(), y64) < 1e-4, (order, T)
+
+
+def test_lfiltic_continuation_on_device():
+    """lfilter over a signal in two parts, the second from lfiltic of the
+    first part's last outputs and inputs (the use shown in
+    _lfilter.py:224-230), equals one pass: order 2 (one section), order 5
+    (general form), a 5-tap FIR, with a0 != 1."""
+    F, dev = _filt(), _dev()
+    import scipy.signal as ss
+    rng = np.random.default_rng(21)
+    b5, a5 = ss.butter(5, 0.2)
+    cases = [(np.array([0.1, 0.2, 0.1]), np.array([1.0, -0.5, 0.1])),
+             (b5 * 2.0, a5 * 2.0), (np.array([0.2, 0.3, 0.1, -0.1, 0.05]), np.array([1.0]))]
+    for b, a in cases:
+        N = max(b.size, a.size) - 1
+        x = torch.tensor(rng.standard_normal(700), device=dev)
+        bt, at = torch.tensor(b, device=dev), torch.tensor(a, device=dev)
+        full = F.lfilter(bt, at, x)
+        y1 = F.lfilter(bt, at, x[:300])
+        zi = F.lfiltic(bt, at, y1.flip(0)[:N], x[:300].flip(0)[:N])
+        assert zi.shape == (N,) and zi.device == x.device
+        y2, _ = F.lfilter(bt, at, x[300:], zi=zi)
+        np.testing.assert_allclose(torch.cat([y1, y2]).cpu().numpy(), full.cpu().numpy(),
+                                   rtol=1e-10, atol=1e-10)

@@ -207,3 +207,27 @@ def test_frequency_response_validation_before_device_work():
     assert issubclass(SOSNormalizationError, FilterDesignError)
     with pytest.raises(SOSNormalizationError, match="a0 = 1"):
         frequency_response_sos(torch.tensor([[1.0, 0.5, 0.25, 2.0, 0.1, 0.05]]))
+
+
+def test_lfiltic_host_math_and_errors():
+    """lfiltic is set-up math on the tensors' device (here the CPU): equal
+    to the oracle's restatement of _lfilter.py:181-305, the reference's
+    ValueErrors, the empty state of a pure gain."""
+    import numpy as np
+    from oracle import oracle
+    from torchscience_amd.signal_processing.filter import lfiltic
+    rng = np.random.default_rng(3)
+    b = np.array([0.1, 0.2, 0.1, 0.05])
+    a = np.array([2.0, -0.5, 0.1])
+    y = rng.standard_normal(5)
+    x = rng.standard_normal(2)
+    zi = lfiltic(torch.tensor(b), torch.tensor(a), torch.tensor(y), torch.tensor(x))
+    np.testing.assert_allclose(zi.numpy(), oracle.lfiltic(b, a, y, x), rtol=1e-12, atol=1e-13)
+    zi = lfiltic(torch.tensor(b), torch.tensor(a), torch.tensor(y))
+    np.testing.assert_allclose(zi.numpy(), oracle.lfiltic(b, a, y), rtol=1e-12, atol=1e-13)
+    assert zi.dtype == torch.float64
+    assert lfiltic(torch.tensor([2.0]), torch.tensor([1.0]), torch.tensor([1.0])).numel() == 0
+    with pytest.raises(ValueError, match="y must be a 1-D tensor"):
+        lfiltic(torch.ones(2), torch.ones(2), torch.ones(2, 2))
+    with pytest.raises(ValueError, match="b must be a 1-D tensor"):
+        lfiltic(torch.ones(2, 2), torch.ones(2), torch.ones(2))

@@ -376,3 +376,33 @@ def test_iir_design_matches_scipy(oracle_mod, family, param):
                 i = int(np.argmin(np.abs(np.array(pool) - zz)))
                 assert abs(pool[i] - zz) < 1e-9
                 pool.pop(i)
+
+
+def test_lfiltic_matches_scipy_and_continues(oracle_mod):
+    """lfiltic against scipy.signal.lfiltic (same convention: most recent
+    first, zero-padded histories; a[0] = 1), and the continuation it exists
+    for: the
+    oracle's lfilter over a signal in two parts, the second started from
+    lfiltic of the first part's last outputs and inputs, equals one pass."""
+    import scipy.signal as ss
+    rng = np.random.default_rng(12)
+    for b, a in [(ss.butter(2, 0.3)), (ss.butter(5, 0.2)), ([0.1, 0.2, 0.1], [2.0, -0.5, 0.1]),
+                 ([1.0, 0.5, 0.25, 0.1], [1.0])]:
+        b, a = np.asarray(b, float), np.asarray(a, float)
+        N = max(b.size, a.size) - 1
+        y = rng.standard_normal(N + 2)
+        xh = rng.standard_normal(N + 1)
+        # scipy's lfiltic does not normalise by a[0] (the reference does,
+        # _lfilter.py:255-257): compare where a[0] = 1; the continuation
+        # below pins every case
+        if a[0] == 1.0:
+            for yy, xx in [(y, xh), (y, None), (y[:1], xh[:2])]:
+                np.testing.assert_allclose(oracle_mod.lfiltic(b, a, yy, xx),
+                                           ss.lfiltic(b, a, yy, xx), rtol=1e-12, atol=1e-13)
+        x = rng.standard_normal(80)
+        full = oracle_mod.lfilter(b, a, x)
+        y1 = oracle_mod.lfilter(b, a, x[:50])
+        zi = oracle_mod.lfiltic(b, a, y1[::-1][:N], x[:50][::-1][:N])
+        y2, _ = oracle_mod.lfilter(b, a, x[50:], zi=zi)
+        np.testing.assert_allclose(np.concatenate([y1, y2]), full, rtol=1e-11, atol=1e-12)
+    assert oracle_mod.lfiltic([2.0], [1.0], [1.0]).size == 0

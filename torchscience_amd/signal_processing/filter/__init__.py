@@ -13,10 +13,10 @@ from ._lfilter import lfilter
 from ._response import (FilterDesignError, SOSNormalizationError, frequency_response_sos,
                         impulse_response, impulse_response_sos, step_response)
 from ._sosfilt import sosfilt
-from ._zi import lfilter_zi, sosfilt_zi
+from ._zi import lfilter_zi, lfiltic, sosfilt_zi
 
 __all__ = ["FilterDesignError", "SOSNormalizationError", "batched_butterworth_design",
            "batched_chebyshev_type_1_design", "batched_chebyshev_type_2_design",
            "batched_filter_apply", "fftfilt",
            "filtfilt", "frequency_response_sos", "impulse_response", "impulse_response_sos",
-           "lfilter", "lfilter_zi", "sosfilt", "sosfilt_zi", "sosfiltfilt", "step_response"]
+           "lfilter", "lfilter_zi", "lfiltic", "sosfilt", "sosfilt_zi", "sosfiltfilt", "step_response"]

@@ -4,12 +4,18 @@ lfilter_zi: src/torchscience/signal_processing/filter/_lfilter_zi.py:70-128
   solve (I - A^T) zi = b[1:] - a[1:] b[0], A the companion matrix of a.
 sosfilt_zi: src/torchscience/signal_processing/filter/_sosfilt_zi.py:54-86
   per section lfilter_zi scaled by the DC gain of the preceding sections.
-Both are small dense solves done with torch ops on the tensors' device.
+lfiltic: src/torchscience/signal_processing/filter/_lfilter.py:181-305
+  the DF2T state that continues a filter from its past outputs y and inputs
+  x (most recent first): zi[k] = sum_i b[k+i+1] x[i] - a[k+i+1] y[i].
+All are small dense computations done with torch ops on the tensors' device.
 """
 
 from __future__ import annotations
 
+from typing import Optional
+
 import torch
+import torch.nn.functional as F
 from torch import Tensor
 
 
@@ -57,3 +63,49 @@ def sosfilt_zi(sos: Tensor) -> Tensor:
         zi[s, :] = scale * lfilter_zi(b, a)
         scale = scale * torch.sum(b) / torch.sum(a)
     return zi
+
+
+def lfiltic(b: Tensor, a: Tensor, y: Tensor, x: Optional[Tensor] = None) -> Tensor:
+    """Initial DF2T state (max(len(b), len(a)) - 1,) under which lfilter
+    continues a filtering whose most recent outputs were y[0], y[1], ... and
+    inputs x[0], x[1], ... (x zeros when None); short histories are padded
+    with zeros (_lfilter.py:181-305).  b and a are normalised by a[0] when
+    a[0] != 1.  zi[k] = sum_{i <= N-1-k} b[k+i+1] x[i] - a[k+i+1] y[i],
+    formed as two Hankel products."""
+    b = torch.atleast_1d(b)
+    a = torch.atleast_1d(a)
+    y = torch.atleast_1d(y)
+    if b.ndim != 1:
+        raise ValueError("b must be a 1-D tensor")
+    if a.ndim != 1:
+        raise ValueError("a must be a 1-D tensor")
+    if y.ndim != 1:
+        raise ValueError("y must be a 1-D tensor")
+    dtype = torch.promote_types(b.dtype, torch.promote_types(a.dtype, y.dtype))
+    if not dtype.is_floating_point:
+        dtype = torch.float64
+    device = y.device
+    b = b.to(dtype=dtype, device=device)
+    a = a.to(dtype=dtype, device=device)
+    y = y.to(dtype=dtype)
+    a0 = a[0]
+    b = torch.where(a0 != 1, b / a0, b)
+    a = torch.where(a0 != 1, a / a0, a)
+    n_coef = max(b.shape[0], a.shape[0])
+    N = n_coef - 1
+    if N == 0:
+        return torch.tensor([], dtype=dtype, device=device)
+    b = F.pad(b, (0, n_coef - b.shape[0]))
+    a = F.pad(a, (0, n_coef - a.shape[0]))
+    x = torch.zeros(N, dtype=dtype, device=device) if x is None else \
+        torch.atleast_1d(x).to(dtype=dtype, device=device)
+    y = F.pad(y, (0, max(N - y.shape[0], 0)))[:N]
+    x = F.pad(x, (0, max(N - x.shape[0], 0)))[:N]
+    k = torch.arange(N, device=device)[:, None]
+    i = torch.arange(N, device=device)[None, :]
+    idx = k + i + 1
+    mask = idx <= N
+    zero = torch.zeros((), dtype=dtype, device=device)
+    hb = torch.where(mask, b[idx.clamp(max=N)], zero)
+    ha = torch.where(mask, a[idx.clamp(max=N)], zero)
+    return hb @ x - ha @ y
