@@ -176,7 +176,8 @@ def test_fftfilt_gradcheck():
 def test_fftfilt_gradients_match_fft_restatement():
     F, dev = _F(), _dev()
     from oracle import torch_ref
-    for K, T in [(21, 100), (64, 3000), (1100, 2500)]:
+    # 2500 taps: the tap-gradient kernel's second 2048-tap pass
+    for K, T in [(21, 100), (64, 3000), (1100, 2500), (2500, 3000)]:
         print("case fftfilt grad", K, T, flush=True)
         b = torch.randn(K, dtype=torch.float64)
         x = torch.randn(3, T, dtype=torch.float64)
