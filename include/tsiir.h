@@ -240,6 +240,17 @@ tsiir_status tsiir_sos_frequency_response(const void* sos, int32_t coef_dtype,
                                           const double* w, int64_t n_freq, double* out,
                                           void* stream);
 
+/* B(z^-1) / A(z^-1) at z = exp(j w[k]) for n_filters direct-form filters:
+ * b (n_filters, nb), a (n_filters, na) FP64 (an empty polynomial is 1:
+ * na = 0 makes an FIR),
+ * out (n_filters, n_freq, 2) interleaved FP64 (re, im).  Both polynomials by
+ * Horner's rule from the highest power.  Replaces the polynomial
+ * evaluations of frequency_response _frequency_response.py:104-116 and
+ * frequency_response_fir _frequency_response_fir.py:66-79. */
+tsiir_status tsiir_ba_frequency_response(const double* b, int32_t nb, const double* a,
+                                         int32_t na, int64_t n_filters, const double* w,
+                                         int64_t n_freq, double* out, void* stream);
+
 /* ---- batched IIR design -------------------------------------------------------
  * FP64 design of n_filters digital filters of one family, band type and
  * order into second-order sections, one cutoff (lowpass / highpass) or one

@@ -33,6 +33,8 @@ It restates, in numpy over the C DF2T core in ``oracle.c``:
   in order with a0 taken as 1);
 * ``sos_frequency_response`` - _frequency_response_sos.py:128-137 (product
   of the section ratios at z = exp(jw));
+* ``ba_frequency_response`` - _frequency_response.py:104-116 (B(z^-1) /
+  A(z^-1) at z = exp(jw));
 * ``iir_design_sos`` - one filter of batched_butterworth_design /
   batched_chebyshev_type_1_design / batched_chebyshev_type_2_design
   (_batched.py:22-344): prototypes (_butterworth_prototype.py:84-91,
@@ -653,3 +655,14 @@ def lfiltic(b, a, y, x=None) -> np.ndarray:
             acc += b[j + 1] * x[j - k] - a[j + 1] * y[j - k]
         zi[k] = acc
     return zi
+
+
+def ba_frequency_response(b, a, w) -> np.ndarray:
+    """B(z^-1) / A(z^-1) at z = exp(jw) (numpy.polyval of the reversed
+    coefficients); an empty polynomial is 1 (_frequency_response.py:139-140)."""
+    zi1 = np.exp(-1j * np.asarray(w, dtype=np.float64))
+
+    def ev(c):
+        c = np.atleast_1d(np.asarray(c, dtype=np.float64))
+        return np.ones_like(zi1) if c.size == 0 else np.polyval(c[::-1], zi1)
+    return ev(b) / ev(a)

@@ -1,10 +1,12 @@
 """Impulse, step and frequency responses through the HIP path (GPU).
 
 Mirrors the reference's tests (test__analysis_functions.py:180-380,
-test__frequency_response.py:21-135): known FIR coefficients and cumulative
-sums, IIR responses against scipy, the Butterworth -3 dB point, unit DC gain,
-stopband attenuation, the a0 validation, whole / sampling-frequency grids;
-plus the batched form against the oracle per filter.  FP64 at 1e-10 unless
+test__frequency_response.py:21-135, 176-290): known FIR coefficients and
+cumulative sums, IIR responses against scipy, the Butterworth -3 dB point,
+unit DC gain, stopband attenuation, the a0 validation, whole /
+sampling-frequency grids; the direct-form and FIR responses against
+scipy.signal.freqz and against the section form; plus the batched form
+against the oracle per filter.  FP64 at 1e-10 unless
 the reference's own test states a looser bound.
 """
 
@@ -140,3 +142,63 @@ def test_frequency_response_batched(oracle_mod):
     _, h32 = F.frequency_response_sos(torch.tensor(sos, dtype=torch.float32, device=dev),
                                       frequencies=64)
     assert h32.dtype == torch.complex64 and h32.shape == (300, 64)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 8])
+def test_frequency_response_ba_matches_scipy_freqz(order, oracle_mod):
+    F, dev = _F(), _dev()
+    import scipy.signal as ss
+    b, a = ss.butter(order, 0.3)
+    a = a * 1.7
+    freqs, h = F.frequency_response(torch.tensor(b, device=dev), torch.tensor(a, device=dev),
+                                    frequencies=256)
+    w, hs = ss.freqz(b, a, worN=256)
+    assert h.dtype == torch.complex128 and h.device.type == "cuda" and h.shape == (256,)
+    assert freqs.dtype == torch.float64
+    np.testing.assert_allclose(h.cpu().numpy(), hs, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(h.cpu().numpy(), oracle_mod.ba_frequency_response(b, a, w),
+                               rtol=1e-12, atol=1e-14)
+
+
+def test_frequency_response_ba_matches_sections():
+    """Direct form and sections of one Butterworth design give one response
+    (test__frequency_response.py:195-209, 262-290)."""
+    F, dev = _F(), _dev()
+    import scipy.signal as ss
+    for order in (2, 3, 4, 5):
+        sos = torch.tensor(ss.butter(order, 0.4, output="sos"), device=dev)
+        b, a = ss.butter(order, 0.4)
+        _, hs = F.frequency_response_sos(sos, frequencies=128)
+        _, hb = F.frequency_response(torch.tensor(b, device=dev), torch.tensor(a, device=dev),
+                                     frequencies=128)
+        np.testing.assert_allclose(hb.cpu().numpy(), hs.cpu().numpy(), rtol=1e-9, atol=1e-12)
+
+
+def test_frequency_response_fir():
+    """Moving average against freqz, the DC gain sum(h), a type I FIR's
+    linear phase, float32 in -> complex64 out, and the grid options
+    (test__frequency_response.py:211-257)."""
+    F, dev = _F(), _dev()
+    import scipy.signal as ss
+    h = torch.full((5,), 0.2, dtype=torch.float64, device=dev)
+    _, r = F.frequency_response_fir(h, frequencies=256)
+    _, hs = ss.freqz(h.cpu().numpy(), [1.0], worN=256)
+    np.testing.assert_allclose(r.cpu().numpy(), hs, rtol=1e-12, atol=1e-14)
+    h32 = torch.tensor([0.1, 0.2, 0.4, 0.2, 0.1], device=dev)
+    f, r = F.frequency_response_fir(h32, frequencies=torch.tensor([0.0], device=dev))
+    assert r.dtype == torch.complex64 and f.dtype == torch.float32
+    assert abs(abs(r.item()) - h32.sum().item()) < 1e-6
+    h64 = h32.double()
+    f, r = F.frequency_response_fir(h64, frequencies=50)
+    w = f.cpu().numpy() * np.pi
+    # symmetric about tap 2: H = real amplitude * exp(-2jw)
+    amp = (r.cpu().numpy() * np.exp(2j * w))
+    np.testing.assert_allclose(amp.imag, 0.0, atol=1e-14)
+    fw, rw = F.frequency_response_fir(h64, frequencies=100, whole=True, sampling_frequency=8.0)
+    assert fw[-1].item() == pytest.approx(8.0 * 99 / 100)
+    _, rn = F.frequency_response_fir(h64, frequencies=100, whole=True)
+    np.testing.assert_allclose(rw.cpu().numpy(), rn.cpu().numpy(), rtol=1e-12, atol=1e-14)
+    _, rc = F.frequency_response_fir(h64.cpu(), frequencies=16)
+    assert rc.device.type == "cpu"
+    with pytest.raises(NotImplementedError):
+        F.frequency_response(h64.to(torch.complex128), torch.ones(1, device=dev))

@@ -319,6 +319,30 @@ def test_responses_match_scipy(oracle_mod):
                                ss.sosfilt(sec1, np.eye(1, 20)[0]), rtol=1e-12, atol=1e-14)
 
 
+def test_ba_frequency_response_matches_scipy_freqz(oracle_mod):
+    """The direct-form restatement equals scipy.signal.freqz, complex values
+    (the reference's own check compares magnitudes,
+    test__frequency_response.py:179-193, 214-229); the phase pins the
+    direction z^-1 (z in its place conjugates H), the DC value is sum(b) /
+    sum(a), and empty polynomials evaluate to 1 (_frequency_response.py:139)."""
+    import scipy.signal as ss
+    w = np.linspace(0, np.pi, 256, endpoint=False)
+    for order in (1, 2, 3, 4, 8):
+        b, a = ss.butter(order, 0.3)
+        a = a * 1.7
+        _, hs = ss.freqz(b, a, worN=256)
+        np.testing.assert_allclose(oracle_mod.ba_frequency_response(b, a, w), hs, rtol=1e-10,
+                                   atol=1e-13)
+    h = np.array([0.1, 0.2, 0.4, 0.2, 0.3])
+    _, hs = ss.freqz(h, [1.0], worN=256)
+    got = oracle_mod.ba_frequency_response(h, [1.0], w)
+    np.testing.assert_allclose(got, hs, rtol=1e-12, atol=1e-14)
+    assert np.max(np.abs(got.imag)) > 0.1
+    np.testing.assert_allclose(oracle_mod.ba_frequency_response(h, [2.0, 0.5], [0.0]),
+                               [h.sum() / 2.5], rtol=1e-14)
+    np.testing.assert_allclose(oracle_mod.ba_frequency_response([], [], w), np.ones(256))
+
+
 def _zpk_of(sos):
     import scipy.signal as ss
     return ss.sos2zpk(sos)

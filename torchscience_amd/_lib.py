@@ -42,6 +42,7 @@ EXPORTED = (
     "tsiir_fir_backward_taps_workspace_bytes",
     "tsiir_fir_backward_taps",
     "tsiir_sos_frequency_response",
+    "tsiir_ba_frequency_response",
     "tsiir_iir_design_sections",
     "tsiir_iir_design_sos",
     "tsiir_iir_design_sos_jacobian",
@@ -169,6 +170,9 @@ def load(path: str | None = None) -> ctypes.CDLL:
     L.tsiir_iir_design_sos.argtypes = [gp, vp, vp, vp]
     L.tsiir_iir_design_sos_jacobian.restype = ctypes.c_int
     L.tsiir_iir_design_sos_jacobian.argtypes = [gp, vp, vp, vp, vp]
+    L.tsiir_ba_frequency_response.restype = ctypes.c_int
+    L.tsiir_ba_frequency_response.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_int32,
+                                              ctypes.c_int64, vp, ctypes.c_int64, vp, vp]
     L.tsiir_serve_table_bytes.restype = sz
     L.tsiir_serve_table_bytes.argtypes = [ctypes.c_int32]
     L.tsiir_serve_prepare_filter.restype = ctypes.c_int
