@@ -7,7 +7,9 @@ fftfilt (the reference's own cases, test__fftfilt.py), the "fir" format of
 batched_filter_apply (broadcast and per-signal), gradients (gradcheck with
 the reference's settings; the reference loop's autograd on the CPU;
 adjoint identities against the oracle forward at larger sizes) and the full
-4096 x 65536 shape through sampled rows.  FP64 at 1e-10, FP32 at 1e-5 of
+4096 x 65536 shape through sampled rows; the FFT (overlap-save) kernel
+through its own entry point against the oracle and the direct kernel, and
+fftfilt at long K against scipy.signal.fftconvolve.  FP64 at 1e-10, FP32 at 1e-5 of
 the output's largest magnitude.
 """
 
@@ -303,3 +305,116 @@ def test_fir_empty_and_errors():
     with pytest.raises(ValueError, match="x batch dim must match"):
         F.batched_filter_apply(torch.ones(2, 3, device=dev), torch.ones(3, 5, device=dev),
                                filter_format="fir", broadcast=False)
+
+
+def _fft_call(h, x, *, channels, shift, n_out, reverse=False, ws_bytes=None):
+    """tsiir_fir_forward_fft through the C-ABI: one shared filter h over
+    `channels` channels reading rows c % rows of x."""
+    from torchscience_amd import _lib
+    from torchscience_amd.signal_processing.filter._fir import _fir_desc
+    dev = x.device
+    y = torch.full((channels, n_out), float("nan"), dtype=x.dtype, device=dev)
+    d = _fir_desc(channels=channels, n_samples=n_out, x_len=x.shape[1], n_taps=h.shape[-1],
+                  dtype=x.dtype, shift=shift, flags=_lib.FIR_REVERSE_TAPS if reverse else 0,
+                  x_rows=x.shape[0], x_row_stride=x.shape[1], y_row_stride=n_out,
+                  chans_per_filter=channels)
+    L = _lib.load()
+    nbytes = L.tsiir_fir_fft_workspace_bytes(d)
+    assert nbytes > 0
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    st = L.tsiir_fir_forward_fft(d, _lib.ptr(h), _lib.ptr(x), _lib.ptr(y), _lib.ptr(ws),
+                                 nbytes if ws_bytes is None else ws_bytes,
+                                 _lib.stream_handle(dev))
+    return st, y
+
+
+@pytest.mark.parametrize("K", [1, 2, 63, 101, 1000, 2048])
+def test_fir_fft_entry_matches_oracle(oracle_mod, K):
+    """Every output block boundary, odd channel counts (the last channel
+    alone in its complex pair), fewer input rows than channels, output
+    longer or shorter than the input, 'same' and reversed-tap shifts; FP64
+    at 1e-10 and FP32 at 1e-5 of the largest output."""
+    from torchscience_amd import _lib
+    dev = _dev()
+    rng = np.random.default_rng(K)
+    h = rng.standard_normal(K)
+    L_blk = 4096 - K + 1
+    for (rows, channels, T_in, T_out, shift, rev) in (
+            (5, 5, 3 * L_blk + 7, 3 * L_blk + 7, 0, False),
+            (3, 7, 2 * L_blk, 2 * L_blk - 1, (K - 1) // 2, False),
+            (2, 2, 1, 1, 0, False),
+            (1, 3, 900, 1200, K - 1, True),
+            (4, 4, L_blk + 1, L_blk + 1, K - 1 - (K - 1) // 2, True)):
+        print("case fir fft", K, rows, channels, T_in, T_out, shift, rev, flush=True)
+        x = rng.standard_normal((rows, T_in))
+        hh = h[::-1].copy() if rev else h
+        xr = x[np.arange(channels) % rows]
+        xe = np.concatenate([xr, np.zeros((channels, max(T_out - T_in, 0)))], axis=1)
+        ref, _ = oracle_mod.fir_rows(hh, xe, shift=shift)
+        ref = ref[:, :T_out]
+        for dt, tol in ((torch.float64, 1e-10), (torch.float32, F32_TOL)):
+            st, y = _fft_call(torch.tensor(h, dtype=dt, device=dev),
+                              torch.tensor(x, dtype=dt, device=dev), channels=channels,
+                              shift=shift, n_out=T_out, reverse=rev)
+            assert st == _lib.OK
+            got = y.cpu().numpy()
+            assert np.isfinite(got).all()
+            assert relerr(got, ref) < tol, (dt, relerr(got, ref))
+
+
+def test_fir_fft_matches_direct_kernel_full_shape():
+    """The FFT and direct kernels agree on 512 x 65536 FP32 rows with a
+    401-tap lowpass (the same sums, different rounding)."""
+    from torchscience_amd import _lib
+    from torchscience_amd.signal_processing.filter._fir import fir_rows_forward
+    import scipy.signal as ss
+    dev = _dev()
+    h = torch.tensor(ss.firwin(401, 0.2), dtype=torch.float32, device=dev)
+    x = torch.randn(512, 65536, dtype=torch.float32, device=dev)
+    st, y_fft = _fft_call(h, x, channels=512, shift=0, n_out=65536)
+    assert st == _lib.OK
+    from torchscience_amd.signal_processing.filter import _fir
+    old = _fir.FFT_MIN_TAPS
+    try:
+        _fir.FFT_MIN_TAPS = 1 << 30
+        y_dir, _ = fir_rows_forward(h.reshape(1, -1), x, None, channels=512,
+                                    chans_per_filter=512)
+    finally:
+        _fir.FFT_MIN_TAPS = old
+    scale = y_dir.abs().max().item()
+    assert (y_fft - y_dir).abs().max().item() < F32_TOL * scale
+
+
+def test_fir_fft_workspace_too_small():
+    from torchscience_amd import _lib
+    dev = _dev()
+    h = torch.randn(100, dtype=torch.float64, device=dev)
+    x = torch.randn(2, 500, dtype=torch.float64, device=dev)
+    st, _ = _fft_call(h, x, channels=2, shift=0, n_out=500, ws_bytes=100)
+    assert st == _lib.EWORKSPACE
+
+
+@pytest.mark.parametrize("K", [192, 1500, 2048])
+def test_fftfilt_long_filters_match_scipy(K):
+    """fftfilt routes these through the FFT kernel: against
+    scipy.signal.fftconvolve(x, b, mode='same') (the reference's own oracle,
+    test__fftfilt.py), FP64 at 1e-10 and FP32 at 1e-5; gradient of x at
+    FP64 against the adjoint (correlation with the same taps)."""
+    F, dev = _F(), _dev()
+    import scipy.signal as ss
+    rng = np.random.default_rng(K)
+    b = rng.standard_normal(K)
+    x = rng.standard_normal((3, 5000))
+    ref = np.stack([ss.fftconvolve(r, b, mode="same") for r in x])
+    for dt, tol in ((torch.float64, 1e-10), (torch.float32, F32_TOL)):
+        y = F.fftfilt(torch.tensor(b, dtype=dt, device=dev), torch.tensor(x, dtype=dt, device=dev))
+        assert y.dtype == dt
+        assert relerr(y.cpu().numpy(), ref) < tol
+    xt = torch.tensor(x, device=dev, requires_grad=True)
+    g = rng.standard_normal(x.shape)
+    F.fftfilt(torch.tensor(b, device=dev), xt).backward(torch.tensor(g, device=dev))
+    # y = C x with C the 'same' convolution; dx = C^T g, C^T is correlation
+    full = np.stack([np.correlate(r, b, mode="full") for r in g])
+    s = K - 1 - (K - 1) // 2
+    gx_ref = full[:, s:s + x.shape[1]]
+    assert relerr(xt.grad.cpu().numpy(), gx_ref) < 1e-10

@@ -70,6 +70,34 @@ def test_workspace_size_query_without_gpu():
     assert L.tsiir_status_string(2) == b"unsupported request"
 
 
+def test_fir_fft_host_checks_without_gpu():
+    """The FFT FIR entry point's size query and the argument checks that
+    return before any device work."""
+    L = _lib.load()
+
+    def desc(**kw):
+        d = _lib.FirDesc(channels=7, n_samples=1000, x_len=1000, n_taps=101, io_dtype=_lib.F32,
+                         shift=0, flags=0, x_rows=7, x_row_stride=1000, y_row_stride=1000,
+                         chans_per_filter=7)
+        for k, v in kw.items():
+            setattr(d, k, v)
+        return d
+    # twiddles (1024) + the taps' transform (4096), complex in the I/O type
+    assert L.tsiir_fir_fft_workspace_bytes(desc()) == 5120 * 8
+    assert L.tsiir_fir_fft_workspace_bytes(desc(io_dtype=_lib.F64)) == 5120 * 16
+    assert L.tsiir_fir_fft_workspace_bytes(desc(n_taps=2049)) == 0
+    assert L.tsiir_fir_fft_workspace_bytes(desc(chans_per_filter=1)) == 0
+    for d, code in ((desc(flags=_lib.HAS_ZI), _lib.EINVAL),
+                    (desc(flags=_lib.WRITE_ZF), _lib.EINVAL),
+                    (desc(chans_per_filter=3), _lib.EINVAL),
+                    (desc(n_taps=2049), _lib.EUNSUPPORTED),
+                    (desc(n_taps=0), _lib.EUNSUPPORTED),
+                    (desc(io_dtype=_lib.BF16), _lib.EUNSUPPORTED)):
+        assert L.tsiir_fir_forward_fft(d, None, None, None, None, 0, None) == code
+    assert L.tsiir_fir_forward_fft(desc(channels=0, x_rows=1), None, None, None, None, 0,
+                                   None) == _lib.OK
+
+
 def test_sos_shape_error_matches_reference():
     with pytest.raises(ValueError, match="shape"):
         sosfilt(torch.ones(1, 3), torch.randn(10))

@@ -39,6 +39,8 @@ EXPORTED = (
     "tsiir_sosfilt_backward_saved_workspace_bytes",
     "tsiir_sosfilt_backward_saved",
     "tsiir_fir_forward",
+    "tsiir_fir_fft_workspace_bytes",
+    "tsiir_fir_forward_fft",
     "tsiir_fir_backward_taps_workspace_bytes",
     "tsiir_fir_backward_taps",
     "tsiir_sos_frequency_response",
@@ -160,6 +162,10 @@ def load(path: str | None = None) -> ctypes.CDLL:
     L.tsiir_fir_backward_taps_workspace_bytes.argtypes = [fp]
     L.tsiir_fir_backward_taps.restype = ctypes.c_int
     L.tsiir_fir_backward_taps.argtypes = [fp, vp, vp, vp, vp, sz, vp]
+    L.tsiir_fir_fft_workspace_bytes.restype = sz
+    L.tsiir_fir_fft_workspace_bytes.argtypes = [fp]
+    L.tsiir_fir_forward_fft.restype = ctypes.c_int
+    L.tsiir_fir_forward_fft.argtypes = [fp, vp, vp, vp, vp, sz, vp]
     L.tsiir_sos_frequency_response.restype = ctypes.c_int
     L.tsiir_sos_frequency_response.argtypes = [vp, ctypes.c_int32, ctypes.c_int64,
                                                ctypes.c_int32, vp, ctypes.c_int64, vp, vp]
