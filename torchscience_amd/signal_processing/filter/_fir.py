@@ -31,9 +31,9 @@ from ... import _lib
 from ._ops import _require_device
 
 # Tap counts routed to the FFT kernel; below FFT_MIN_TAPS the direct kernel
-# is faster (measured on MI355X over 4096 x 65536 rows: direct faster at 101
-# taps, the FFT kernel at 256, FP32 and FP64 alike).
-FFT_MIN_TAPS = 192
+# is faster (measured on MI355X over 4096 x 65536 rows: direct faster at 48
+# taps, the FFT kernel at 64, FP32 and FP64 alike).
+FFT_MIN_TAPS = 64
 FFT_MAX_TAPS = 2048
 
 
