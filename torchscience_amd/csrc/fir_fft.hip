@@ -178,7 +178,9 @@ __global__ __launch_bounds__(256) void fft_twiddle_kernel(typename CxOf<T>::type
 }
 
 // Transform of the taps (zero-padded to N, reversed with REV, scaled by
-// 1 / N), left in the forward passes' permuted order.  One workgroup.
+// 1 / N), left in the forward passes' permuted order and stored lane-major:
+// H[k * 256 + t] is point k = m1 * 4 + m2 of lane t in the last forward
+// pass, so the filtering kernel's loads of it are coalesced.  One workgroup.
 template <typename T>
 __global__ __launch_bounds__(256) void fft_taps_kernel(const T* __restrict__ taps, int K,
                                                        int rev,
@@ -208,7 +210,7 @@ __global__ __launch_bounds__(256) void fft_taps_kernel(const T* __restrict__ tap
 #pragma unroll
   for (int m1 = 0; m1 < 4; ++m1)
 #pragma unroll
-    for (int m2 = 0; m2 < 4; ++m2) H[point<4>(t, m1, m2)] = v[m1][m2];
+    for (int m2 = 0; m2 < 4; ++m2) H[(m1 * 4 + m2) * kFftThreads + t] = v[m1][m2];
 }
 
 template <typename T>
@@ -265,7 +267,8 @@ __global__ __launch_bounds__(256) void fir_fft_kernel(FftArgs<T> a) {
 #pragma unroll
   for (int m1 = 0; m1 < 4; ++m1)
 #pragma unroll
-    for (int m2 = 0; m2 < 4; ++m2) v[m1][m2] = cmul(v[m1][m2], a.H[point<4>(t, m1, m2)]);
+    for (int m2 = 0; m2 < 4; ++m2)
+      v[m1][m2] = cmul(v[m1][m2], a.H[(m1 * 4 + m2) * kFftThreads + t]);
   inv16<4>(v, a.tw, t);
   lds_store<4>(lds, t, v);
   __syncthreads();

@@ -31,9 +31,10 @@ from ... import _lib
 from ._ops import _require_device
 
 # Tap counts routed to the FFT kernel; below FFT_MIN_TAPS the direct kernel
-# is faster (measured on MI355X over 4096 x 65536 rows: direct faster at 48
-# taps, the FFT kernel at 64, FP32 and FP64 alike).
-FFT_MIN_TAPS = 64
+# is faster (measured on MI355X over 4096 x 65536 rows: the FFT kernel
+# faster from 48 taps in FP32 and FP64, the direct kernel at 16; at 32 they
+# are within 5%).
+FFT_MIN_TAPS = 48
 FFT_MAX_TAPS = 2048
 
 
