@@ -214,14 +214,16 @@ typedef struct tsiir_fir_desc {
 tsiir_status tsiir_fir_forward(const tsiir_fir_desc* desc, const void* taps, const void* x,
                                void* y, const void* zi, void* zf, void* stream);
 
-/* FFT (overlap-save) form of tsiir_fir_forward for one filter shared by all
- * channels (chans_per_filter >= channels), no state flags (HAS_ZI / WRITE_ZF
- * give TSIIR_EINVAL), 1 <= K <= 2048 (else TSIIR_EUNSUPPORTED); honours
- * shift, x_rows, the strides and TSIIR_FIR_REVERSE_TAPS.  Same outputs up to
+/* FFT (overlap-save) form of tsiir_fir_forward: taps (n_filters, K), one
+ * filter per chans_per_filter consecutive channels (channels must be a
+ * multiple of chans_per_filter unless chans_per_filter >= channels, one
+ * filter), no state flags (HAS_ZI / WRITE_ZF and a ragged last filter give
+ * TSIIR_EINVAL), 1 <= K <= 2048 (else TSIIR_EUNSUPPORTED); honours shift,
+ * x_rows, the strides and TSIIR_FIR_REVERSE_TAPS.  Same outputs up to
  * rounding; its cost per output does not grow with K.  Replaces the
  * rfft / multiply / irfft of fftfilt _fftfilt.py:76-91.  The workspace (size
  * from tsiir_fir_fft_workspace_bytes, 0 for an invalid descriptor) holds the
- * twiddles and the taps' transform; it is rewritten on every call, so calls
+ * twiddles and one transform per filter (32 KB FP32, 64 KB FP64 each); it is rewritten on every call, so calls
  * that share one workspace must be ordered on one stream. */
 size_t tsiir_fir_fft_workspace_bytes(const tsiir_fir_desc* desc);
 tsiir_status tsiir_fir_forward_fft(const tsiir_fir_desc* desc, const void* taps, const void* x,

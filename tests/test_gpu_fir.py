@@ -421,3 +421,42 @@ def test_fftfilt_long_filters_match_scipy(K):
     s = K - 1 - (K - 1) // 2
     gx_ref = full[:, s:s + x.shape[1]]
     assert relerr(xt.grad.cpu().numpy(), gx_ref) < 1e-10
+
+
+@pytest.mark.parametrize("cpf", [1, 2, 3])
+def test_fir_fft_per_filter_matches_oracle(oracle_mod, cpf):
+    """One filter per group of cpf channels (odd groups leave a channel
+    alone in its complex pair), through the C-ABI, FP64 at 1e-10 and FP32 at
+    1e-5; and batched_filter_apply's per-signal "fir" format at 700 taps
+    (routed to the FFT kernel) against the oracle."""
+    from torchscience_amd import _lib
+    from torchscience_amd.signal_processing.filter._fir import _fir_desc
+    dev = _dev()
+    rng = np.random.default_rng(cpf)
+    nf, K, T = 3, 300, 9000
+    channels = nf * cpf
+    h = rng.standard_normal((nf, K))
+    x = rng.standard_normal((channels, T))
+    ref, _ = oracle_mod.fir_rows(np.repeat(h, cpf, axis=0), x, shift=5)
+    for dt, tol in ((torch.float64, 1e-10), (torch.float32, F32_TOL)):
+        print("case fir fft per filter", cpf, dt, flush=True)
+        ht = torch.tensor(h, dtype=dt, device=dev)
+        xt = torch.tensor(x, dtype=dt, device=dev)
+        y = torch.full((channels, T), float("nan"), dtype=dt, device=dev)
+        d = _fir_desc(channels=channels, n_samples=T, x_len=T, n_taps=K, dtype=dt, shift=5,
+                      flags=0, x_rows=channels, x_row_stride=T, y_row_stride=T,
+                      chans_per_filter=cpf)
+        L = _lib.load()
+        nbytes = L.tsiir_fir_fft_workspace_bytes(d)
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        assert L.tsiir_fir_forward_fft(d, _lib.ptr(ht), _lib.ptr(xt), _lib.ptr(y),
+                                       _lib.ptr(ws), nbytes, _lib.stream_handle(dev)) == _lib.OK
+        got = y.cpu().numpy()
+        assert np.isfinite(got).all() and relerr(got, ref) < tol
+    F = _F()
+    b = rng.standard_normal((4, 700))
+    xs = rng.standard_normal((4, 5000))
+    yb = F.batched_filter_apply(torch.tensor(b, device=dev), torch.tensor(xs, device=dev),
+                                filter_format="fir", broadcast=False)
+    refb, _ = oracle_mod.fir_rows(b, xs)     # lfilter(b[i], [1], x[i]) per row
+    assert yb.shape == refb.shape and relerr(yb.cpu().numpy(), refb) < 1e-10

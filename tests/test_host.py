@@ -82,11 +82,14 @@ def test_fir_fft_host_checks_without_gpu():
         for k, v in kw.items():
             setattr(d, k, v)
         return d
-    # twiddles (1024) + the taps' transform (4096), complex in the I/O type
+    # twiddles (1024) + one taps' transform (4096) per filter, complex in the
+    # I/O type
     assert L.tsiir_fir_fft_workspace_bytes(desc()) == 5120 * 8
     assert L.tsiir_fir_fft_workspace_bytes(desc(io_dtype=_lib.F64)) == 5120 * 16
+    assert L.tsiir_fir_fft_workspace_bytes(desc(chans_per_filter=1)) == (1024 + 7 * 4096) * 8
+    assert L.tsiir_fir_fft_workspace_bytes(desc(chans_per_filter=9)) == 5120 * 8
     assert L.tsiir_fir_fft_workspace_bytes(desc(n_taps=2049)) == 0
-    assert L.tsiir_fir_fft_workspace_bytes(desc(chans_per_filter=1)) == 0
+    assert L.tsiir_fir_fft_workspace_bytes(desc(chans_per_filter=2)) == 0
     for d, code in ((desc(flags=_lib.HAS_ZI), _lib.EINVAL),
                     (desc(flags=_lib.WRITE_ZF), _lib.EINVAL),
                     (desc(chans_per_filter=3), _lib.EINVAL),

@@ -1,5 +1,5 @@
-// fir_fft.hip - FIR filtering by FFT (overlap-save) for long filters shared by
-// every channel.
+// fir_fft.hip - FIR filtering by FFT (overlap-save) for long filters, one filter
+// per group of chans_per_filter consecutive channels.
 //
 // Computes the same y[n] = sum_{j<K} h[j] x[n + shift - j] (x zero outside
 // [0, x_len)) as the direct kernel, the way fftfilt (_fftfilt.py:60-92)
@@ -8,9 +8,10 @@
 // per output no longer grows with K.
 //
 // Work decomposition: one workgroup of 256 lanes filters one block of
-// L = 4096 - K + 1 consecutive outputs of TWO channels at once: channel 2p
-// is the real part and channel 2p + 1 the imaginary part of one complex
-// input window.  The taps are real, so the circular convolution keeps the
+// L = 4096 - K + 1 consecutive outputs of TWO channels of one filter at
+// once: the first is the real part and the second the imaginary part of one
+// complex input window (a filter with an odd channel count leaves its last
+// channel alone, with a zero imaginary part).  The taps are real, so the circular convolution keeps the
 // two parts apart and no post-processing of a real transform is needed.
 //
 // The transform: six radix-4 stages, decimation in frequency (natural order
@@ -180,7 +181,8 @@ __global__ __launch_bounds__(256) void fft_twiddle_kernel(typename CxOf<T>::type
 // Transform of the taps (zero-padded to N, reversed with REV, scaled by
 // 1 / N), left in the forward passes' permuted order and stored lane-major:
 // H[k * 256 + t] is point k = m1 * 4 + m2 of lane t in the last forward
-// pass, so the filtering kernel's loads of it are coalesced.  One workgroup.
+// pass, so the filtering kernel's loads of it are coalesced.  One workgroup
+// per filter (blockIdx.x).
 template <typename T>
 __global__ __launch_bounds__(256) void fft_taps_kernel(const T* __restrict__ taps, int K,
                                                        int rev,
@@ -189,6 +191,8 @@ __global__ __launch_bounds__(256) void fft_taps_kernel(const T* __restrict__ tap
   using C = typename CxOf<T>::type;
   __shared__ C lds[kFftLds];
   const int t = threadIdx.x;
+  taps += static_cast<int64_t>(blockIdx.x) * K;
+  H += static_cast<int64_t>(blockIdx.x) * kFftN;
   const T scale = static_cast<T>(1.0 / kFftN);
   for (int r = 0; r < kFftN / kFftThreads; ++r) {
     const int i = t + r * kFftThreads;
@@ -227,6 +231,8 @@ struct FftArgs {
   int64_t y_stride;
   int64_t shift;
   int64_t blocks;  // output blocks per channel pair
+  int64_t cpf;     // channels per filter (<= channels)
+  int64_t ppf;     // channel pairs per filter
   int K;
   int L;           // outputs per block, N - K + 1
 };
@@ -238,8 +244,11 @@ __global__ __launch_bounds__(256) void fir_fft_kernel(FftArgs<T> a) {
   const int t = threadIdx.x;
   const int64_t pair = blockIdx.x / a.blocks;
   const int64_t blk = blockIdx.x - pair * a.blocks;
-  const int64_t c0 = 2 * pair;
-  const bool has1 = c0 + 1 < a.channels;
+  const int64_t f = pair / a.ppf;
+  const int64_t c_in = 2 * (pair - f * a.ppf);
+  const int64_t c0 = f * a.cpf + c_in;
+  const bool has1 = c_in + 1 < a.cpf;
+  const C* __restrict__ H = a.H + f * kFftN;
   const T* __restrict__ x0 = a.x + (c0 % a.x_rows) * a.x_stride;
   const T* __restrict__ x1 = has1 ? a.x + ((c0 + 1) % a.x_rows) * a.x_stride : x0;
   const int64_t n0 = blk * a.L;
@@ -268,7 +277,7 @@ __global__ __launch_bounds__(256) void fir_fft_kernel(FftArgs<T> a) {
   for (int m1 = 0; m1 < 4; ++m1)
 #pragma unroll
     for (int m2 = 0; m2 < 4; ++m2)
-      v[m1][m2] = cmul(v[m1][m2], a.H[(m1 * 4 + m2) * kFftThreads + t]);
+      v[m1][m2] = cmul(v[m1][m2], H[(m1 * 4 + m2) * kFftThreads + t]);
   inv16<4>(v, a.tw, t);
   lds_store<4>(lds, t, v);
   __syncthreads();
@@ -300,14 +309,20 @@ tsiir_status check_fft(const tsiir_fir_desc* d) {
   if (d->x_rows < 1 || d->chans_per_filter < 1) return TSIIR_EINVAL;
   if (d->x_row_stride < 0 || d->y_row_stride < 0) return TSIIR_EINVAL;
   if (d->flags & (TSIIR_HAS_ZI | TSIIR_WRITE_ZF)) return TSIIR_EINVAL;
-  if (d->chans_per_filter < d->channels) return TSIIR_EINVAL;
+  if (d->chans_per_filter < d->channels && d->channels % d->chans_per_filter != 0)
+    return TSIIR_EINVAL;
   if (d->n_taps < 1 || d->n_taps > kFftMaxTaps) return TSIIR_EUNSUPPORTED;
   return TSIIR_OK;
 }
 
+int64_t fft_filters(const tsiir_fir_desc* d) {
+  if (d->channels == 0) return 1;
+  return d->chans_per_filter >= d->channels ? 1 : d->channels / d->chans_per_filter;
+}
+
 size_t fft_ws_bytes(const tsiir_fir_desc* d) {
   const size_t c = d->io_dtype == TSIIR_F64 ? sizeof(double2) : sizeof(float2);
-  return (kFftTw + kFftN) * c;
+  return (kFftTw + static_cast<size_t>(fft_filters(d)) * kFftN) * c;
 }
 
 template <typename T>
@@ -331,12 +346,16 @@ tsiir_status launch_fft(const tsiir_fir_desc* d, const void* taps, const void* x
   a.K = d->n_taps;
   a.L = kFftN - d->n_taps + 1;
   a.blocks = (d->n_samples + a.L - 1) / a.L;
-  const int64_t grid = (d->channels + 1) / 2 * a.blocks;
-  if (grid > 0x7fffffff) return TSIIR_EUNSUPPORTED;
+  const int64_t nf = fft_filters(d);
+  a.cpf = d->channels / nf;
+  a.ppf = (a.cpf + 1) / 2;
+  const int64_t grid = nf * a.ppf * a.blocks;
+  if (grid > 0x7fffffff || nf > 0x7fffffff) return TSIIR_EUNSUPPORTED;
   hipLaunchKernelGGL(fft_twiddle_kernel<T>, dim3((kFftTw + 255) / 256), dim3(256), 0, st, tw);
   if (hipGetLastError() != hipSuccess) return TSIIR_ELAUNCH;
   const int rev = (d->flags & TSIIR_FIR_REVERSE_TAPS) ? 1 : 0;
-  hipLaunchKernelGGL(fft_taps_kernel<T>, dim3(1), dim3(kFftThreads), 0, st,
+  hipLaunchKernelGGL(fft_taps_kernel<T>, dim3(static_cast<unsigned>(nf)), dim3(kFftThreads), 0,
+                     st,
                      static_cast<const T*>(taps), d->n_taps, rev, tw, H);
   if (hipGetLastError() != hipSuccess) return TSIIR_ELAUNCH;
   hipLaunchKernelGGL(fir_fft_kernel<T>, dim3(static_cast<unsigned>(grid)), dim3(kFftThreads), 0,

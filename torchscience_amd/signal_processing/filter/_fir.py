@@ -10,8 +10,9 @@ filtering path:
 * fftfilt(b, x) (_fftfilt.py:9-99): the 'same' slice of the full convolution
   (offset (K - 1) // 2).
 
-One filter shared by every channel with FFT_MIN_TAPS..FFT_MAX_TAPS taps and
-no state goes through the overlap-save FFT kernel instead
+Filters of FFT_MIN_TAPS..FFT_MAX_TAPS taps without state (one shared filter,
+or one per equal group of channels) go through the overlap-save FFT kernel
+instead
 (tsiir_fir_forward_fft), whose cost per output does not grow with K; this
 covers the forward and the input gradient (the same filter, reversed).
 
@@ -83,8 +84,8 @@ def fir_rows_forward(taps: Tensor, x: Tensor, zi: Tensor | None, *, channels: in
                   flags=flags, x_rows=x.shape[0], x_row_stride=T_in, y_row_stride=T,
                   chans_per_filter=chans_per_filter)
     L = _lib.load()
-    if (zi is None and not want_zf and chans_per_filter >= channels
-            and FFT_MIN_TAPS <= K <= FFT_MAX_TAPS):
+    if (zi is None and not want_zf and FFT_MIN_TAPS <= K <= FFT_MAX_TAPS
+            and (chans_per_filter >= channels or channels % chans_per_filter == 0)):
         nbytes = L.tsiir_fir_fft_workspace_bytes(d)
         ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
         _lib.check(L.tsiir_fir_forward_fft(d, _lib.ptr(taps), _lib.ptr(x), _lib.ptr(y),
