@@ -45,6 +45,30 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_BYTES = {"c2a": 2156240896}
 
 
+# The first launches after the GPU leaves idle run at full speed, then the
+# power management slows them for a while (C2a: dispatches 5-25 at
+# 0.50-0.54 ms against 0.425 ms before and after; profiles/
+# sos_fwd32_c2a_warmup_trace.txt).  Warm-up therefore runs at least W steps
+# and at least WARMUP_MIN_S seconds of them, so the K timed steps see the
+# settled clocks; the JSON reports how many warm-up steps ran.
+WARMUP_MIN_S = 0.6
+
+
+def warm_up(step, n_min):
+    """Run step() at least n_min times and until WARMUP_MIN_S seconds of
+    GPU work have passed; returns (last result, steps run)."""
+    import torch
+    out, n = None, 0
+    t0 = time.perf_counter()
+    while n < n_min or time.perf_counter() - t0 < WARMUP_MIN_S:
+        out = step()
+        n += 1
+        if n % 8 == 0 or n >= n_min:
+            torch.cuda.synchronize()
+    torch.cuda.synchronize()
+    return out, n
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -206,10 +230,7 @@ def fir_bench(args, dev):
     def step():
         return fftfilt(h, x) if long_k else lfilter(h, one, x)
     shift = (K - 1) // 2 if long_k else 0
-    y = None
-    for _ in range(args.warmup):
-        y = step()
-    torch.cuda.synchronize()
+    y, n_warm = warm_up(step, args.warmup)
     stream = torch.cuda.current_stream(dev)
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
@@ -237,7 +258,7 @@ def fir_bench(args, dev):
         "metric": "FIR G sample·taps/s (1 GPU)",
         "value": round(B * T * K / (ms * 1e-3) / 1e9, 2),
         "unit": "G sample·taps/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "warmup_steps_run": n_warm,
         "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64" if dt == torch.float64 else "f32",
         "data": f"synthetic: x ~ N(0,1) from a seeded device generator; {K}-tap Hamming "
@@ -295,10 +316,7 @@ def iir8_bench(args, dev):
     b_np, a_np = butter(8, 0.2)
     b = torch.tensor(b_np, dtype=torch.float32, device=dev)
     a = torch.tensor(a_np, dtype=torch.float32, device=dev)
-    y = None
-    for _ in range(args.warmup):
-        y = lfilter(b, a, x)
-    torch.cuda.synchronize()
+    y, n_warm = warm_up(lambda: lfilter(b, a, x), args.warmup)
     stream = torch.cuda.current_stream(dev)
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
@@ -320,7 +338,8 @@ def iir8_bench(args, dev):
     return {
         "metric": "general-order lfilter G samples/s (1 GPU)",
         "value": round(B * T / (ms * 1e-3) / 1e9, 2), "unit": "G samples/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "warmup_steps_run": n_warm,
+        "ms_per_step": round(ms, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic: x ~ N(0,1) from a seeded device generator; order-8 Butterworth "
                 "lowpass 0.2 as (b, a)",
@@ -365,10 +384,7 @@ def design_bench(args, dev):
     rng = np.random.default_rng(2)
     cut_np = rng.uniform(0.02, 0.45, n)
     cut = torch.tensor(cut_np, device=dev)
-    sos = None
-    for _ in range(args.warmup):
-        sos = batched_butterworth_design(8, cut)
-    torch.cuda.synchronize()
+    sos, n_warm = warm_up(lambda: batched_butterworth_design(8, cut), args.warmup)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         sos = batched_butterworth_design(8, cut)
@@ -381,7 +397,8 @@ def design_bench(args, dev):
     out = {
         "metric": "batched IIR design filters/s (1 GPU)",
         "value": round(n / (ms * 1e-3), 1), "unit": "filters/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "steps": args.steps, "warmup": args.warmup, "warmup_steps_run": n_warm,
+        "ms_per_step": round(ms, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: 4096 cutoffs U[0.02, 0.45] (seeded)",
         "config": {"workload": "batched_butterworth_design(8, cutoffs[4096]) -> (4096, 4, 6)",
@@ -575,10 +592,7 @@ def main():
             return sosfilt(sos, x, precision=prec)
         algo_bytes = B * T * io_bytes * 2 + sos.numel() * sos.element_size()
 
-    step_out = None
-    for _ in range(args.warmup):
-        step_out = step()
-    torch.cuda.synchronize()
+    step_out, n_warm = warm_up(step, args.warmup)
     stream = torch.cuda.current_stream(dev)
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
@@ -623,6 +637,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "warmup_steps_run": n_warm,
         "ms_per_step": round(ms, 4),
         "higher_is_better": True,
         "scaling": "weak",

@@ -262,3 +262,20 @@ def test_lfiltic_host_math_and_errors():
         lfiltic(torch.ones(2), torch.ones(2), torch.ones(2, 2))
     with pytest.raises(ValueError, match="b must be a 1-D tensor"):
         lfiltic(torch.ones(2, 2), torch.ones(2), torch.ones(2))
+
+
+@pytest.mark.parametrize("zi_shape", [(2, 4), (5, 4), (4, 3), (3, 2, 4)])
+def test_lfilter_zi_rows_checked_before_launch(zi_shape):
+    """lfilter of order > 2 (FIR and general form) reads zi as one state row
+    per channel, or one row for all: any other row count raises before any
+    device work (the reference fails on it too, _lfilter.py:136-166)."""
+    from torchscience_amd.signal_processing.filter._plan import plan_lfilter_zi
+    x = torch.randn(4, 50)
+    zi = torch.zeros(zi_shape)
+    for a in (torch.ones(1), torch.tensor([1.0, -0.5, 0.1, 0.0, 0.01])):
+        with pytest.raises(RuntimeError, match="zi must have shape"):
+            lfilter(torch.ones(5), a, x, zi=zi)
+    assert plan_lfilter_zi((4, 4), (4,), 4) is False
+    assert plan_lfilter_zi((4,), (4,), 4) is True
+    assert plan_lfilter_zi((1, 4), (4,), 4) is True
+    assert plan_lfilter_zi((2, 2, 4), (2, 2), 4) is False

@@ -128,3 +128,42 @@ def test_budget_too_small_is_rejected():
     eng = ServingEngine(FakeTable(sos, chans), None, hbm_budget_bytes=16)
     with pytest.raises(ValueError, match="HBM budget"):
         eng.run(reqs)
+
+
+class CheckedTable(FakeTable):
+    """A fake table that knows its streams' channel counts and device, so
+    the engine validates requests against them."""
+
+    device = torch.device("cpu")
+
+    def channels(self, sid):
+        return self.ch[sid]
+
+
+@pytest.mark.parametrize("bad", ["channels", "dtype", "noncontig", "stream", "y_shape", "dims"])
+def test_engine_rejects_requests_it_cannot_serve(oracle_mod, bad):
+    """Requests whose shape, dtype, layout or stream do not match what the
+    kernel's state slot and offsets assume raise before any launch."""
+    from torchscience_amd.signal_processing.filter._design import butterworth_sos
+    sos = {0: butterworth_sos(4, 0.2)}
+    table = CheckedTable(sos, {0: 3})
+    x = torch.zeros(3, 100)
+    y = None
+    sid = 0
+    if bad == "channels":
+        x = torch.zeros(4, 100)
+    elif bad == "dtype":
+        x = torch.zeros(3, 100, dtype=torch.float64)
+    elif bad == "noncontig":
+        x = torch.zeros(100, 3).t()
+    elif bad == "stream":
+        sid = 7
+    elif bad == "y_shape":
+        y = torch.zeros(3, 99)
+    elif bad == "dims":
+        x = torch.zeros(3, 10, 10)
+    clock = iter(range(10 ** 6)).__next__
+    eng = ServingEngine(table, FakeExecutor(oracle_mod, lambda: 0.0, None),
+                        clock=lambda: float(clock()))
+    with pytest.raises(ValueError):
+        eng.run([Request(0, sid, x, 0.0, y=y)])

@@ -180,12 +180,22 @@ class HipExecutor:
         streams = self.lanes[lane]
         stream = streams[self._next[lane]]
         self._next[lane] = (self._next[lane] + 1) % len(streams)
+        # the inputs (and fresh outputs) were produced on the caller's stream
+        stream.wait_stream(torch.cuda.current_stream(self.device))
         with torch.cuda.stream(stream):
             dev_items = host.to(self.device, non_blocking=True)
             _lib.check(self._lib.tsiir_serve_batch(dev_items.data_ptr(), n, self.mode,
                                                    stream.cuda_stream), "tsiir_serve_batch")
             ev = torch.cuda.Event()
             ev.record(stream)
+        # keep the caching allocator from reusing these blocks before the
+        # side stream is done with them
+        for r in batch:
+            r.x.record_stream(stream)
+            r.y.record_stream(stream)
+            table.state(r.stream_id).record_stream(stream)
+            table.table(r.stream_id).record_stream(stream)
+        dev_items.record_stream(stream)
         return (ev, host, dev_items)
 
     def done(self, handle) -> bool:
@@ -238,9 +248,38 @@ class ServingEngine:
         self.clock = clock
         self.batches: list[list[int]] = []  # request ids per launched batch (for tests)
 
+    def validate(self, r: Request) -> None:
+        """Reject a request the kernel cannot serve safely: the state slot
+        and the x / y offsets it is given assume a registered stream, a
+        (channels, samples) float32 contiguous chunk on the stream table's
+        device whose channel count is the stream's."""
+        dev = getattr(self.table, "device", None)
+        if not hasattr(self.table, "channels"):
+            return
+        try:
+            ch = self.table.channels(r.stream_id)
+        except KeyError:
+            raise ValueError(f"request {r.rid}: stream {r.stream_id} is not registered") from None
+        for name, t in (("x", r.x), ("y", r.y)):
+            if t is None:
+                continue
+            if t.dim() != 2 or t.dtype != torch.float32 or not t.is_contiguous():
+                raise ValueError(f"request {r.rid}: {name} must be a contiguous float32 "
+                                 f"(channels, samples) tensor, got {tuple(t.shape)} {t.dtype}")
+            if dev is not None and t.device != torch.device(dev):
+                raise ValueError(f"request {r.rid}: {name} is on {t.device}, the stream "
+                                 f"table on {dev}")
+        if r.x.shape[0] != ch:
+            raise ValueError(f"request {r.rid}: {r.x.shape[0]} channels for stream "
+                             f"{r.stream_id} of {ch} channels")
+        if r.y is not None and r.y.shape != r.x.shape:
+            raise ValueError(f"request {r.rid}: y {tuple(r.y.shape)} does not match x "
+                             f"{tuple(r.x.shape)}")
+
     def run(self, requests: list[Request]) -> ServeStats:
         reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
         for r in reqs:
+            self.validate(r)
             if r.nbytes > self.budget:
                 raise ValueError(f"request {r.rid} needs {r.nbytes} B, above the HBM budget")
             if r.channels > self.max_channels:

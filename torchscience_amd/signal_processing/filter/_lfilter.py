@@ -29,7 +29,7 @@ from torch import Tensor
 from ... import _lib
 from ._fir import fir_rows
 from ._ops import _desc, _require_device
-from ._plan import plan_lfilter
+from ._plan import plan_lfilter, plan_lfilter_zi
 from ._sosfilt import sosfilt
 
 
@@ -88,8 +88,12 @@ def _state_out(zf, batch, order, out):
 
 
 def _lfilter_fir(b, a, x, axis, zi, order):
-    dev = _require_device(x)
     ax = axis % x.dim()
+    zb = False
+    if zi is not None:
+        bshape = x.shape[:ax] + x.shape[ax + 1:]
+        zb = plan_lfilter_zi(zi.shape, bshape, order)
+    dev = _require_device(x)
     out = torch.promote_types(torch.promote_types(b.dtype, a.dtype), x.dtype)
     if out.is_complex:
         return _complex_split(b, a, x, axis, zi, out)
@@ -111,8 +115,7 @@ def _lfilter_fir(b, a, x, axis, zi, order):
         zf = rows.new_empty((0, order))
     else:
         y, zf = fir_rows(taps, rows, zi_r, channels=B, chans_per_filter=B,
-                         want_zf=zi is not None,
-                         zi_broadcast=zi_r is not None and zi_r.shape[0] == 1 and B > 1)
+                         want_zf=zi is not None, zi_broadcast=zb)
     y = torch.movedim(y.reshape(batch + (T,)), -1, ax).to(out)
     if zi is None:
         return y
@@ -265,9 +268,12 @@ class LfilterRows(torch.autograd.Function):
 
 
 def _lfilter_general(b, a, x, axis, zi, order):
-    dev = _require_device(x)
     nd = x.dim()
     ax = axis % nd
+    zb = False
+    if zi is not None:
+        zb = plan_lfilter_zi(zi.shape, x.shape[:ax] + x.shape[ax + 1:], order)
+    dev = _require_device(x)
     out = torch.promote_types(torch.promote_types(b.dtype, a.dtype), x.dtype)
     if out.is_complex:
         return _complex_split(b, a, x, axis, zi, out)
@@ -288,7 +294,6 @@ def _lfilter_general(b, a, x, axis, zi, order):
     zi_r = None
     if zi is not None:
         zi_r = _state_rows(zi, order, dev, comp)
-    zb = zi_r is not None and zi_r.shape[0] == 1 and B > 1
     y, zf = LfilterRows.apply(rows, bn, an, zi_r, zi is not None, zb)
     y = torch.movedim(y.reshape(batch + (T,)), -1, ax).to(out)
     if zi is None:

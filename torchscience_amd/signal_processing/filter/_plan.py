@@ -117,3 +117,31 @@ def plan_lfilter(b_shape, a_shape):
     if len(a_shape) != 1:
         raise ValueError("a must be a 1-D tensor")
     return max(b_shape[0], a_shape[0]) - 1
+
+
+def plan_lfilter_zi(zi_shape, batch_shape, order: int) -> bool:
+    """Check an lfilter zi against the signal's batch shape; True when one
+    state row serves every channel.
+
+    The reference reads zi as ``zi.reshape(-1, order)`` rows, one per
+    flattened channel (_lfilter.py:136-141), so a zi whose row count is
+    neither the channel count nor 1 fails there; here it raises before any
+    device work (a kernel would read past or short of the state rows)."""
+    zi_shape = tuple(zi_shape)
+    B = 1
+    for d in batch_shape:
+        B *= d
+    numel = 1
+    for d in zi_shape:
+        numel *= d
+    if order < 1 or len(zi_shape) == 0 or zi_shape[-1] != order or numel % order:
+        raise RuntimeError(
+            f"zi must have shape {tuple(batch_shape) + (order,)} (or ({order},)), "
+            f"got {zi_shape}")
+    rows = numel // order
+    if rows != B and rows != 1:
+        raise RuntimeError(
+            f"zi must have shape {tuple(batch_shape) + (order,)} (or ({order},)), "
+            f"got {zi_shape}: {rows} state rows for {B} channels")
+    return rows == 1 and B > 1
+
