@@ -75,6 +75,12 @@ typedef enum tsiir_mode {
  * (for at least as many channels as the GPU holds waves).  At most one. */
 #define TSIIR_PATH_WINDOW 0x10u
 #define TSIIR_PATH_CHANNEL 0x20u
+/* FP32 recurrences (FP32 / BF16 I/O, plain mode) in the reference's DF2T
+ * update order with FP64 carries between windows and waves, in place of the
+ * default direct-form-I kernel whose FP32 lane scan loses accuracy on poles
+ * very close to the unit circle.  Callers set it for such filters (the
+ * Python layer does for any pole radius above 0.995). */
+#define TSIIR_FP32_DF2T 0x80u
 
 typedef struct tsiir_sosfilt_desc {
   int64_t channels;          /* output channels B                          */

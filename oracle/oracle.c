@@ -145,4 +145,59 @@ void oracle_sosfilt_f32io(const double* sos, int S, int64_t chans_per_filter,
   }
 }
 
+/* FP32 restatement of the reference's FP32 behaviour: with float32 sos and
+ * x the reference runs every operation of _lfilter.py:96-98 (the a0
+ * normalisation) and :153-166 (the DF2T update) in float32, one rounding per
+ * ATen op, sections in order (_sosfilt.py:117-128).  Same layout as
+ * oracle_sosfilt_f64; zi / zf (B, S, 2) float or NULL. */
+static void df2t_channel_f32(float b0, float b1, float b2, float a1, float a2,
+                             float* v, int64_t T, float* z) {
+  float z0 = z[0], z1 = z[1];
+  for (int64_t n = 0; n < T; ++n) {
+    const float xn = v[n];
+    const float t0 = b0 * xn;
+    const float yn = t0 + z0;
+    const float p1 = b1 * xn;
+    const float q1 = a1 * yn;
+    const float d1 = p1 - q1;
+    z0 = d1 + z1;
+    const float p2 = b2 * xn;
+    const float q2 = a2 * yn;
+    z1 = p2 - q2;
+    v[n] = yn;
+  }
+  z[0] = z0;
+  z[1] = z1;
+}
+
+void oracle_sosfilt_f32(const float* sos, int S, int64_t chans_per_filter,
+                        const float* x, float* y, int64_t B, int64_t T,
+                        const float* zi, float* zf, int n_threads) {
+#pragma omp parallel for schedule(static) num_threads(n_threads > 0 ? n_threads : 1)
+  for (int64_t c = 0; c < B; ++c) {
+    const float* f = sos + (c / chans_per_filter) * (int64_t)S * 6;
+    float* yc = y + c * T;
+    if (yc != x + c * T) memcpy(yc, x + c * T, sizeof(float) * (size_t)T);
+    for (int s = 0; s < S; ++s) {
+      float b0 = f[6 * s], b1 = f[6 * s + 1], b2 = f[6 * s + 2];
+      float a0 = f[6 * s + 3], a1 = f[6 * s + 4], a2 = f[6 * s + 5];
+      if (a0 != 1.0f) {
+        b0 = b0 / a0;
+        b1 = b1 / a0;
+        b2 = b2 / a0;
+        a1 = a1 / a0;
+        a2 = a2 / a0;
+      }
+      float z[2];
+      z[0] = zi ? zi[(c * S + s) * 2 + 0] : 0.0f;
+      z[1] = zi ? zi[(c * S + s) * 2 + 1] : 0.0f;
+      df2t_channel_f32(b0, b1, b2, a1, a2, yc, T, z);
+      if (zf) {
+        zf[(c * S + s) * 2 + 0] = z[0];
+        zf[(c * S + s) * 2 + 1] = z[1];
+      }
+    }
+  }
+}
+
 int oracle_abi_version(void) { return 1; }

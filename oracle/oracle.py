@@ -92,6 +92,9 @@ def lib():
         L.oracle_sosfilt_f32io.argtypes = [_f64p, ctypes.c_int, ctypes.c_int64,
                                            _f32p, _f32p, ctypes.c_int64, ctypes.c_int64,
                                            ctypes.c_int]
+        L.oracle_sosfilt_f32.argtypes = [_f32p, ctypes.c_int, ctypes.c_int64,
+                                         _f32p, _f32p, ctypes.c_int64, ctypes.c_int64,
+                                         _f32p, _f32p, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -143,6 +146,27 @@ def sosfilt_rows_f32io(sos: np.ndarray, x: np.ndarray, chans_per_filter: int | N
     lib().oracle_sosfilt_f32io(_p(sos), sos.shape[1], cpf, _p(x, _f32p), _p(y, _f32p),
                                B, T, n_threads)
     return y
+
+
+def sosfilt_rows_f32(sos: np.ndarray, x: np.ndarray, zi: np.ndarray | None = None,
+                     chans_per_filter: int | None = None, n_threads: int = 1):
+    """The reference's FP32 arithmetic: float32 sos and x, every operation of
+    the a0 normalisation and the DF2T update rounded to float32
+    (_lfilter.py:96-98, 153-166; _sosfilt.py:117-128).  sos (S, 6) or
+    (n_filters, S, 6); zi (B, S, 2) or None.  Returns float32 (y, zf)."""
+    sos = np.ascontiguousarray(sos, dtype=np.float32)
+    if sos.ndim == 2:
+        sos = sos[None]
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    B, T = x.shape
+    S = sos.shape[1]
+    cpf = B if chans_per_filter is None else max(chans_per_filter, 1)
+    y = np.empty_like(x)
+    zf = np.empty((B, S, 2), dtype=np.float32)
+    zi_c = None if zi is None else np.ascontiguousarray(zi, dtype=np.float32)
+    lib().oracle_sosfilt_f32(_p(sos, _f32p), S, cpf, _p(x, _f32p), _p(y, _f32p), B, T,
+                             _p(zi_c, _f32p), _p(zf, _f32p), n_threads)
+    return y, zf
 
 
 def lfilter_rows(b: np.ndarray, a: np.ndarray, x: np.ndarray,

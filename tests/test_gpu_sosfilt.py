@@ -138,11 +138,14 @@ def test_fp32_near_unit_circle(oracle_mod):
     y64b, _ = oracle_mod.sosfilt_rows(sos.astype(np.float32).astype(np.float64),
                                       x.astype(np.float64), n_threads=4)
     err = relerr(y32.cpu().numpy(), y64b)
-    print("fp32 cheby16 relerr", err)
-    # plain FP32 recurrence: rounding inside the loop is amplified by the
-    # noise gain (a few 1e-3 in DF2T order, a few 1e-2 in the FP32 kernel's
-    # direct form I); reported, not a parity claim
-    assert err < 5e-2
+    # the reference's own FP32 arithmetic on the same input (its DF2T loop in
+    # float32): rounding inside the loop is amplified by the noise gain
+    ref32, _ = oracle_mod.sosfilt_rows_f32(sos.astype(np.float32), x, n_threads=4)
+    err_ref = relerr(ref32, y64b)
+    print("fp32 cheby16 relerr", err, "reference fp32", err_ref)
+    # poles beyond radius 0.995 run the DF2T form with FP64 carries: no worse
+    # than the reference's FP32 loop
+    assert err <= err_ref
     # the compensated form consumes the float64 coefficients as given
     yc = F.sosfilt(torch.tensor(sos, device=dev), torch.tensor(x, device=dev), precision="fp32c")
     assert yc.dtype == torch.float32

@@ -22,6 +22,7 @@ MODE_PLAIN, MODE_COMPENSATED = 0, 1
 HAS_ZI, WRITE_ZF, ZI_BROADCAST, REVERSE = 0x1, 0x2, 0x4, 0x8
 PATH_WINDOW, PATH_CHANNEL = 0x10, 0x20
 FIR_REVERSE_TAPS = 0x40
+FP32_DF2T = 0x80
 
 # Every symbol include/tsiir.h declares.
 EXPORTED = (

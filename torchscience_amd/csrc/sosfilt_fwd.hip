@@ -55,8 +55,8 @@ static bool fwd32_enabled() {
 }
 static inline bool use_fwd32(const tsiir_sosfilt_desc* d) {
   return (d->io_dtype == TSIIR_F32 || d->io_dtype == TSIIR_BF16) &&
-         d->mode == TSIIR_MODE_PLAIN && d->n_sections <= kWave && fwd32_enabled() &&
-         fwd_lds_io();
+         d->mode == TSIIR_MODE_PLAIN && d->n_sections <= kWave &&
+         (d->flags & TSIIR_FP32_DF2T) == 0 && fwd32_enabled() && fwd_lds_io();
 }
 static inline FwdPolicy fwd_policy(const tsiir_sosfilt_desc* d) {
   if (d->channels >= 1024 && d->n_sections <= 64) return FwdPolicy{1, true};

@@ -11,6 +11,7 @@ from __future__ import annotations
 import torch
 
 from ... import _lib
+from ._plan import fp32_needs_df2t
 
 
 def _require_device(*tensors: torch.Tensor) -> torch.device:
@@ -79,6 +80,8 @@ def sosfilt_rows_forward(sos: torch.Tensor, x: torch.Tensor, zi: torch.Tensor | 
         flags |= {"window": _lib.PATH_WINDOW, "channel": _lib.PATH_CHANNEL}[path]
     if reverse:
         flags |= _lib.REVERSE
+    if io != torch.float64 and mode == _lib.MODE_PLAIN and fp32_needs_df2t(sos):
+        flags |= _lib.FP32_DF2T
     d = _desc(channels=channels, n_samples=T, n_sections=S, io_dtype=io, coef_dtype=sos.dtype,
               flags=flags, x_rows=max(x.shape[0], 1),
               x_row_stride=x.stride(0) if x.dim() == 2 else T, y_row_stride=T,

@@ -44,7 +44,10 @@ def sosfilt(
     storage but runs every recurrence update in float64 with float64
     coefficients (the compensated form for poles near the unit circle) and
     returns float32; "bf16io" reads and writes bfloat16
-    with float32 arithmetic and returns bfloat16.
+    with float32 arithmetic and returns bfloat16; "bf16c" reads and writes
+    bfloat16 with float64 recurrence updates (the compensated form with
+    bfloat16 I/O).  Float32 recurrences of filters with a pole beyond radius
+    0.995 run in the DF2T update order with float64 carries.
     """
     plan = plan_sosfilt(sos.shape, sos.dtype, x.shape, x.dtype, axis,
                         None if zi is None else zi.shape,
