@@ -49,7 +49,9 @@ typedef enum tsiir_status {
   TSIIR_EINVAL = 1,        /* bad descriptor, pointer or size */
   TSIIR_EUNSUPPORTED = 2,  /* valid request this build does not implement */
   TSIIR_ELAUNCH = 3,       /* a kernel launch failed */
-  TSIIR_EWORKSPACE = 4     /* workspace missing or too small */
+  TSIIR_EWORKSPACE = 4,    /* workspace missing or too small */
+  TSIIR_ESCRATCH = 5       /* needs more than the fixed scratch of
+                              tsiir_sosfilt_forward_noalloc */
 } tsiir_status;
 
 typedef enum tsiir_dtype {
@@ -124,6 +126,23 @@ tsiir_status tsiir_sosfilt_forward(const tsiir_sosfilt_desc* desc,
                                    const void* zi, void* zf,
                                    void* workspace, size_t workspace_bytes,
                                    void* stream);
+
+/* tsiir_sosfilt_forward without a workspace argument, for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops
+ * and harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * The workspace is a fixed device array of the library,
+ * TSIIR_NOALLOC_SCRATCH_BYTES per GPU, held from the first call on a GPU
+ * until the process ends; descriptors whose workspace
+ * (tsiir_sosfilt_workspace_bytes) exceeds it return TSIIR_ESCRATCH.  The
+ * call allocates and frees nothing.  The first call on a GPU zero-fills that
+ * GPU's array on `stream` (make it outside graph capture).  Every call on a
+ * GPU shares its array: calls that overlap in time on one GPU race, and the
+ * caller must order them (one stream, or events).  Same kernels and results
+ * as tsiir_sosfilt_forward. */
+#define TSIIR_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+tsiir_status tsiir_sosfilt_forward_noalloc(const tsiir_sosfilt_desc* desc,
+                                           const void* x, void* y, const void* sos,
+                                           const void* zi, void* zf, void* stream);
 
 /* y = lfilter(b, a, x, zi) for a filter of order N = desc->n_sections
  * (1..64), coefficients shared by every channel.  `coef` holds b[0..N] then

@@ -16,7 +16,8 @@ import torch
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "_lib", "libtsiir.so")
 
-OK, EINVAL, EUNSUPPORTED, ELAUNCH, EWORKSPACE = range(5)
+OK, EINVAL, EUNSUPPORTED, ELAUNCH, EWORKSPACE, ESCRATCH = range(6)
+NOALLOC_SCRATCH_BYTES = 64 << 20
 F32, F64, BF16 = 0, 1, 2
 MODE_PLAIN, MODE_COMPENSATED = 0, 1
 HAS_ZI, WRITE_ZF, ZI_BROADCAST, REVERSE = 0x1, 0x2, 0x4, 0x8
@@ -31,6 +32,7 @@ EXPORTED = (
     "tsiir_sosfilt_workspace_bytes",
     "tsiir_workspace_init",
     "tsiir_sosfilt_forward",
+    "tsiir_sosfilt_forward_noalloc",
     "tsiir_lfilter_forward",
     "tsiir_lfilter_workspace_bytes",
     "tsiir_lfilter_forward_ws",
@@ -140,6 +142,8 @@ def load(path: str | None = None) -> ctypes.CDLL:
     L.tsiir_workspace_init.argtypes = [vp, sz, vp]
     L.tsiir_sosfilt_forward.restype = ctypes.c_int
     L.tsiir_sosfilt_forward.argtypes = [dp, vp, vp, vp, vp, vp, vp, sz, vp]
+    L.tsiir_sosfilt_forward_noalloc.restype = ctypes.c_int
+    L.tsiir_sosfilt_forward_noalloc.argtypes = [dp, vp, vp, vp, vp, vp, vp]
     L.tsiir_lfilter_forward.restype = ctypes.c_int
     L.tsiir_lfilter_forward.argtypes = [dp, vp, vp, vp, vp, vp, vp]
     L.tsiir_lfilter_workspace_bytes.restype = sz

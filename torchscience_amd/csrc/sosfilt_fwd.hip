@@ -902,6 +902,8 @@ const char* tsiir_status_string(tsiir_status s) {
     case TSIIR_EUNSUPPORTED: return "unsupported request";
     case TSIIR_ELAUNCH: return "kernel launch failed";
     case TSIIR_EWORKSPACE: return "workspace missing or too small";
+    case TSIIR_ESCRATCH:
+      return "request needs more workspace than the fixed scratch of the no-allocation entry point";
   }
   return "unknown status";
 }
