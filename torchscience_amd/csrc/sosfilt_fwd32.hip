@@ -348,6 +348,11 @@ __device__ __forceinline__ void lookback_groups(const uint64_t* status, int64_t 
                                                 double zi1, double& sw0, double& sw1) {
   constexpr int kPbase = 6 + kLog2Nwv<NWV>;
   const int lane = lane_now();
+  if (gi >= 1 && span_power_is_zero(tb, kPbase)) {
+    lookback_predecessor(status + ((((gi - 1) * NWV + NWV - 1) * B + c) * S + s) * kStatusWords,
+                         hdr, flag, sw0, sw1);
+    return;
+  }
   double acc0 = 0.0, acc1 = 0.0;
   Mat2 P{1.0, 0.0, 0.0, 1.0};
   int64_t base = gi - 1;
@@ -427,10 +432,34 @@ __device__ __forceinline__ LanePow lane_pow(const float4* mq) {
 // publication of the group's inclusive state, and the lane's incoming state
 // (pw0, pw1) in the scaled units of the section, which becomes the pending
 // zero-input response.
+// Look-back form, wave 0: the predecessor group's aggregate words of a
+// section whose group-span power is exactly zero (then they hold the group's
+// incoming state, see lookback_predecessor), loaded before the section's
+// sweep so that the round trip overlaps it.
+struct GroupPrefetch {
+  PairWords words;
+  bool valid;
+};
+template <int NWV, bool LB>
+__device__ __forceinline__ GroupPrefetch prefetch_group_in(const FwdArgs& a,
+                                                           const Win32<NWV>& w, int s) {
+  GroupPrefetch pf;
+  pf.valid = false;
+  if constexpr (LB) {
+    const double* tb = w.tf + static_cast<int64_t>(s) * kTableStride;
+    if (w.wv == 0 && w.kg > 0 && span_power_is_zero(tb, 6 + kLog2Nwv<NWV>)) {
+      load_pair_words(a.status + (((w.kg - 1) * a.B + w.c) * a.S + s) * kStatusWords,
+                      pf.words);
+      pf.valid = true;
+    }
+  }
+  return pf;
+}
+
 template <int NWV, bool LB>
 __device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32<NWV>& w, int s,
                                               const LanePow& lp, float z0, float z1, float& pw0,
-                                              float& pw1) {
+                                              float& pw1, const GroupPrefetch& pf) {
   const int lane = lane_now();
   const int S = a.S;
   const uint32_t flag = w.flag;
@@ -485,7 +514,7 @@ __device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32<NWV>
       }
       sw0 = zi0;
       sw1 = zi1;
-      if (w.kg > 0)
+      if (w.kg > 0 && !(pf.valid && decode_pair(pf.words, flag, sw0, sw1)))
         lookback_groups<NWV>(a.status, a.B, S, a.hdr, w.kg / NWV, w.c, s, flag, tb, zi0, zi1,
                              sw0, sw1);
       const Mat2 Mg = load_mat(table_pow(tb, 6 + kLog2Nwv<NWV>));
@@ -585,14 +614,16 @@ __device__ __forceinline__ void run_cascade(const FwdArgs& a, const Win32<NWV>& 
   pw0 = 0.0f;
   pw1 = 0.0f;
   LanePow lp = lane_pow(w.ff->mq);
+  GroupPrefetch pf = prefetch_group_in<NWV, LB>(a, w, 0);
   df1_sweep<UNIT, false>(v, w.ff, table_g(w.tf), 0.0f, 0.0f, z0, z1);
-  section_carry<NWV, LB>(a, w, 0, lp, z0, z1, pw0, pw1);
+  section_carry<NWV, LB>(a, w, 0, lp, z0, z1, pw0, pw1, pf);
   if (SAVED && S > 1) keep_section<TIO, NWV>(a, w, 0, v, pw0, pw1);
   for (int s = 1; s < S; ++s) {
     lp = lane_pow(w.ff[s].mq);
+    pf = prefetch_group_in<NWV, LB>(a, w, s);
     const float* pg = table_g(w.tf + static_cast<int64_t>(s - 1) * kTableStride);
     df1_sweep<UNIT, true>(v, w.ff + s, pg, pw0, pw1, z0, z1);
-    section_carry<NWV, LB>(a, w, s, lp, z0, z1, pw0, pw1);
+    section_carry<NWV, LB>(a, w, s, lp, z0, z1, pw0, pw1, pf);
     if (SAVED && s + 1 < S) keep_section<TIO, NWV>(a, w, s, v, pw0, pw1);
   }
 }
