@@ -83,6 +83,11 @@ typedef enum tsiir_mode {
  * very close to the unit circle.  Callers set it for such filters (the
  * Python layer does for any pole radius above 0.995). */
 #define TSIIR_FP32_DF2T 0x80u
+/* Testing only: every look-back wait of the forward gives up at once, as if
+ * its spin limit had passed (see tsiir_workspace_set_error_mailbox): the
+ * windows after the first of each channel come out NaN and the error is
+ * reported, with every kernel exiting normally. */
+#define TSIIR_DEBUG_GIVE_UP 0x100u
 
 typedef struct tsiir_sosfilt_desc {
   int64_t channels;          /* output channels B                          */
@@ -114,6 +119,19 @@ size_t tsiir_sosfilt_workspace_bytes(const tsiir_sosfilt_desc* desc);
  * per allocation; later calls keep it consistent themselves. */
 tsiir_status tsiir_workspace_init(void* workspace, size_t workspace_bytes,
                                   void* stream);
+
+/* Error reporting of a workspace.  A look-back wait (a window waiting for
+ * the state published by the window before it) that exceeds its spin limit
+ * gives up, so that every kernel terminates: it sets a bit in the workspace
+ * header's error word, writes the bit to the registered mailbox, and makes
+ * the affected windows' outputs (and those that depend on them) NaN.  The
+ * mailbox is a 32-bit word the kernels can write: pinned host memory
+ * (hipHostMalloc; pass its host address) or device memory; NULL unregisters
+ * it.  A caller polls the mailbox without synchronising.  The header is
+ * zeroed by tsiir_workspace_init, so register after it (stream-ordered).
+ * No counterpart in the reference (its Python loop cannot time out). */
+tsiir_status tsiir_workspace_set_error_mailbox(void* workspace, size_t workspace_bytes,
+                                               uint32_t* mailbox, void* stream);
 
 /* y = sosfilt(sos, x, zi) along time; zf = final state (when requested).
  * Replaces the section loop _sosfilt.py:117-128 and the per-sample DF2T loop

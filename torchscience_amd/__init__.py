@@ -7,3 +7,4 @@ thin Python layer that keeps the reference's operator signatures.
 __version__ = "0.1.0"
 
 from . import signal_processing  # noqa: F401,E402
+from ._lib import TsiirError, check_errors  # noqa: F401,E402

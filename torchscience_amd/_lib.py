@@ -24,6 +24,7 @@ HAS_ZI, WRITE_ZF, ZI_BROADCAST, REVERSE = 0x1, 0x2, 0x4, 0x8
 PATH_WINDOW, PATH_CHANNEL = 0x10, 0x20
 FIR_REVERSE_TAPS = 0x40
 FP32_DF2T = 0x80
+DEBUG_GIVE_UP = 0x100
 
 # Every symbol include/tsiir.h declares.
 EXPORTED = (
@@ -31,6 +32,7 @@ EXPORTED = (
     "tsiir_status_string",
     "tsiir_sosfilt_workspace_bytes",
     "tsiir_workspace_init",
+    "tsiir_workspace_set_error_mailbox",
     "tsiir_sosfilt_forward",
     "tsiir_sosfilt_forward_noalloc",
     "tsiir_lfilter_forward",
@@ -140,6 +142,8 @@ def load(path: str | None = None) -> ctypes.CDLL:
     L.tsiir_sosfilt_workspace_bytes.argtypes = [dp]
     L.tsiir_workspace_init.restype = ctypes.c_int
     L.tsiir_workspace_init.argtypes = [vp, sz, vp]
+    L.tsiir_workspace_set_error_mailbox.restype = ctypes.c_int
+    L.tsiir_workspace_set_error_mailbox.argtypes = [vp, sz, vp, vp]
     L.tsiir_sosfilt_forward.restype = ctypes.c_int
     L.tsiir_sosfilt_forward.argtypes = [dp, vp, vp, vp, vp, vp, vp, sz, vp]
     L.tsiir_sosfilt_forward_noalloc.restype = ctypes.c_int
@@ -220,40 +224,79 @@ def io_code(dtype: torch.dtype) -> int:
 
 
 class _Workspaces:
-    """One zero-initialised workspace per (device, stream), grown on demand.
+    """One zero-initialised workspace per (device, stream, kind), grown on
+    demand, each with an error mailbox.
 
     The kernels keep the workspace header consistent across launches, so a
     buffer is initialised once, when it is allocated.  Keying by stream keeps
-    concurrent streams from sharing scan status words."""
+    concurrent streams from sharing scan status words.  The mailbox is a
+    pinned host word the kernels write when a look-back wait gives up (the
+    affected outputs are then NaN): every later call that takes the
+    workspace reads it without synchronising and raises TsiirError, and
+    `check_errors` reads them all after a synchronisation."""
 
     def __init__(self):
-        self._bufs: dict[tuple[int, int, str], torch.Tensor] = {}
+        self._bufs: dict[tuple[int, int, str], tuple[torch.Tensor, torch.Tensor]] = {}
+
+    @staticmethod
+    def _key(device, kind):
+        stream = stream_handle(device)
+        return (device.index if device.index is not None else torch.cuda.current_device(),
+                stream, kind), stream
 
     def get(self, device: torch.device, nbytes: int, kind: str = "fwd") -> torch.Tensor:
-        stream = stream_handle(device)
-        key = (device.index if device.index is not None else torch.cuda.current_device(),
-               stream, kind)
+        key, stream = self._key(device, kind)
         with _lock:
-            buf = self._bufs.get(key)
-            if buf is None or buf.numel() < nbytes:
+            entry = self._bufs.get(key)
+            if entry is not None and int(entry[1][0]) != 0:
+                code = int(entry[1][0])
+                del self._bufs[key]
+                raise TsiirError(
+                    f"a look-back wait of an earlier call on this stream gave up (error "
+                    f"{code}): the outputs of the affected windows are NaN")
+            if entry is None or entry[0].numel() < nbytes:
                 size = max(nbytes, 1 << 20)
                 size = (size + 255) // 256 * 256
                 buf = torch.empty(size, dtype=torch.uint8, device=device)
                 check(load().tsiir_workspace_init(buf.data_ptr(), size, stream),
                       "tsiir_workspace_init")
-                self._bufs[key] = buf
-            return buf
+                mailbox = torch.zeros(1, dtype=torch.int32, pin_memory=True)
+                check(load().tsiir_workspace_set_error_mailbox(buf.data_ptr(), size,
+                                                               mailbox.data_ptr(), stream),
+                      "tsiir_workspace_set_error_mailbox")
+                entry = (buf, mailbox)
+                self._bufs[key] = entry
+            return entry[0]
+
+    def check_errors(self, device: torch.device | None = None) -> None:
+        """Synchronise and raise TsiirError if a look-back wait of any call
+        gave up (on `device`, or on every device)."""
+        with _lock:
+            items = list(self._bufs.items())
+        for key, (buf, mailbox) in items:
+            if device is not None and buf.device != torch.device(device):
+                continue
+            torch.cuda.synchronize(buf.device)
+            if int(mailbox[0]) != 0:
+                with _lock:
+                    self._bufs.pop(key, None)
+                raise TsiirError(f"a look-back wait gave up on {buf.device} (error "
+                                 f"{int(mailbox[0])}): the outputs of the affected windows "
+                                 "are NaN")
 
     def header_error(self, device: torch.device, kind: str = "fwd") -> int:
         """Error word of the workspace header (non-zero after a look-back
-        wait hit its spin limit).  Synchronises the stream."""
-        stream = stream_handle(device)
-        key = (device.index if device.index is not None else torch.cuda.current_device(),
-               stream, kind)
-        buf = self._bufs.get(key)
-        if buf is None:
+        wait gave up).  Synchronises the stream."""
+        key, _ = self._key(device, kind)
+        entry = self._bufs.get(key)
+        if entry is None:
             return 0
-        return int(buf[12:16].view(torch.int32).item())
+        return int(entry[0][12:16].view(torch.int32).item())
 
 
 workspaces = _Workspaces()
+
+
+def check_errors(device: torch.device | None = None) -> None:
+    """Raise TsiirError if a look-back wait of an earlier call gave up."""
+    workspaces.check_errors(device)

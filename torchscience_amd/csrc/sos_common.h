@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "tsiir.h"
+
 namespace tsiir {
 
 constexpr int kWave = 64;        // CDNA wavefront
@@ -24,10 +26,28 @@ struct WsHeader {
   uint32_t reserved0;
   uint32_t reserved1;
   uint32_t epoch;
-  uint32_t error;  // set when a look-back wait exceeded kSpinLimit
-  uint32_t pad[60];
+  uint32_t error;      // set when a look-back wait gave up (see give_up)
+  uint32_t* mailbox;   // caller's device-visible word that also gets it, or null
+  uint32_t pad[58];
 };
 static_assert(sizeof(WsHeader) == 256, "header size");
+
+// A look-back wait that gives up (its spin limit passed, or the caller's
+// TSIIR_DEBUG_GIVE_UP): `bit` goes into the header's error word and the
+// caller's mailbox, and the incoming state becomes NaN, so every output that
+// depends on the missing state is NaN instead of silently wrong.
+__device__ __forceinline__ void give_up(WsHeader* hdr, uint32_t bit, double& sw0, double& sw1) {
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    atomicOr(&hdr->error, bit);
+    uint32_t* mb = hdr->mailbox;
+    if (mb != nullptr) __hip_atomic_store(mb, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+  sw0 = __builtin_nan("");
+  sw1 = __builtin_nan("");
+}
+__device__ __forceinline__ bool debug_give_up(uint32_t flags) {
+  return (flags & TSIIR_DEBUG_GIVE_UP) != 0;
+}
 
 struct Mat2 {
   double a, b, c, d;  // [[a, b], [c, d]]
@@ -258,12 +278,34 @@ __device__ __forceinline__ bool span_power_is_zero(const double* tb, int p) {
 // The aggregate is published before the look-back that leads to the
 // inclusive state, so its word alone is polled.
 __device__ __forceinline__ void lookback_predecessor(const uint64_t* e, WsHeader* hdr,
-                                                     uint32_t flag, double& sw0, double& sw1) {
+                                                     uint32_t flag, uint32_t flags,
+                                                     double& sw0, double& sw1) {
+  if (debug_give_up(flags)) {
+    give_up(hdr, 1u, sw0, sw1);
+    return;
+  }
   uint32_t spins = 0;
   while (!read_pair(e, flag, sw0, sw1)) {
     if (++spins > kSpinLimit) {
-      if ((threadIdx.x & (kWave - 1)) == 0) atomicOr(&hdr->error, 1u);
-      sw0 = sw1 = 0.0;
+      give_up(hdr, 1u, sw0, sw1);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+}
+
+// Waits for one inclusive pair (a chained predecessor); gives up as above.
+__device__ __forceinline__ void wait_inclusive(const uint64_t* incl, WsHeader* hdr,
+                                               uint32_t flag, uint32_t flags, double& sw0,
+                                               double& sw1) {
+  if (debug_give_up(flags)) {
+    give_up(hdr, 1u, sw0, sw1);
+    return;
+  }
+  uint32_t spins = 0;
+  while (!read_pair(incl, flag, sw0, sw1)) {
+    if (++spins > kSpinLimit) {
+      give_up(hdr, 1u, sw0, sw1);
       break;
     }
     __builtin_amdgcn_s_sleep(2);
@@ -282,11 +324,11 @@ __device__ __forceinline__ void lookback_predecessor(const uint64_t* e, WsHeader
 __device__ __forceinline__ void lookback(const uint64_t* status, int64_t B, int S, WsHeader* hdr,
                                          int64_t k, int64_t c, int s, uint32_t flag,
                                          const double* tb, int pbase, double zi0, double zi1,
-                                         double& sw0, double& sw1) {
+                                         double& sw0, double& sw1, uint32_t flags = 0) {
   const int lane = threadIdx.x & (kWave - 1);
-  if (k >= 1 && span_power_is_zero(tb, pbase)) {
-    lookback_predecessor(status + (((k - 1) * B + c) * S + s) * kStatusWords, hdr, flag, sw0,
-                         sw1);
+  if (k >= 1 && (debug_give_up(flags) || span_power_is_zero(tb, pbase))) {
+    lookback_predecessor(status + (((k - 1) * B + c) * S + s) * kStatusWords, hdr, flag, flags,
+                         sw0, sw1);
     return;
   }
   double acc0 = 0.0, acc1 = 0.0;
@@ -316,7 +358,7 @@ __device__ __forceinline__ void lookback(const uint64_t* status, int64_t B, int 
     const uint64_t need = (f >= 63) ? ~0ull : ((1ull << (f + 1)) - 1ull);
     if ((ready & need) != need) {
       if (++spins > kSpinLimit) {
-        if (lane == 0) atomicOr(&hdr->error, 1u);
+        give_up(hdr, 1u, acc0, acc1);
         break;
       }
       __builtin_amdgcn_s_sleep(2);

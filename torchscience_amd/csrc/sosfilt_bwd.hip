@@ -166,7 +166,7 @@ __global__ __launch_bounds__(64) void sos_adjoint_kernel(AdjArgs a, int64_t k_ta
                  lane);
   } else if (k > 0) {
     const uint64_t* st = a.status + ((k - 1) * a.B + c) * kStatusWords;
-    if (!read_pair(st + 4, flag, sw0, sw1) && lane == 0) atomicOr(&a.hdr->error, 2u);
+    if (!read_pair(st + 4, flag, sw0, sw1)) give_up(a.hdr, 2u, sw0, sw1);
   }
   double s0, s1;
   lane_incoming(tb, lane, sw0, sw1, x0, x1, s0, s1);
@@ -423,14 +423,7 @@ __global__ __launch_bounds__(64) void sos_adjoint_fused_kernel(AdjFArgs a, int64
       sw1 = readlane_d(pv1, s);
       if (!__builtin_amdgcn_readlane(static_cast<int>(pok), s)) {
         const uint64_t* st = a.status + (((k - 1) * a.B + c) * S + s) * kStatusWords;
-        uint32_t spins = 0;
-        while (!read_pair(st + 4, flag, sw0, sw1)) {
-          if (++spins > kSpinLimit) {
-            if (lane == 0) atomicOr(&a.hdr->error, 1u);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(2);
-        }
+        wait_inclusive(st + 4, a.hdr, flag, 0u, sw0, sw1);
       }
     }
     double e0 = static_cast<double>(p1), e1 = static_cast<double>(p2);

@@ -224,16 +224,9 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
       } else {
         sw0 = __shfl(pv0, s);
         sw1 = __shfl(pv1, s);
-        if (!__shfl(static_cast<int>(pok), s)) {
+        if (!__shfl(static_cast<int>(pok), s) || debug_give_up(a.flags)) {
           const uint64_t* st = a.status + (((k - 1) * a.B + c) * a.S + s) * kStatusWords;
-          uint32_t spins = 0;
-          while (!read_pair(st + 4, flag, sw0, sw1)) {
-            if (++spins > kSpinLimit) {
-              if (lane == 0) atomicOr(&a.hdr->error, 1u);
-              break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-          }
+          wait_inclusive(st + 4, a.hdr, flag, a.flags, sw0, sw1);
         }
       }
       // lane 0 carries the window's incoming state: e_0 += M sw
@@ -294,13 +287,13 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
         publish_pair(st, g0, g1, flag, lane);
         if (k > 0)
           lookback(a.status, a.B, a.S, a.hdr, k, c, s, flag, tb, 6 + kLog2NW, zi0, zi1, sw0,
-                   sw1);
+                   sw1, a.flags);
         const Mat2 Mwin = load_mat(table_pow(tb, 6 + kLog2NW));
         publish_pair(st + 4, Mwin.a * sw0 + Mwin.b * sw1 + g0, Mwin.c * sw0 + Mwin.d * sw1 + g1,
                      flag, lane);
       } else if (k > 0) {
         const uint64_t* st = a.status + (((k - 1) * a.B + c) * a.S + s) * kStatusWords;
-        if (!read_pair(st + 4, flag, sw0, sw1) && lane == 0) atomicOr(&a.hdr->error, 2u);
+        if (!read_pair(st + 4, flag, sw0, sw1)) give_up(a.hdr, 2u, sw0, sw1);
       }
       if (NW > 1 && lane == 0) {
         wsin[0] = sw0;
@@ -624,6 +617,8 @@ __global__ void copy_state_kernel(const TACC* zi, TACC* zf, int64_t B, int S, in
   zf[i] = zi ? zi[(bcast ? 0 : row) * S * 2 + within] : TACC(0);
 }
 
+__global__ void set_mailbox_kernel(WsHeader* hdr, uint32_t* mailbox) { hdr->mailbox = mailbox; }
+
 __global__ void fill_u32_kernel(uint32_t* p, size_t n) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
@@ -920,6 +915,23 @@ tsiir_status tsiir_workspace_init(void* workspace, size_t bytes, void* stream) {
   const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
   hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks), dim3(256), 0,
                      static_cast<hipStream_t>(stream), static_cast<uint32_t*>(workspace), n);
+  return hipGetLastError() == hipSuccess ? TSIIR_OK : TSIIR_ELAUNCH;
+}
+
+tsiir_status tsiir_workspace_set_error_mailbox(void* workspace, size_t workspace_bytes,
+                                               uint32_t* mailbox, void* stream) {
+  if (!workspace || workspace_bytes < sizeof(WsHeader)) return TSIIR_EINVAL;
+  uint32_t* dev = mailbox;
+  if (mailbox != nullptr) {
+    // pinned host memory: its device-side address; device memory: as given
+    void* dp = nullptr;
+    if (hipHostGetDevicePointer(&dp, mailbox, 0) == hipSuccess && dp != nullptr)
+      dev = static_cast<uint32_t*>(dp);
+    else
+      (void)hipGetLastError();
+  }
+  hipLaunchKernelGGL(set_mailbox_kernel, dim3(1), dim3(1), 0, static_cast<hipStream_t>(stream),
+                     static_cast<WsHeader*>(workspace), dev);
   return hipGetLastError() == hipSuccess ? TSIIR_OK : TSIIR_ELAUNCH;
 }
 

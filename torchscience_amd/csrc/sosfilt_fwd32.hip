@@ -344,13 +344,14 @@ static_assert((1 << kLbLog2) == kLbLanes, "look-back lanes");
 template <int NWV>
 __device__ __forceinline__ void lookback_groups(const uint64_t* status, int64_t B, int S,
                                                 WsHeader* hdr, int64_t gi, int64_t c, int s,
-                                                uint32_t flag, const double* tb, double zi0,
-                                                double zi1, double& sw0, double& sw1) {
+                                                uint32_t flag, uint32_t flags, const double* tb,
+                                                double zi0, double zi1, double& sw0,
+                                                double& sw1) {
   constexpr int kPbase = 6 + kLog2Nwv<NWV>;
   const int lane = lane_now();
-  if (gi >= 1 && span_power_is_zero(tb, kPbase)) {
+  if (gi >= 1 && (debug_give_up(flags) || span_power_is_zero(tb, kPbase))) {
     lookback_predecessor(status + ((((gi - 1) * NWV + NWV - 1) * B + c) * S + s) * kStatusWords,
-                         hdr, flag, sw0, sw1);
+                         hdr, flag, flags, sw0, sw1);
     return;
   }
   double acc0 = 0.0, acc1 = 0.0;
@@ -383,7 +384,7 @@ __device__ __forceinline__ void lookback_groups(const uint64_t* status, int64_t 
     const uint64_t need = (f >= 63) ? ~0ull : ((1ull << (f + 1)) - 1ull);
     if ((ready & need) != need) {
       if (++spins > kSpinLimit) {
-        if (lane == 0) atomicOr(&hdr->error, 1u);
+        give_up(hdr, 1u, acc0, acc1);
         break;
       }
       __builtin_amdgcn_s_sleep(2);
@@ -514,9 +515,10 @@ __device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32<NWV>
       }
       sw0 = zi0;
       sw1 = zi1;
-      if (w.kg > 0 && !(pf.valid && decode_pair(pf.words, flag, sw0, sw1)))
-        lookback_groups<NWV>(a.status, a.B, S, a.hdr, w.kg / NWV, w.c, s, flag, tb, zi0, zi1,
-                             sw0, sw1);
+      if (w.kg > 0 && !(pf.valid && !debug_give_up(a.flags) &&
+                        decode_pair(pf.words, flag, sw0, sw1)))
+        lookback_groups<NWV>(a.status, a.B, S, a.hdr, w.kg / NWV, w.c, s, flag, a.flags, tb,
+                             zi0, zi1, sw0, sw1);
       const Mat2 Mg = load_mat(table_pow(tb, 6 + kLog2Nwv<NWV>));
       const double i0 = fma(Mg.a, sw0, fma(Mg.b, sw1, g0));
       const double i1 = fma(Mg.c, sw0, fma(Mg.d, sw1, g1));
@@ -541,14 +543,7 @@ __device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32<NWV>
     sw1 = w.prev[2 * s + 1];
     if (!w.prev_ok[s]) {
       const uint64_t* sp = a.status + (((w.kg - 1) * a.B + w.c) * S + s) * kStatusWords;
-      uint32_t spins = 0;
-      while (!read_pair(sp + 4, flag, sw0, sw1)) {
-        if (++spins > kSpinLimit) {
-          if (lane == 0) atomicOr(&a.hdr->error, 1u);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(2);
-      }
+      wait_inclusive(sp + 4, a.hdr, flag, a.flags, sw0, sw1);
     }
   }
   // this wave's incoming state: the group's, carried over the windows of
@@ -699,7 +694,7 @@ __device__ __forceinline__ void read_group_in(const FwdArgs& a, const Win32<NWV>
   int ok = 1;
   if (w.kg > 0) {
     const uint64_t* st = a.status + (((w.kg - 1) * a.B + w.c) * S + lane) * kStatusWords;
-    ok = read_pair(st + 4, w.flag, p0, p1) ? 1 : 0;
+    ok = (read_pair(st + 4, w.flag, p0, p1) && !debug_give_up(a.flags)) ? 1 : 0;
   } else if (w.has_zi) {
     const float* zp = static_cast<const float*>(a.zi) + (w.zi_row * S + lane) * 2;
     p0 = static_cast<double>(zp[0]);

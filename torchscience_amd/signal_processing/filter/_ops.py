@@ -52,7 +52,8 @@ def sosfilt_rows_forward(sos: torch.Tensor, x: torch.Tensor, zi: torch.Tensor | 
                          *, channels: int, chans_per_filter: int, want_zf: bool,
                          zi_broadcast: bool = False, io_dtype: torch.dtype | None = None,
                          mode: int = _lib.MODE_PLAIN, path: str | None = None,
-                         keep_sections: bool = False, reverse: bool = False):
+                         keep_sections: bool = False, reverse: bool = False,
+                         _debug_give_up: bool = False):
     """y (channels, T) and zf (channels, S, 2) for planned row inputs.
 
     Channel c reads x row c % x.shape[0] and filter c // chans_per_filter.
@@ -60,7 +61,8 @@ def sosfilt_rows_forward(sos: torch.Tensor, x: torch.Tensor, zi: torch.Tensor | 
     work decomposition; results agree to rounding).  keep_sections: also
     return the outputs of sections 0..S-2, (S-1, channels, T), for the
     backward (else None).  reverse: filter along decreasing time (input read
-    and output written from the end; zi is the state at the end)."""
+    and output written from the end; zi is the state at the end).
+    _debug_give_up (tests only): every look-back wait gives up at once."""
     dev = _require_device(sos, x, zi)
     L = _lib.load()
     io = io_dtype or x.dtype
@@ -82,6 +84,8 @@ def sosfilt_rows_forward(sos: torch.Tensor, x: torch.Tensor, zi: torch.Tensor | 
         flags |= _lib.REVERSE
     if io != torch.float64 and mode == _lib.MODE_PLAIN and fp32_needs_df2t(sos):
         flags |= _lib.FP32_DF2T
+    if _debug_give_up:
+        flags |= _lib.DEBUG_GIVE_UP
     d = _desc(channels=channels, n_samples=T, n_sections=S, io_dtype=io, coef_dtype=sos.dtype,
               flags=flags, x_rows=max(x.shape[0], 1),
               x_row_stride=x.stride(0) if x.dim() == 2 else T, y_row_stride=T,
