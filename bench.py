@@ -609,8 +609,8 @@ def main():
     value = B * T * S / (ms * 1e-3) / 1e9
     achieved = algo_bytes / (ev_ms * 1e-3) / 1e9
 
-    from torchscience_amd import _lib
-    assert _lib.workspaces.header_error(dev) == 0, "look-back spin limit hit"
+    from torchscience_amd import check_errors
+    check_errors(dev)  # raises if a look-back wait gave up
 
     # achievable streaming bandwidth on this device: a 1 GiB device copy
     n_acc = ACCURACY_CHANNELS.get(args.workload, 0)

@@ -229,8 +229,8 @@ def test_deterministic_repeat():
     y0 = F.sosfilt(sos, x)
     for _ in range(5):
         assert torch.equal(F.sosfilt(sos, x), y0)
-    from torchscience_amd import _lib
-    assert _lib.workspaces.header_error(dev) == 0
+    from torchscience_amd import check_errors
+    check_errors(dev)
 
 
 def test_bf16_io(oracle_mod):

@@ -279,3 +279,78 @@ def test_lfilter_zi_rows_checked_before_launch(zi_shape):
     assert plan_lfilter_zi((4,), (4,), 4) is True
     assert plan_lfilter_zi((1, 4), (4,), 4) is True
     assert plan_lfilter_zi((2, 2, 4), (2, 2), 4) is False
+
+
+# ---- the operator boundary: TORCH_LIBRARY schemas, Meta implementations ----
+
+def test_operator_schemas_registered():
+    """libtsiir_torch.so registers tsiir::sosfilt_forward / sosfilt_backward
+    (the counterpart of the reference's TORCH_LIBRARY registration,
+    src/torchscience/csrc/torchscience.cpp:811) with HIP and Meta kernels."""
+    ops = _lib.ops()
+    fwd = str(ops.sosfilt_forward.default._schema)
+    bwd = str(ops.sosfilt_backward.default._schema)
+    assert fwd.startswith("tsiir::sosfilt_forward(Tensor x, Tensor sos, Tensor? zi")
+    assert "-> (Tensor y, Tensor zf, Tensor saved)" in fwd
+    assert "-> (Tensor dx, Tensor dsos, Tensor dzi)" in bwd
+    for name in ("sosfilt_forward", "sosfilt_backward"):
+        op = getattr(ops, name).default
+        assert torch._C._dispatch_has_kernel_for_dispatch_key(op.name(), "CUDA")
+        assert torch._C._dispatch_has_kernel_for_dispatch_key(op.name(), "Meta")
+
+
+@pytest.mark.parametrize("sos_dt,x_dt,out_dt", [(torch.float32, torch.float32, torch.float32),
+                                                (torch.float64, torch.float32, torch.float64),
+                                                (torch.float32, torch.bfloat16, torch.float32)])
+def test_meta_shape_inference(sos_dt, x_dt, out_dt):
+    """sosfilt on meta tensors: output shape, dtype promotion and zf layout
+    (zi with the batch dims of x and zi's dtype), with no device work."""
+    sos = torch.empty(4, 6, device="meta", dtype=sos_dt)
+    x = torch.empty(3, 7, 1000, device="meta", dtype=x_dt)
+    y = sosfilt(sos, x, axis=1)
+    assert y.shape == x.shape and y.device.type == "meta" and y.dtype == out_dt
+    zi = torch.empty(3, 1000, 4, 2, device="meta", dtype=torch.float64)
+    y, zf = sosfilt(sos, x, axis=1, zi=zi)
+    assert y.shape == x.shape and zf.shape == zi.shape and zf.dtype == torch.float64
+    yb, zfb = sosfilt(sos, x, zi=torch.empty(4, 2, device="meta"))
+    assert zfb.shape == (3, 7, 4, 2)
+    y2 = batched_filter_apply(torch.empty(5, 2, 6, device="meta"),
+                              torch.empty(5, 300, device="meta"), broadcast=False)
+    assert y2.shape == (5, 300)
+
+
+def test_torch_compile_traces_sosfilt_fullgraph():
+    """torch.compile(fullgraph=True) traces sosfilt (forward) and its
+    gradient (the autograd.Function around the operators) without a graph
+    break, on meta tensors (FakeTensor tracing through the Meta kernels)."""
+    torch._dynamo.reset()
+    sos = torch.randn(4, 6, device="meta")
+    x = torch.randn(3, 5, 1000, device="meta")
+    f = torch.compile(sosfilt, fullgraph=True, backend="eager")
+    assert f(sos, x, axis=1).shape == x.shape
+    sos_g = torch.randn(4, 6, device="meta", requires_grad=True)
+    x_g = torch.randn(8, 1000, device="meta", requires_grad=True)
+    zi_g = torch.zeros(8, 4, 2, device="meta", requires_grad=True)
+
+    def loss(s, v, z):
+        y, zf = sosfilt(s, v, zi=z)
+        return y.square().sum() + zf.sum()
+    g = torch.compile(loss, fullgraph=True, backend="aot_eager")
+    g(sos_g, x_g, zi_g).backward()
+    assert sos_g.grad.shape == (4, 6) and x_g.grad.shape == (8, 1000)
+    assert zi_g.grad.shape == (8, 4, 2)
+
+
+def test_pole_radius_routing_criterion():
+    """tsiir::max_pole_radius (the FP32 DF2T routing criterion) equals the
+    largest |root| of a0 z^2 + a1 z + a2 over the sections."""
+    import numpy as np
+    from torchscience_amd.signal_processing.filter._design import (butterworth_sos,
+                                                                    chebyshev1_sos)
+    for sos in (chebyshev1_sos(16, 0.01, 1.0), butterworth_sos(6, [0.05, 0.3], "bandpass"),
+                butterworth_sos(8, 0.02), np.array([[1.0, 0, 0, 2.0, -3.0, 1.12]]),
+                np.array([[1.0, 0, 0, 1.0, -1.9, 0.9]])):
+        r = _lib.ops().max_pole_radius(torch.tensor(sos))
+        ref = max(np.max(np.abs(np.roots(row[3:]))) for row in sos)
+        assert abs(r - ref) < 1e-12
+    assert _lib.ops().max_pole_radius(torch.tensor([[1.0, 0, 0, 0.0, 1.0, 0.5]])) == float("inf")

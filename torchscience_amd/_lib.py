@@ -121,6 +121,25 @@ class TsiirError(RuntimeError):
 
 _lib = None
 _lock = threading.Lock()
+TORCH_LIB_PATH = os.path.join(_PKG, "_lib", "libtsiir_torch.so")
+_ops_loaded = False
+
+
+def ops():
+    """torch.ops.tsiir: the operators registered by libtsiir_torch.so
+    (sosfilt_forward / sosfilt_backward with HIP and Meta implementations).
+    Loaded once; raises if the library has not been built."""
+    global _ops_loaded
+    if not _ops_loaded:
+        with _lock:
+            if not _ops_loaded:
+                load()
+                if not os.path.exists(TORCH_LIB_PATH):
+                    raise ImportError(f"libtsiir_torch.so not found at {TORCH_LIB_PATH}; build "
+                                      "it with `python -m torchscience_amd.build`")
+                torch.ops.load_library(TORCH_LIB_PATH)
+                _ops_loaded = True
+    return torch.ops.tsiir
 
 
 def load(path: str | None = None) -> ctypes.CDLL:
@@ -223,80 +242,17 @@ def io_code(dtype: torch.dtype) -> int:
     raise TypeError(f"unsupported I/O dtype {dtype}")
 
 
-class _Workspaces:
-    """One zero-initialised workspace per (device, stream, kind), grown on
-    demand, each with an error mailbox.
-
-    The kernels keep the workspace header consistent across launches, so a
-    buffer is initialised once, when it is allocated.  Keying by stream keeps
-    concurrent streams from sharing scan status words.  The mailbox is a
-    pinned host word the kernels write when a look-back wait gives up (the
-    affected outputs are then NaN): every later call that takes the
-    workspace reads it without synchronising and raises TsiirError, and
-    `check_errors` reads them all after a synchronisation."""
-
-    def __init__(self):
-        self._bufs: dict[tuple[int, int, str], tuple[torch.Tensor, torch.Tensor]] = {}
-
-    @staticmethod
-    def _key(device, kind):
-        stream = stream_handle(device)
-        return (device.index if device.index is not None else torch.cuda.current_device(),
-                stream, kind), stream
-
-    def get(self, device: torch.device, nbytes: int, kind: str = "fwd") -> torch.Tensor:
-        key, stream = self._key(device, kind)
-        with _lock:
-            entry = self._bufs.get(key)
-            if entry is not None and int(entry[1][0]) != 0:
-                code = int(entry[1][0])
-                del self._bufs[key]
-                raise TsiirError(
-                    f"a look-back wait of an earlier call on this stream gave up (error "
-                    f"{code}): the outputs of the affected windows are NaN")
-            if entry is None or entry[0].numel() < nbytes:
-                size = max(nbytes, 1 << 20)
-                size = (size + 255) // 256 * 256
-                buf = torch.empty(size, dtype=torch.uint8, device=device)
-                check(load().tsiir_workspace_init(buf.data_ptr(), size, stream),
-                      "tsiir_workspace_init")
-                mailbox = torch.zeros(1, dtype=torch.int32, pin_memory=True)
-                check(load().tsiir_workspace_set_error_mailbox(buf.data_ptr(), size,
-                                                               mailbox.data_ptr(), stream),
-                      "tsiir_workspace_set_error_mailbox")
-                entry = (buf, mailbox)
-                self._bufs[key] = entry
-            return entry[0]
-
-    def check_errors(self, device: torch.device | None = None) -> None:
-        """Synchronise and raise TsiirError if a look-back wait of any call
-        gave up (on `device`, or on every device)."""
-        with _lock:
-            items = list(self._bufs.items())
-        for key, (buf, mailbox) in items:
-            if device is not None and buf.device != torch.device(device):
-                continue
-            torch.cuda.synchronize(buf.device)
-            if int(mailbox[0]) != 0:
-                with _lock:
-                    self._bufs.pop(key, None)
-                raise TsiirError(f"a look-back wait gave up on {buf.device} (error "
-                                 f"{int(mailbox[0])}): the outputs of the affected windows "
-                                 "are NaN")
-
-    def header_error(self, device: torch.device, kind: str = "fwd") -> int:
-        """Error word of the workspace header (non-zero after a look-back
-        wait gave up).  Synchronises the stream."""
-        key, _ = self._key(device, kind)
-        entry = self._bufs.get(key)
-        if entry is None:
-            return 0
-        return int(entry[0][12:16].view(torch.int32).item())
-
-
-workspaces = _Workspaces()
-
-
 def check_errors(device: torch.device | None = None) -> None:
-    """Raise TsiirError if a look-back wait of an earlier call gave up."""
-    workspaces.check_errors(device)
+    """Synchronise `device` (or all devices) and raise TsiirError if a
+    look-back wait of any earlier call gave up (its outputs are NaN).  Calls
+    also report it themselves, without synchronising, on the next call on
+    the same stream."""
+    if device is not None:
+        torch.cuda.synchronize(device)
+    else:
+        for i in range(torch.cuda.device_count()):
+            torch.cuda.synchronize(i)
+    code = int(ops().take_error())
+    if code != 0:
+        raise TsiirError(f"a look-back wait gave up (error {code}): the outputs of the "
+                         "affected windows are NaN")

@@ -31,8 +31,31 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (ROCm toolchain required to build libtsiir.so)")
 
 
+TORCH_LIB_PATH = os.path.join(LIB_DIR, "libtsiir_torch.so")
+TORCH_OPS_SRC = os.path.join(CSRC, "torch_ops.cpp")
+
+
 def sources() -> list[str]:
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+
+
+def torch_ops_cmd(out: str) -> list[str]:
+    """The PyTorch operator registration (host C++ only, no device code):
+    torch's headers, HIP's runtime headers, linked against libtsiir.so
+    (found next to it at run time) and torch's libraries (already loaded by
+    `import torch` when the library is)."""
+    import torch
+    from torch.utils.cpp_extension import include_paths, library_paths
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    cxx = os.environ.get("CXX", "g++")
+    return [
+        cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wno-unused-parameter",
+        *[f"-I{p}" for p in include_paths()], "-I/opt/rocm/include", f"-I{INCLUDE}",
+        "-D__HIP_PLATFORM_AMD__=1", "-DUSE_ROCM=1", f"-D_GLIBCXX_USE_CXX11_ABI={abi}",
+        TORCH_OPS_SRC, *[f"-L{p}" for p in library_paths()], f"-L{LIB_DIR}",
+        "-ltsiir", "-lc10", "-lc10_hip", "-ltorch_cpu", "-ltorch_hip", "-ltorch",
+        "-Wl,-rpath,$ORIGIN", "-o", out,
+    ]
 
 
 OBJ_DIR = os.path.join(REPO_DIR, "_build", "obj")
@@ -87,6 +110,12 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB_PATH)
+    tmp = TORCH_LIB_PATH + ".tmp"
+    cmd = torch_ops_cmd(tmp)
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, TORCH_LIB_PATH)
     return LIB_PATH
 
 

@@ -11,15 +11,20 @@ from __future__ import annotations
 import torch
 
 from ... import _lib
-from ._plan import fp32_needs_df2t
+
+# the operators tsiir::sosfilt_forward / sosfilt_backward (HIP and Meta
+# implementations, libtsiir_torch.so), registered when this module loads
+_lib.ops()
 
 
 def _require_device(*tensors: torch.Tensor) -> torch.device:
+    """The common device of the tensors: a HIP device, or "meta" (shape
+    inference, tracing)."""
     dev = None
     for t in tensors:
         if t is None:
             continue
-        if t.device.type != "cuda":
+        if t.device.type not in ("cuda", "meta"):
             raise RuntimeError(
                 "torchscience_amd filters run on MI355X (HIP) devices; got a tensor on "
                 f"{t.device}. Move the inputs to a 'cuda' device.")
@@ -54,7 +59,8 @@ def sosfilt_rows_forward(sos: torch.Tensor, x: torch.Tensor, zi: torch.Tensor | 
                          mode: int = _lib.MODE_PLAIN, path: str | None = None,
                          keep_sections: bool = False, reverse: bool = False,
                          _debug_give_up: bool = False):
-    """y (channels, T) and zf (channels, S, 2) for planned row inputs.
+    """y (channels, T) and zf (channels, S, 2) for planned row inputs,
+    through the operator tsiir::sosfilt_forward.
 
     Channel c reads x row c % x.shape[0] and filter c // chans_per_filter.
     path: None (chosen from the shape), "window" or "channel" (forces the
@@ -63,103 +69,42 @@ def sosfilt_rows_forward(sos: torch.Tensor, x: torch.Tensor, zi: torch.Tensor | 
     backward (else None).  reverse: filter along decreasing time (input read
     and output written from the end; zi is the state at the end).
     _debug_give_up (tests only): every look-back wait gives up at once."""
-    dev = _require_device(sos, x, zi)
-    L = _lib.load()
+    _require_device(sos, x, zi)
     io = io_dtype or x.dtype
-    T = x.shape[1]
-    S = sos.shape[1]
-    acc = torch.float64 if io == torch.float64 else torch.float32
-    y = torch.empty((channels, T), dtype=io, device=dev)
-    zf = torch.empty((channels, S, 2), dtype=acc, device=dev) if want_zf else None
+    if x.dtype != io:
+        x = x.to(io)
     flags = 0
-    if zi is not None:
-        flags |= _lib.HAS_ZI
-        if zi_broadcast:
-            flags |= _lib.ZI_BROADCAST
-    if want_zf:
-        flags |= _lib.WRITE_ZF
     if path is not None:
         flags |= {"window": _lib.PATH_WINDOW, "channel": _lib.PATH_CHANNEL}[path]
     if reverse:
         flags |= _lib.REVERSE
-    if io != torch.float64 and mode == _lib.MODE_PLAIN and fp32_needs_df2t(sos):
-        flags |= _lib.FP32_DF2T
     if _debug_give_up:
         flags |= _lib.DEBUG_GIVE_UP
-    d = _desc(channels=channels, n_samples=T, n_sections=S, io_dtype=io, coef_dtype=sos.dtype,
-              flags=flags, x_rows=max(x.shape[0], 1),
-              x_row_stride=x.stride(0) if x.dim() == 2 else T, y_row_stride=T,
-              chans_per_filter=max(chans_per_filter, 1), mode=mode)
-    nbytes = L.tsiir_sosfilt_workspace_bytes(d)
-    if nbytes == 0:
-        raise _lib.TsiirError("invalid sosfilt descriptor")
-    ws = _lib.workspaces.get(dev, nbytes, "fwd")
-    st = _lib.stream_handle(dev)
+    y, zf, saved = torch.ops.tsiir.sosfilt_forward(x, sos, zi, channels, chans_per_filter, want_zf,
+                                              zi_broadcast, mode, flags, keep_sections)
+    zf = zf if want_zf else None
     if keep_sections:
-        saved = torch.empty((max(S - 1, 0), channels, T), dtype=io, device=dev)
-        _lib.check(L.tsiir_sosfilt_forward_saved(d, _lib.ptr(x), _lib.ptr(y), _lib.ptr(sos),
-                                                 _lib.ptr(zi), _lib.ptr(zf),
-                                                 _lib.ptr(saved) if S > 1 else None,
-                                                 ws.data_ptr(), ws.numel(), st),
-                   "tsiir_sosfilt_forward_saved")
         return y, zf, saved
-    _lib.check(L.tsiir_sosfilt_forward(d, _lib.ptr(x), _lib.ptr(y), _lib.ptr(sos), _lib.ptr(zi),
-                                       _lib.ptr(zf), ws.data_ptr(), ws.numel(), st),
-               "tsiir_sosfilt_forward")
     return y, zf
 
 
-def sosfilt_rows_backward(sos, x, zi, dy, dzf, *, chans_per_filter: int,
-                          zi_broadcast: bool = False, y=None, saved=None):
-    """(dx, dsos, dzi) for sosfilt_rows_forward with x rows == channels.
-
-    With y (the forward output) and saved (the section outputs that
-    sosfilt_rows_forward(keep_sections=True) returned) the forward is not
-    recomputed."""
-    dev = _require_device(sos, x, zi, dy, dzf)
-    L = _lib.load()
-    B, T = dy.shape
-    S = sos.shape[1]
-    acc = torch.float64 if x.dtype == torch.float64 else torch.float32
-    flags = 0
-    if zi is not None:
-        flags |= _lib.HAS_ZI
-        if zi_broadcast:
-            flags |= _lib.ZI_BROADCAST
-    if dzf is not None:
-        flags |= _lib.WRITE_ZF
-    d = _desc(channels=B, n_samples=T, n_sections=S, io_dtype=x.dtype, coef_dtype=sos.dtype,
-              flags=flags, x_rows=max(B, 1), x_row_stride=T, y_row_stride=T,
-              chans_per_filter=max(chans_per_filter, 1))
-    use_saved = y is not None
-    nbytes = (L.tsiir_sosfilt_backward_saved_workspace_bytes(d) if use_saved
-              else L.tsiir_sosfilt_backward_workspace_bytes(d))
-    if nbytes == 0:
-        raise _lib.TsiirError("invalid sosfilt backward descriptor")
-    ws = _lib.workspaces.get(dev, nbytes, "bwd")
-    dx = torch.empty_like(dy)
-    dsos = torch.empty_like(sos)
-    dzi = torch.empty((B, S, 2), dtype=acc, device=dev)
-    st = _lib.stream_handle(dev)
-    if use_saved:
-        _lib.check(L.tsiir_sosfilt_backward_saved(
-            d, _lib.ptr(x), _lib.ptr(y), _lib.ptr(saved) if S > 1 else None, _lib.ptr(sos),
-            _lib.ptr(zi), _lib.ptr(dy), _lib.ptr(dzf), _lib.ptr(dx), _lib.ptr(dsos),
-            _lib.ptr(dzi), ws.data_ptr(), ws.numel(), st), "tsiir_sosfilt_backward_saved")
-        return dx, dsos, dzi
-    _lib.check(L.tsiir_sosfilt_backward(d, _lib.ptr(x), _lib.ptr(sos), _lib.ptr(zi),
-                                        _lib.ptr(dy), _lib.ptr(dzf), _lib.ptr(dx),
-                                        _lib.ptr(dsos), _lib.ptr(dzi), ws.data_ptr(),
-                                        ws.numel(), st),
-               "tsiir_sosfilt_backward")
-    return dx, dsos, dzi
+def sosfilt_rows_backward(sos, x, zi, dy, dzf, *, chans_per_filter: int, y, saved,
+                          zi_broadcast: bool = False):
+    """(dx, dsos, dzi) through the operator tsiir::sosfilt_backward, from the
+    forward output y and the section outputs `saved` that
+    sosfilt_rows_forward(keep_sections=True) returned (no forward recompute);
+    x has one row per channel."""
+    _require_device(sos, x, zi, dy, dzf)
+    return torch.ops.tsiir.sosfilt_backward(x, y, saved, sos, zi, dy, dzf, chans_per_filter,
+                                       zi_broadcast)
 
 
 class SosfiltRows(torch.autograd.Function):
-    """Autograd over sosfilt_rows_forward.  The forward keeps every section's
-    output (tsiir_sosfilt_forward_saved); backward runs the hand-written
-    adjoint kernels (time-reversed section filters with coefficient-gradient
-    accumulation) on them through tsiir_sosfilt_backward_saved."""
+    """Autograd over the operators.  The forward (tsiir::sosfilt_forward with
+    keep_sections) keeps every section's output; the backward
+    (tsiir::sosfilt_backward) runs the hand-written adjoint kernels
+    (time-reversed section filters with coefficient-gradient accumulation)
+    on them."""
 
     @staticmethod
     def forward(ctx, x, sos, zi, channels, chans_per_filter, want_zf, zi_broadcast):

@@ -146,49 +146,3 @@ def plan_lfilter_zi(zi_shape, batch_shape, order: int) -> bool:
             f"zi must have shape {tuple(batch_shape) + (order,)} (or ({order},)), "
             f"got {zi_shape}: {rows} state rows for {B} channels")
     return rows == 1 and B > 1
-
-
-# Pole radius above which the FP32 recurrence runs in DF2T order with FP64
-# carries (TSIIR_FP32_DF2T) instead of the direct-form-I kernel.  Measured on
-# 1024 x 65536 and 16 x 2^20 FP32 signals against the FP64 oracle: the
-# direct-form-I kernel stays below the reference's own FP32 DF2T error for
-# every Butterworth / Chebyshev I filter with 1 - r >= 6e-3, and exceeds it
-# (up to 2x: 4.3e-2 against 2.0e-2 on the order-16 Chebyshev I at cutoff 0.01,
-# 1 - r = 2.75e-4) only below 3.1e-3; the DF2T kernel is below the
-# reference's error on all of them.
-DF2T_POLE_RADIUS = 0.995
-_route_cache: dict = {}
-
-
-def max_pole_radius(sos: torch.Tensor) -> float:
-    """Largest |pole| over the sections of sos (..., 6): the roots of
-    a0 z^2 + a1 z + a2 (infinite for a0 == 0)."""
-    a = sos.detach().to(torch.float64)[..., 3:6].reshape(-1, 3)
-    if a.shape[0] == 0:
-        return 0.0
-    a0, a1, a2 = a.unbind(-1)
-    disc = a1 * a1 - 4.0 * a0 * a2
-    rc = torch.sqrt(torch.abs(a2 / a0))
-    sq = torch.sqrt(torch.clamp(disc, min=0.0))
-    rr = torch.maximum(torch.abs((-a1 + sq) / (2.0 * a0)), torch.abs((-a1 - sq) / (2.0 * a0)))
-    r = torch.where(disc < 0, rc, rr)
-    r = torch.where(a0 == 0, torch.full_like(r, float("inf")), r)
-    return float(torch.nan_to_num(r, nan=float("inf")).max())
-
-
-def fp32_needs_df2t(sos: torch.Tensor) -> bool:
-    """True when some pole of sos lies beyond DF2T_POLE_RADIUS.  Device
-    tensors are read once per (storage, version): the answer only picks
-    between two correct kernels, so a stale entry after an address is reused
-    can cost accuracy on such poles, never correctness."""
-    if sos.device.type == "cpu":
-        return max_pole_radius(sos) > DF2T_POLE_RADIUS
-    key = (sos.data_ptr(), sos._version, tuple(sos.shape), sos.dtype, str(sos.device))
-    hit = _route_cache.get(key)
-    if hit is None:
-        if len(_route_cache) > 256:
-            _route_cache.clear()
-        hit = max_pole_radius(sos) > DF2T_POLE_RADIUS
-        _route_cache[key] = hit
-    return hit
-
