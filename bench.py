@@ -170,6 +170,33 @@ def cpu_baseline(w, budget_s):
                       f"FP64 DF2T C oracle, fp32 I/O, OpenMP {threads} threads, {dt:.1f} s"}
 
 
+def cpu_reference_loop(w, n_samples=1024):
+    """The reference's own algorithm on the host: oracle/torch_ref.py's
+    op-for-op restatement of the per-sample DF2T loop (_lfilter.py:153-166,
+    vectorised over the batch, sections in order as _sosfilt.py:117-128),
+    timed on the first n_samples of every channel with all host threads and
+    extrapolated linearly to the full length (its cost per sample is
+    constant)."""
+    import torch
+    from oracle import torch_ref
+    B, T, sos = w["B"], w["T"], w["sos"]
+    S = sos.shape[-2]
+    n = min(n_samples, T)
+    x = w["x"][:, :n].float().cpu()
+    if sos.ndim == 3:  # per-channel filters: the reference loops over them
+        return None
+    sos_t = torch.tensor(sos, dtype=torch.float32)
+    t0 = time.perf_counter()
+    torch_ref.sosfilt_loop_timed(sos_t, x, n)
+    dt = time.perf_counter() - t0
+    return {"value": B * n * S / dt / 1e9, "unit": "G sample·sections/s",
+            "cores": torch.get_num_threads(), "kind": "port",
+            "sample": f"all {B} channels x the first {n} of {T} samples, {S} sections, the "
+                      f"reference's per-sample loop restated in torch ops (FP32), "
+                      f"{torch.get_num_threads()} threads, {dt:.2f} s; extrapolated to full T: "
+                      f"{dt * T / n:.1f} s per step"}
+
+
 def accuracy(y, x, sos_np, coef_dtype, n):
     """Normwise errors of the first n channels against the FP64 oracle: per
     channel max|y - y64| / max|y64| (worst channel) and the global RMS ratio,
@@ -660,6 +687,7 @@ def main():
                                    np.float64 if sos.dtype == torch.float64 else np.float32, n_acc)
     if not args.no_cpu_baseline and args.workload != "c3":
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
+        out["cpu_reference_loop"] = cpu_reference_loop(w)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
