@@ -30,7 +30,8 @@
  *  - sos is (n_filters, n_sections, 6) contiguous, rows [b0 b1 b2 a0 a1 a2];
  *    each row is normalised by a0 when a0 != 1, as _lfilter.py:96-98 does.
  *  - zi / zf are (channels, n_sections, 2) contiguous, in the compute type
- *    (float for TSIIR_F32 / TSIIR_BF16 I/O, double for TSIIR_F64).
+ *    (float for TSIIR_F32 / TSIIR_BF16 I/O, double for TSIIR_F64), or in
+ *    double with TSIIR_STATE_F64.
  */
 #ifndef TSIIR_H
 #define TSIIR_H
@@ -88,6 +89,9 @@ typedef enum tsiir_mode {
  * windows after the first of each channel come out NaN and the error is
  * reported, with every kernel exiting normally. */
 #define TSIIR_DEBUG_GIVE_UP 0x100u
+/* zi / zf hold FP64 values whatever io_dtype is (a caller that carries a
+ * stream's state between chunks in FP64 with FP32 or BF16 samples). */
+#define TSIIR_STATE_F64 0x200u
 
 typedef struct tsiir_sosfilt_desc {
   int64_t channels;          /* output channels B                          */

@@ -93,3 +93,45 @@ def test_alone_equals_in_mix(precision):
         torch.cuda.synchronize()
         assert torch.equal(r.y, mixed[s])
         assert np.array_equal(table.snapshot(s), mixed_state[s])
+
+
+def test_long_chunks_commit_on_success_and_evict_restore(oracle_mod):
+    """Long chunks (window-parallel forward from the FP64 state): a launch
+    whose look-back gives up (forced from the host) is marked failed and its
+    stream keeps its state; the next chunk continues from it.  A stream
+    evicted to the host and restored continues exactly."""
+    from torchscience_amd.serving import HipExecutor, Request, ServingEngine, StreamTable
+    from torchscience_amd.signal_processing.filter._design import butterworth_sos
+    dev = _dev()
+    rng = np.random.default_rng(21)
+    sos = butterworth_sos(8, 0.1)
+    C = 3
+    table = StreamTable(dev)
+    table.register(0, sos, C)
+    table.ready()
+    xs = [rng.standard_normal((C, n)).astype(np.float32) for n in (40000, 30000, 50000)]
+    # chunk 0 fails (forced give-up), chunk 1 then runs from the initial state
+    bad = Request(0, 0, torch.from_numpy(xs[0]).to(dev), 0.0)
+    ServingEngine(table, HipExecutor(dev, 2, _debug_give_up=True),
+                  long_samples=4096).run([bad])
+    torch.cuda.synchronize()
+    assert bad.status == "failed"
+    assert np.array_equal(table.snapshot(0), np.zeros((C, 4, 2)))
+    ok = Request(1, 0, torch.from_numpy(xs[1]).to(dev), 0.0)
+    ServingEngine(table, HipExecutor(dev, 2), long_samples=4096).run([ok])
+    torch.cuda.synchronize()
+    assert ok.status == "done"
+    host = table.evict(0)
+    assert not table.resident(0)
+    with pytest.raises(ValueError, match="evicted"):
+        ServingEngine(table, HipExecutor(dev, 2)).run([Request(9, 0, ok.x, 0.0)])
+    table.restore(0, host)
+    last = Request(2, 0, torch.from_numpy(xs[2]).to(dev), 0.0)
+    ServingEngine(table, HipExecutor(dev, 2), long_samples=4096).run([last])
+    torch.cuda.synchronize()
+    xcat = np.concatenate([xs[1], xs[2]], axis=1).astype(np.float64)
+    ycat = np.concatenate([ok.y.cpu().numpy(), last.y.cpu().numpy()], axis=1)
+    ref, zf = oracle_mod.sosfilt_rows(sos, xcat)
+    err = np.max(np.abs(ycat - ref), axis=1) / np.max(np.abs(ref), axis=1)
+    assert err.max() < 1e-5
+    np.testing.assert_allclose(table.snapshot(0), zf, rtol=1e-6, atol=1e-7)

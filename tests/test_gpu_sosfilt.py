@@ -323,3 +323,32 @@ def test_lfiltic_continuation_on_device():
         y2, _ = F.lfilter(bt, at, x[300:], zi=zi)
         np.testing.assert_allclose(torch.cat([y1, y2]).cpu().numpy(), full.cpu().numpy(),
                                    rtol=1e-10, atol=1e-10)
+
+
+@pytest.mark.parametrize("B", [5, 1100])
+def test_compensated_form_carries_float64_state(oracle_mod, B):
+    """fp32c with a float64 zi keeps the state in float64 between chunks
+    (TSIIR_STATE_F64; zf has zi's dtype): streaming three uneven chunks equals
+    one-shot FP64 filtering of the whole signal (few channels: windows with
+    the look-back; many: chained windows)."""
+    F, dev = _filt(), _dev()
+    from torchscience_amd.signal_processing.filter._design import chebyshev1_sos
+    sos = chebyshev1_sos(8, 0.02, 1.0)
+    rng = np.random.default_rng(B)
+    lens = [5000, 70001, 12345]
+    x = rng.standard_normal((B, sum(lens))).astype(np.float32)
+    zi = torch.zeros(B, 4, 2, dtype=torch.float64, device=dev)
+    st = torch.tensor(sos, device=dev)
+    ys, t0 = [], 0
+    for n in lens:
+        y, zi = F.sosfilt(st, torch.tensor(x[:, t0:t0 + n], device=dev), zi=zi,
+                          precision="fp32c")
+        assert y.dtype == torch.float32 and zi.dtype == torch.float64
+        ys.append(y.cpu().numpy())
+        t0 += n
+    ref, zf_ref = oracle_mod.sosfilt_rows(sos, x.astype(np.float64), n_threads=8)
+    err = relerr(np.concatenate(ys, axis=1), ref)
+    print("fp32c streamed with float64 state", B, err)
+    assert err < 1e-5
+    np.testing.assert_allclose(zi.cpu().numpy(), zf_ref, rtol=1e-6,
+                               atol=1e-6 * np.max(np.abs(zf_ref)))

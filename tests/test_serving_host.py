@@ -167,3 +167,55 @@ def test_engine_rejects_requests_it_cannot_serve(oracle_mod, bad):
                         clock=lambda: float(clock()))
     with pytest.raises(ValueError):
         eng.run([Request(0, sid, x, 0.0, y=y)])
+
+
+def test_cancelled_and_expired_requests_are_dropped(oracle_mod):
+    """A request cancelled, or past its deadline, before its launch is never
+    launched, gets its status, and its stream's state does not advance: the
+    stream's served chunks equal one-shot filtering of their concatenation
+    without the dropped chunks."""
+    import time
+    sos, chans, reqs = _workload(11, n_streams=4, n_req=40, rate=1e7)
+    table = FakeTable(sos, chans)
+    ex = FakeExecutor(oracle_mod, time.perf_counter, None, per_sample=1e-6)
+    eng = ServingEngine(table, ex, max_inflight=1)
+    cancelled = {reqs[3].rid, reqs[17].rid}
+    for rid in cancelled:
+        eng.cancel(rid)
+    # deadlines already passed on arrival (all requests arrive at once)
+    for r in reqs[20:24]:
+        r.deadline = -1.0
+    stats = eng.run(reqs)
+    launched = {i for b in eng.batches for i in b}
+    for r in reqs:
+        if r.rid in cancelled:
+            assert r.status == "cancelled" and r.rid not in launched
+        elif r.deadline is not None:
+            assert r.status == "expired" and r.rid not in launched
+        else:
+            assert r.status == "done" and r.rid in launched
+    assert stats.latencies.size == len(reqs) - 6
+    by_stream = {}
+    for r in sorted(reqs, key=lambda r: (r.arrival, r.rid)):
+        if r.status == "done":
+            by_stream.setdefault(r.stream_id, []).append(r)
+    for sid, rs in by_stream.items():
+        xcat = np.concatenate([r.x.numpy() for r in rs], axis=1).astype(np.float64)
+        ycat = np.concatenate([r.y.numpy() for r in rs], axis=1)
+        ref, _ = oracle_mod.sosfilt_rows(sos[sid], xcat)
+        np.testing.assert_allclose(ycat, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_cancel_from_another_thread(oracle_mod):
+    """cancel() is safe to call while run() is going."""
+    import threading
+    import time
+    sos, chans, reqs = _workload(12, n_streams=3, n_req=30, rate=300.0)
+    eng = ServingEngine(FakeTable(sos, chans),
+                        FakeExecutor(oracle_mod, time.perf_counter, None), max_inflight=1)
+    last = max(reqs, key=lambda r: r.arrival)
+    th = threading.Thread(target=lambda: eng.cancel(last.rid))
+    th.start()
+    eng.run(reqs)
+    th.join()
+    assert last.status == "cancelled"

@@ -25,6 +25,7 @@ PATH_WINDOW, PATH_CHANNEL = 0x10, 0x20
 FIR_REVERSE_TAPS = 0x40
 FP32_DF2T = 0x80
 DEBUG_GIVE_UP = 0x100
+STATE_F64 = 0x200
 
 # Every symbol include/tsiir.h declares.
 EXPORTED = (

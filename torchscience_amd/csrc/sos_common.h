@@ -49,6 +49,21 @@ __device__ __forceinline__ bool debug_give_up(uint32_t flags) {
   return (flags & TSIIR_DEBUG_GIVE_UP) != 0;
 }
 
+// Element idx of zi / zf ((rows, S, 2) row-major): double with
+// TSIIR_STATE_F64, else the compute type TACC.
+template <typename TACC>
+__device__ __forceinline__ double state_in(const void* zi, uint32_t flags, int64_t idx) {
+  if (flags & TSIIR_STATE_F64) return static_cast<const double*>(zi)[idx];
+  return static_cast<double>(static_cast<const TACC*>(zi)[idx]);
+}
+template <typename TACC>
+__device__ __forceinline__ void state_out(void* zf, uint32_t flags, int64_t idx, double v) {
+  if (flags & TSIIR_STATE_F64)
+    static_cast<double*>(zf)[idx] = v;
+  else
+    static_cast<TACC*>(zf)[idx] = static_cast<TACC>(v);
+}
+
 struct Mat2 {
   double a, b, c, d;  // [[a, b], [c, d]]
 };

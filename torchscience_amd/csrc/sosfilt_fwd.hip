@@ -217,9 +217,9 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
       double sw0 = 0.0, sw1 = 0.0;
       if (k == 0) {
         if (has_zi) {
-          const TACC* zp = static_cast<const TACC*>(a.zi) + (zi_row * a.S + s) * 2;
-          sw0 = static_cast<double>(zp[0]);
-          sw1 = static_cast<double>(zp[1]);
+          const int64_t zi_i = (zi_row * a.S + s) * 2;
+          sw0 = state_in<TACC>(a.zi, a.flags, zi_i);
+          sw1 = state_in<TACC>(a.zi, a.flags, zi_i + 1);
         }
       } else {
         sw0 = __shfl(pv0, s);
@@ -277,9 +277,9 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
     if (wv == 0) {
       double zi0 = 0.0, zi1 = 0.0;
       if (has_zi) {
-        const TACC* zp = static_cast<const TACC*>(a.zi) + (zi_row * a.S + s) * 2;
-        zi0 = static_cast<double>(zp[0]);
-        zi1 = static_cast<double>(zp[1]);
+        const int64_t zi_i = (zi_row * a.S + s) * 2;
+        zi0 = state_in<TACC>(a.zi, a.flags, zi_i);
+        zi1 = state_in<TACC>(a.zi, a.flags, zi_i + 1);
       }
       double sw0 = zi0, sw1 = zi1;
       if (!TAIL) {
@@ -349,16 +349,16 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
       pw0 = 0;
       pw1 = 0;
       if (write_zf) {
-        TACC* zp = static_cast<TACC*>(a.zf) + (c * a.S + s) * 2;
-        zp[0] = static_cast<TACC>(w0 + z0);
-        zp[1] = static_cast<TACC>(w1 + z1);
+        const int64_t zf_i = (c * a.S + s) * 2;
+        state_out<TACC>(a.zf, a.flags, zf_i, static_cast<double>(w0 + z0));
+        state_out<TACC>(a.zf, a.flags, zf_i + 1, static_cast<double>(w1 + z1));
       }
     } else if (write_zf) {
       // state after the window's last sample: zero-state part + A^W s_q
       const Mat2 M = load_mat(table_pow(tb, 0));
-      TACC* zp = static_cast<TACC*>(a.zf) + (c * a.S + s) * 2;
-      zp[0] = static_cast<TACC>(static_cast<double>(z0) + M.a * s0 + M.b * s1);
-      zp[1] = static_cast<TACC>(static_cast<double>(z1) + M.c * s0 + M.d * s1);
+      const int64_t zf_i = (c * a.S + s) * 2;
+      state_out<TACC>(a.zf, a.flags, zf_i, static_cast<double>(z0) + M.a * s0 + M.b * s1);
+      state_out<TACC>(a.zf, a.flags, zf_i + 1, static_cast<double>(z1) + M.c * s0 + M.d * s1);
     }
     if (a.saved != nullptr && s + 1 < a.S) {
       // make the registers exact now (the pending correction becomes zero)
@@ -501,9 +501,9 @@ __global__ __launch_bounds__(64) void sos_chan_kernel(FwdArgs a,
   double ws0 = 0.0, ws1 = 0.0;  // lane s < S: section s's state
   if ((a.flags & TSIIR_HAS_ZI) && lane < S) {
     const int64_t zrow = (a.flags & TSIIR_ZI_BROADCAST) ? 0 : c;
-    const TACC* zp = static_cast<const TACC*>(a.zi) + (zrow * S + lane) * 2;
-    ws0 = static_cast<double>(zp[0]);
-    ws1 = static_cast<double>(zp[1]);
+    const int64_t zi_i = (zrow * S + lane) * 2;
+    ws0 = state_in<TACC>(a.zi, a.flags, zi_i);
+    ws1 = state_in<TACC>(a.zi, a.flags, zi_i + 1);
   }
 
   u32x4 pf[L::kNV];
@@ -535,9 +535,9 @@ __global__ __launch_bounds__(64) void sos_chan_kernel(FwdArgs a,
     store_run<TIO, TACC, W>(yr + (m > 0 ? t0 : 0), false, m, v);
   }
   if ((a.flags & TSIIR_WRITE_ZF) && lane < S) {
-    TACC* zp = static_cast<TACC*>(a.zf) + (c * S + lane) * 2;
-    zp[0] = static_cast<TACC>(ws0);
-    zp[1] = static_cast<TACC>(ws1);
+    const int64_t zf_i = (c * S + lane) * 2;
+    state_out<TACC>(a.zf, a.flags, zf_i, ws0);
+    state_out<TACC>(a.zf, a.flags, zf_i + 1, ws1);
   }
 }
 
@@ -832,7 +832,7 @@ tsiir_status forward_impl(const tsiir_sosfilt_desc* d, const void* x, void* y, c
     const int64_t n = d->channels * d->n_sections * 2;
     const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
     const int bc = (d->flags & TSIIR_ZI_BROADCAST) ? 1 : 0;
-    if (f64)
+    if (f64 || (d->flags & TSIIR_STATE_F64))
       hipLaunchKernelGGL(copy_state_kernel<double>, dim3(blocks), dim3(256), 0, st,
                          has_zi ? static_cast<const double*>(zi) : nullptr,
                          static_cast<double*>(zf), d->channels, d->n_sections, bc);

@@ -509,9 +509,9 @@ __device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32<NWV>
       publish_pair(st, g0, g1, flag, lane);
       double zi0 = 0.0, zi1 = 0.0;
       if (w.has_zi) {
-        const float* zp = static_cast<const float*>(a.zi) + (w.zi_row * S + s) * 2;
-        zi0 = static_cast<double>(zp[0]);
-        zi1 = static_cast<double>(zp[1]);
+        const int64_t zi_i = (w.zi_row * S + s) * 2;
+        zi0 = state_in<float>(a.zi, a.flags, zi_i);
+        zi1 = state_in<float>(a.zi, a.flags, zi_i + 1);
       }
       sw0 = zi0;
       sw1 = zi1;
@@ -524,9 +524,8 @@ __device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32<NWV>
       const double i1 = fma(Mg.c, sw0, fma(Mg.d, sw1, g1));
       publish_pair(st + 4, i0, i1, flag, lane);
       if (w.write_zf && lane == 0) {
-        float* zp = static_cast<float*>(a.zf) + (w.c * S + s) * 2;
-        zp[0] = static_cast<float>(i0);
-        zp[1] = static_cast<float>(i1);
+        state_out<float>(a.zf, a.flags, (w.c * S + s) * 2, i0);
+        state_out<float>(a.zf, a.flags, (w.c * S + s) * 2 + 1, i1);
       }
       if (NWV > 1 && lane == 0) {
         sg[0] = sw0;
@@ -564,9 +563,8 @@ __device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32<NWV>
     uint64_t* st = a.status + (((w.kg + NWV - 1) * a.B + w.c) * S + s) * kStatusWords;
     publish_pair(st + 4, i0, i1, flag, lane);
     if (w.write_zf && lane == 0) {
-      float* zp = static_cast<float*>(a.zf) + (w.c * S + s) * 2;
-      zp[0] = static_cast<float>(i0);
-      zp[1] = static_cast<float>(i1);
+      state_out<float>(a.zf, a.flags, (w.c * S + s) * 2, i0);
+      state_out<float>(a.zf, a.flags, (w.c * S + s) * 2 + 1, i1);
     }
   }
 
@@ -696,9 +694,9 @@ __device__ __forceinline__ void read_group_in(const FwdArgs& a, const Win32<NWV>
     const uint64_t* st = a.status + (((w.kg - 1) * a.B + w.c) * S + lane) * kStatusWords;
     ok = (read_pair(st + 4, w.flag, p0, p1) && !debug_give_up(a.flags)) ? 1 : 0;
   } else if (w.has_zi) {
-    const float* zp = static_cast<const float*>(a.zi) + (w.zi_row * S + lane) * 2;
-    p0 = static_cast<double>(zp[0]);
-    p1 = static_cast<double>(zp[1]);
+    const int64_t zi_i = (w.zi_row * S + lane) * 2;
+    p0 = state_in<float>(a.zi, a.flags, zi_i);
+    p1 = state_in<float>(a.zi, a.flags, zi_i + 1);
   }
   prev_ok[lane] = ok;
   prev[2 * lane] = p0;

@@ -182,10 +182,17 @@ void check_rows(const at::Tensor& x, const at::Tensor& sos, int64_t channels,
               "tsiir: bad channel counts");
 }
 
+// The state type: double for double samples, and when zi comes in double
+// with float or bfloat16 samples (TSIIR_STATE_F64: a caller that carries the
+// state in FP64 between chunks); else float.
+bool state_f64(const at::Tensor& x, const c10::optional<at::Tensor>& zi, int64_t mode) {
+  return x.scalar_type() != at::kDouble && zi.has_value() && zi->scalar_type() == at::kDouble;
+}
+
 Fwd forward_shapes(const at::Tensor& x, const at::Tensor& sos, int64_t channels, bool want_zf,
-                   bool keep_sections) {
+                   bool keep_sections, bool f64_state) {
   const int64_t S = sos.size(1), T = x.size(1);
-  const auto acc = x.options().dtype(acc_type(x.scalar_type()));
+  const auto acc = x.options().dtype(f64_state ? at::kDouble : acc_type(x.scalar_type()));
   at::Tensor y = at::empty({channels, T}, x.options());
   at::Tensor zf = want_zf ? at::empty({channels, S, 2}, acc) : at::empty({0}, acc);
   at::Tensor saved = (keep_sections && S > 1) ? at::empty({S - 1, channels, T}, x.options())
@@ -198,7 +205,7 @@ Fwd sosfilt_forward_meta(const at::Tensor& x, const at::Tensor& sos,
                          int64_t chans_per_filter, bool want_zf, bool zi_broadcast, int64_t mode,
                          int64_t path_flags, bool keep_sections) {
   check_rows(x, sos, channels, chans_per_filter);
-  return forward_shapes(x, sos, channels, want_zf, keep_sections);
+  return forward_shapes(x, sos, channels, want_zf, keep_sections, state_f64(x, zi, mode));
 }
 
 tsiir_sosfilt_desc make_desc(int64_t channels, int64_t T, int64_t S, at::ScalarType io,
@@ -229,7 +236,8 @@ Fwd sosfilt_forward_hip(const at::Tensor& x, const at::Tensor& sos,
   TORCH_CHECK(sos.device() == x.device(), "tsiir: sos on ", sos.device(), ", x on ", x.device());
   const at::ScalarType io = x.scalar_type();
   const int64_t S = sos.size(1), T = x.size(1);
-  Fwd out = forward_shapes(x, sos, channels, want_zf, keep_sections);
+  const bool f64_state = state_f64(x, zi, mode);
+  Fwd out = forward_shapes(x, sos, channels, want_zf, keep_sections, f64_state);
   if (channels == 0) return out;
   at::Tensor& y = std::get<0>(out);
   at::Tensor& zf = std::get<1>(out);
@@ -237,13 +245,15 @@ Fwd sosfilt_forward_hip(const at::Tensor& x, const at::Tensor& sos,
   uint32_t flags = static_cast<uint32_t>(path_flags) & kCallerFlags;
   if (zi.has_value()) {
     TORCH_CHECK(zi->device() == x.device() && zi->is_contiguous() &&
-                    zi->scalar_type() == acc_type(io),
-                "tsiir: zi must be contiguous, on x's device, in the compute type");
+                    (zi->scalar_type() == acc_type(io) || f64_state),
+                "tsiir: zi must be contiguous, on x's device, in the compute type or in "
+                "double");
     TORCH_CHECK(zi->numel() == (zi_broadcast ? 1 : channels) * S * 2,
                 "tsiir: zi must hold ", zi_broadcast ? 1 : channels, " x ", S, " x 2 values");
     flags |= TSIIR_HAS_ZI | (zi_broadcast ? TSIIR_ZI_BROADCAST : 0u);
   }
   if (want_zf) flags |= TSIIR_WRITE_ZF;
+  if (f64_state) flags |= TSIIR_STATE_F64;
   if (io != at::kDouble && mode == TSIIR_MODE_PLAIN && needs_fp32_df2t(sos))
     flags |= TSIIR_FP32_DF2T;
   const tsiir_sosfilt_desc d = make_desc(channels, T, S, io, sos.scalar_type(), mode, flags,

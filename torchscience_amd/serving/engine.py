@@ -5,24 +5,39 @@ count; its filter state (the zi/zf of the reference's sosfilt,
 src/torchscience/signal_processing/filter/_sosfilt.py:98-133) stays resident on
 the device between chunks, in FP64.  Requests are chunks (channels x samples)
 of a stream.  The engine admits arrived requests while the HBM budget allows,
-groups ready requests of distinct, idle streams into one batch (one kernel
-launch over all their channels, `tsiir_serve_batch`), and keeps up to
-`max_inflight` batches running on separate HIP streams, so requests that
-arrive while others are on the device are launched as soon as slots, budget
-and their stream free up, and their work overlaps on the device.
+groups ready requests of distinct, idle streams into batches and keeps up to
+`max_inflight` batches per lane running on separate HIP streams, so requests
+that arrive while others are on the device are launched as soon as slots,
+budget and their stream free up, and their work overlaps on the device.
+
+Two lanes: chunks up to `long_samples` go to one ragged-batch launch
+(`tsiir_serve_batch`: one wave per channel walks its chunk, no waits between
+items); longer chunks run the window-parallel forward (`tsiir_sosfilt_forward`
+with the stream's FP64 state, TSIIR_STATE_F64), which splits a chunk's
+samples over many waves chained by the look-back, so a long chunk's latency
+does not grow with one wave walking it.
 
 Ordering: a stream's chunks run one at a time, in arrival order (its state
-threads through them).  Each batch item's result depends only on its own
-chunk and state, never on what else is in the batch.
+threads through them).  Each item's result depends only on its own chunk
+and state, never on what else is in the batch.
+
+Failure paths: a request can be cancelled (`cancel`) or given a deadline
+until it is launched (then it is dropped, status "cancelled" / "expired",
+and its stream's state does not advance); long chunks commit their new
+state only when their launch completed without a look-back give-up
+(status "failed" otherwise, state unchanged); a stream's state can be
+evicted to the host and restored.
 
 The executor interface (`launch`, `done`) separates the scheduler from the
-device; `HipExecutor` runs the HIP kernel, and tests drive the scheduler with
-a CPU executor.
+device; `HipExecutor` runs the HIP kernels, and tests drive the scheduler
+with a CPU executor.
 """
 
 from __future__ import annotations
 
 import ctypes
+import gc
+import threading
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -45,6 +60,8 @@ class Request:
     y: torch.Tensor | None = None
     t_launch: float = -1.0
     t_done: float = -1.0
+    deadline: float | None = None  # latest launch time (seconds after the start)
+    status: str = "pending"        # done, failed, cancelled or expired at the end
 
     @property
     def channels(self) -> int:
@@ -79,6 +96,7 @@ class StreamTable:
         self._state: dict[int, torch.Tensor] = {}
         self._sections: dict[int, int] = {}
         self._channels: dict[int, int] = {}
+        self._sos: dict[int, torch.Tensor] = {}
 
     def register(self, stream_id: int, sos, channels: int, zi=None) -> None:
         L = _lib.load()
@@ -98,8 +116,7 @@ class StreamTable:
         self._state[stream_id] = state
         self._sections[stream_id] = S
         self._channels[stream_id] = channels
-        self._sos_keep = getattr(self, "_sos_keep", [])
-        self._sos_keep.append(sos_t)
+        self._sos[stream_id] = sos_t.reshape(1, S, 6)
 
     def sections(self, stream_id: int) -> int:
         return self._sections[stream_id]
@@ -113,6 +130,15 @@ class StreamTable:
     def state(self, stream_id: int) -> torch.Tensor:
         return self._state[stream_id]
 
+    def sos(self, stream_id: int) -> torch.Tensor:
+        """The stream's (1, S, 6) FP64 sections on the device."""
+        return self._sos[stream_id]
+
+    def commit(self, stream_id: int, state: torch.Tensor) -> None:
+        """Make `state` the stream's state (after the chunk that wrote it
+        completed)."""
+        self._state[stream_id] = state
+
     def ready(self) -> None:
         """Wait for the table builds (they run on the registering stream)."""
         torch.cuda.synchronize(self.device)
@@ -122,7 +148,22 @@ class StreamTable:
         return self._state[stream_id].cpu().numpy()
 
     def restore(self, stream_id: int, state) -> None:
-        self._state[stream_id].copy_(torch.as_tensor(state, dtype=torch.float64))
+        """Load a stream's state from a host copy (after evict, or a checkpoint)."""
+        if stream_id not in self._state:
+            raise KeyError(f"stream {stream_id} is not registered")
+        shape = (self._channels[stream_id], self._sections[stream_id], 2)
+        st = torch.as_tensor(state, dtype=torch.float64).reshape(shape)
+        self._state[stream_id] = st.to(self.device).contiguous()
+
+    def evict(self, stream_id: int) -> np.ndarray:
+        """Move a stream's state to the host and release its device copy; the
+        stream stays registered and `restore` brings the state back."""
+        host = self.snapshot(stream_id)
+        self._state[stream_id] = torch.empty((0,), dtype=torch.float64, device=self.device)
+        return host
+
+    def resident(self, stream_id: int) -> bool:
+        return self._state[stream_id].numel() > 0
 
 
 class Executor(Protocol):
@@ -131,15 +172,40 @@ class Executor(Protocol):
     def done(self, handle: object) -> bool: ...
 
 
-class HipExecutor:
-    """Launches one `tsiir_serve_batch` per batch.  Batches go round-robin
-    over the HIP streams (default priority) of their lane: lane 0 for batches
-    of short chunks, lane 1 for batches holding a long chunk, so short
-    requests never queue behind a long one in a hardware queue.
-    precision "fp32c" (default) runs FP64 recurrence updates on FP32 data;
-    "fp32" runs the FP32 recurrence."""
+class _LongWorkspace:
+    """Workspace of the window-parallel forward on one HIP stream, with a
+    pinned error mailbox read when a batch completes."""
 
-    def __init__(self, device: torch.device, n_streams: int = 4, precision: str = "fp32c"):
+    def __init__(self, device: torch.device, stream: torch.cuda.Stream, nbytes: int):
+        L = _lib.load()
+        size = max(int(nbytes), 1 << 20)
+        size = (size + 255) // 256 * 256
+        self.buf = torch.empty(size, dtype=torch.uint8, device=device)
+        self.mailbox = torch.zeros(1, dtype=torch.int32, pin_memory=True)
+        _lib.check(L.tsiir_workspace_init(self.buf.data_ptr(), size, stream.cuda_stream),
+                   "tsiir_workspace_init")
+        _lib.check(L.tsiir_workspace_set_error_mailbox(self.buf.data_ptr(), size,
+                                                       self.mailbox.data_ptr(),
+                                                       stream.cuda_stream),
+                   "tsiir_workspace_set_error_mailbox")
+
+    def error(self) -> int:
+        return int(self.mailbox[0])
+
+
+class HipExecutor:
+    """Runs batches on the device.  Lane 0 (short chunks): one
+    `tsiir_serve_batch` launch per batch, states updated in place.  Lane 1
+    (long chunks): per request, the window-parallel `tsiir_sosfilt_forward`
+    from the stream's FP64 state into a fresh state buffer, committed when the
+    batch completes without a look-back give-up.  Batches go round-robin over
+    the HIP streams (default priority) of their lane, so short requests never
+    queue behind a long one in a hardware queue.  precision "fp32c" (default)
+    runs FP64 recurrence updates on FP32 data; "fp32" the FP32 recurrence."""
+
+    def __init__(self, device: torch.device, n_streams: int = 4, precision: str = "fp32c",
+                 parallel_long: bool = True, parallel_long_max_channels: int = 1024,
+                 timing: bool = False, _debug_give_up: bool = False):
         self.device = device
         n_long = max(1, n_streams // 2)
         n_short = max(1, n_streams - n_long)
@@ -150,15 +216,45 @@ class HipExecutor:
         if precision not in ("fp32", "fp32c"):
             raise ValueError("precision must be 'fp32' or 'fp32c'")
         self.mode = _lib.MODE_COMPENSATED if precision == "fp32c" else _lib.MODE_PLAIN
+        self.parallel_long = parallel_long
+        # a long-chunk batch with more channels than this fills the GPU with
+        # one wave per channel, so it runs as one ragged-batch launch; smaller
+        # ones split every chunk over many waves (latency)
+        self.parallel_long_max_channels = int(parallel_long_max_channels)
+        self._ws: dict[int, _LongWorkspace] = {}
+        # per-launch device time (HIP events on the launch stream) and the
+        # algorithmic bytes it moved (x read + y written), when timing
+        self.timing = timing
+        self.kernel_ms: list[float] = []
+        self.kernel_bytes: list[int] = []
+        self._timed: list[tuple] = []
+        # staging of item arrays: pinned host / device slot pairs reused across
+        # launches (a slot returns to the pool when its batch completes), so
+        # the launch path allocates nothing in steady state
+        self._slots: list[tuple[torch.Tensor, torch.Tensor]] = []
+        self._free: list[int] = []
+        # tests only: every look-back wait of the long lane gives up at once
+        self._debug_flags = _lib.DEBUG_GIVE_UP if _debug_give_up else 0
+
+    def _stream(self, lane: int) -> torch.cuda.Stream:
+        streams = self.lanes[lane]
+        stream = streams[self._next[lane]]
+        self._next[lane] = (self._next[lane] + 1) % len(streams)
+        # the inputs (and fresh outputs) were produced on the caller's stream
+        stream.wait_stream(torch.cuda.current_stream(self.device))
+        return stream
 
     def launch(self, batch: list[Request], table: StreamTable, lane: int = 0):
+        for r in batch:
+            if r.y is None:
+                r.y = torch.empty_like(r.x)
+        if (lane == 1 and self.parallel_long and
+                sum(r.channels for r in batch) <= self.parallel_long_max_channels):
+            return self._launch_long(batch, table)
         C = np.fromiter((r.channels for r in batch), dtype=np.int64, count=len(batch))
         T = np.fromiter((r.samples for r in batch), dtype=np.int64, count=len(batch))
         S = np.fromiter((table.sections(r.stream_id) for r in batch), dtype=np.int64,
                         count=len(batch))
-        for r in batch:
-            if r.y is None:
-                r.y = torch.empty_like(r.x)
         xb = np.fromiter((r.x.data_ptr() for r in batch), dtype=np.uint64, count=len(batch))
         yb = np.fromiter((r.y.data_ptr() for r in batch), dtype=np.uint64, count=len(batch))
         tb = np.fromiter((table.table(r.stream_id).data_ptr() for r in batch), dtype=np.uint64,
@@ -168,7 +264,9 @@ class HipExecutor:
         n = int(C.sum())
         req = np.repeat(np.arange(len(batch)), C)            # request of each item
         ch = np.arange(n) - np.repeat(np.cumsum(C) - C, C)   # channel within its request
-        host = torch.empty(n * ITEM_DTYPE.itemsize, dtype=torch.uint8, pin_memory=True)
+        slot = self._acquire(n * ITEM_DTYPE.itemsize)
+        host_full, dev_full = self._slots[slot]
+        host = host_full[:n * ITEM_DTYPE.itemsize]
         items = host.numpy().view(ITEM_DTYPE)
         items["x"] = xb[req] + (4 * ch * T[req]).astype(np.uint64)
         items["y"] = yb[req] + (4 * ch * T[req]).astype(np.uint64)
@@ -177,17 +275,16 @@ class HipExecutor:
         items["n_samples"] = T[req]
         items["n_sections"] = S[req]
         items["reserved"] = 0
-        streams = self.lanes[lane]
-        stream = streams[self._next[lane]]
-        self._next[lane] = (self._next[lane] + 1) % len(streams)
-        # the inputs (and fresh outputs) were produced on the caller's stream
-        stream.wait_stream(torch.cuda.current_stream(self.device))
+        stream = self._stream(lane)
         with torch.cuda.stream(stream):
-            dev_items = host.to(self.device, non_blocking=True)
+            dev_items = dev_full[:n * ITEM_DTYPE.itemsize]
+            dev_items.copy_(host, non_blocking=True)
+            ev0 = self._start(stream)
             _lib.check(self._lib.tsiir_serve_batch(dev_items.data_ptr(), n, self.mode,
                                                    stream.cuda_stream), "tsiir_serve_batch")
-            ev = torch.cuda.Event()
+            ev = torch.cuda.Event(enable_timing=self.timing)
             ev.record(stream)
+            self._stop(ev0, ev, batch)
         # keep the caching allocator from reusing these blocks before the
         # side stream is done with them
         for r in batch:
@@ -195,11 +292,98 @@ class HipExecutor:
             r.y.record_stream(stream)
             table.state(r.stream_id).record_stream(stream)
             table.table(r.stream_id).record_stream(stream)
-        dev_items.record_stream(stream)
-        return (ev, host, dev_items)
+        return {"event": ev, "slot": slot, "batch": batch, "commit": None, "table": table}
+
+    def _launch_long(self, batch: list[Request], table: StreamTable):
+        stream = self._stream(1)
+        L = self._lib
+        news = []
+        with torch.cuda.stream(stream):
+            ev0 = self._start(stream)
+            for r in batch:
+                C, T, S = r.channels, r.samples, table.sections(r.stream_id)
+                zi = table.state(r.stream_id)
+                zf = torch.empty_like(zi)
+                flags = _lib.HAS_ZI | _lib.WRITE_ZF | _lib.STATE_F64 | self._debug_flags
+                d = _lib.SosfiltDesc(channels=C, n_samples=T, n_sections=S, io_dtype=_lib.F32,
+                                     coef_dtype=_lib.F64, mode=self.mode, flags=flags,
+                                     reserved=0, x_rows=C, x_row_stride=T, y_row_stride=T,
+                                     chans_per_filter=C)
+                nbytes = L.tsiir_sosfilt_workspace_bytes(d)
+                key = stream.cuda_stream
+                ws = self._ws.get(key)
+                if ws is None or ws.buf.numel() < nbytes:
+                    ws = self._ws[key] = _LongWorkspace(self.device, stream, nbytes)
+                _lib.check(L.tsiir_sosfilt_forward(d, r.x.data_ptr(), r.y.data_ptr(),
+                                                   table.sos(r.stream_id).data_ptr(),
+                                                   zi.data_ptr(), zf.data_ptr(),
+                                                   ws.buf.data_ptr(), ws.buf.numel(),
+                                                   stream.cuda_stream),
+                           "tsiir_sosfilt_forward")
+                news.append((r, zf))
+                for t in (r.x, r.y, zi, zf, table.sos(r.stream_id)):
+                    t.record_stream(stream)
+            ev = torch.cuda.Event(enable_timing=self.timing)
+            ev.record(stream)
+            self._stop(ev0, ev, batch)
+        return {"event": ev, "slot": None, "batch": batch, "commit": (self._ws[key], news),
+                "table": table}
+
+    def _acquire(self, nbytes: int) -> int:
+        """A free staging slot of at least nbytes (a new one when none is)."""
+        for k in range(len(self._free) - 1, -1, -1):
+            i = self._free[k]
+            if self._slots[i][0].numel() >= nbytes:
+                del self._free[k]
+                return i
+        size = max(1 << 16, 1 << (max(nbytes, 1) - 1).bit_length())
+        self._slots.append((torch.empty(size, dtype=torch.uint8, pin_memory=True),
+                            torch.empty(size, dtype=torch.uint8, device=self.device)))
+        return len(self._slots) - 1
+
+    def _start(self, stream):
+        if not self.timing:
+            return None
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        return ev0
+
+    def _stop(self, ev0, ev1, batch):
+        if self.timing:
+            self._timed.append((ev0, ev1, sum(8 * r.x.numel() for r in batch)))
+
+    def collect_timing(self) -> None:
+        """Move the finished launches' event times into kernel_ms /
+        kernel_bytes (synchronises)."""
+        torch.cuda.synchronize(self.device)
+        for ev0, ev1, nb in self._timed:
+            self.kernel_ms.append(ev0.elapsed_time(ev1))
+            self.kernel_bytes.append(nb)
+        self._timed.clear()
 
     def done(self, handle) -> bool:
-        return handle[0].query()
+        """True once the batch completed; long-lane batches then commit their
+        new states, or mark their requests failed (states unchanged) when a
+        look-back wait of the batch's stream gave up."""
+        if not handle["event"].query():
+            return False
+        if handle.get("slot") is not None:
+            self._free.append(handle["slot"])
+            handle["slot"] = None
+        commit = handle.get("commit")
+        if commit is not None:
+            ws, news = commit
+            failed = ws.error() != 0
+            for r, zf in news:
+                if failed:
+                    r.status = "failed"
+                else:
+                    handle["table"].commit(r.stream_id, zf)
+            if failed:
+                # the workspace's error word stays set: start a fresh one
+                self._ws = {k: v for k, v in self._ws.items() if v is not ws}
+            handle["commit"] = None
+        return True
 
 
 @dataclass
@@ -247,6 +431,14 @@ class ServingEngine:
         self.long_samples = int(long_samples)
         self.clock = clock
         self.batches: list[list[int]] = []  # request ids per launched batch (for tests)
+        self._cancel: set[int] = set()
+        self._cancel_lock = threading.Lock()
+
+    def cancel(self, rid: int) -> None:
+        """Cancel request `rid` if it has not been launched yet (safe to call
+        from another thread while `run` is going)."""
+        with self._cancel_lock:
+            self._cancel.add(rid)
 
     def validate(self, r: Request) -> None:
         """Reject a request the kernel cannot serve safely: the state slot
@@ -275,8 +467,23 @@ class ServingEngine:
         if r.y is not None and r.y.shape != r.x.shape:
             raise ValueError(f"request {r.rid}: y {tuple(r.y.shape)} does not match x "
                              f"{tuple(r.x.shape)}")
+        if hasattr(self.table, "resident") and not self.table.resident(r.stream_id):
+            raise ValueError(f"request {r.rid}: stream {r.stream_id} is evicted; restore it "
+                             "first")
 
     def run(self, requests: list[Request]) -> ServeStats:
+        """Serve `requests` (arrival times in seconds after the start) to
+        completion.  Python's cyclic garbage collector is paused meanwhile,
+        so that none of its pauses lands in a request's latency."""
+        was = gc.isenabled()
+        gc.disable()
+        try:
+            return self._run(requests)
+        finally:
+            if was:
+                gc.enable()
+
+    def _run(self, requests: list[Request]) -> ServeStats:
         reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
         for r in reqs:
             self.validate(r)
@@ -298,6 +505,19 @@ class ServingEngine:
             while i < n and reqs[i].arrival <= now:
                 ready.append(reqs[i])
                 i += 1
+            # requests cancelled, or past their deadline, before their launch
+            with self._cancel_lock:
+                cancel = set(self._cancel)
+            if cancel or any(r.deadline is not None for r in ready):
+                keep = deque()
+                for r in ready:
+                    if r.rid in cancel:
+                        r.status, r.t_done = "cancelled", now
+                    elif r.deadline is not None and now > r.deadline:
+                        r.status, r.t_done = "expired", now
+                    else:
+                        keep.append(r)
+                ready = keep
             still = []
             for rec in inflight:
                 h, batch, nb, _ = rec
@@ -305,6 +525,8 @@ class ServingEngine:
                     tdone = self.clock() - t0
                     for r in batch:
                         r.t_done = tdone
+                        if r.status == "pending":
+                            r.status = "done"
                         busy.discard(r.stream_id)
                     used -= nb
                 else:
@@ -352,9 +574,11 @@ class ServingEngine:
                 if wait > 2e-4:
                     time.sleep(wait - 1e-4)
         wall = self.clock() - t0
-        lat = np.array([r.latency for r in reqs]) if reqs else np.zeros(1)
-        ss = float(sum(r.channels * r.samples * self.table.sections(r.stream_id) for r in reqs))
-        smp = float(sum(r.channels * r.samples for r in reqs))
+        served = [r for r in reqs if r.status == "done"]
+        lat = np.array([r.latency for r in served]) if served else np.zeros(1)
+        ss = float(sum(r.channels * r.samples * self.table.sections(r.stream_id)
+                       for r in served))
+        smp = float(sum(r.channels * r.samples for r in served))
         nbat = len(self.batches)
         return ServeStats(n, wall, ss, smp, lat, nbat, n / max(nbat, 1))
 
@@ -364,10 +588,12 @@ def run_plain(requests: list[Request], sos_by_stream: dict, table_channels: dict
               precision: str = "fp32c") -> ServeStats:
     """Per-request plain calls of the drop-in sosfilt (one call per request
     in arrival order, each from the stream's current zi, waiting for its
-    result), the baseline the engine is compared with."""
+    result), the baseline the engine is compared with.  With "fp32c" the
+    state is carried in float64 between chunks, as the engine carries it."""
     from ..signal_processing.filter import sosfilt
     reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
-    zi = {sid: torch.zeros((c, sos_by_stream[sid].shape[0], 2), dtype=torch.float32,
+    sdt = torch.float64 if precision == "fp32c" else torch.float32
+    zi = {sid: torch.zeros((c, sos_by_stream[sid].shape[0], 2), dtype=sdt,
                            device=device) for sid, c in table_channels.items()}
     cdt = torch.float64 if precision == "fp32c" else torch.float32
     sos_dev = {sid: torch.as_tensor(s, dtype=cdt).to(device) for sid, s in sos_by_stream.items()}
@@ -382,6 +608,7 @@ def run_plain(requests: list[Request], sos_by_stream: dict, table_channels: dict
         torch.cuda.current_stream(device).synchronize()
         r.y = y
         r.t_done = clock() - t0
+        r.status = "done"
     wall = clock() - t0
     lat = np.array([r.latency for r in reqs])
     ss = float(sum(r.channels * r.samples * sos_by_stream[r.stream_id].shape[0] for r in reqs))

@@ -88,7 +88,11 @@ def sosfilt(
     xr = _to_rows(x, plan.axis, B, T).to(plan.io_dtype).contiguous()
     zr = None
     if zi is not None:
-        zr = zi.to(device=dev, dtype=comp)
+        # the compensated forms carry a float64 zi as it is (zf then keeps
+        # float64, as zf has zi's dtype); otherwise the state is in the
+        # compute type
+        sdt = torch.float64 if (plan.compensated and zi.dtype == torch.float64) else comp
+        zr = zi.to(device=dev, dtype=sdt)
         zr = zr.reshape(1, S, 2) if plan.zi_broadcast else zr.reshape(B, S, 2)
         zr = zr.contiguous()
     y, zf = _run(sos_c, xr, zr, B, comp, plan.io_dtype, zi is not None, plan.zi_broadcast,
