@@ -333,8 +333,8 @@ def _fft_call(h, x, *, channels, shift, n_out, reverse=False, ws_bytes=None):
 
 @pytest.mark.parametrize("K", [1, 2, 63, 101, 1000, 2048])
 def test_fir_fft_entry_matches_oracle(oracle_mod, K):
-    """Every output block boundary, odd channel counts (the last channel
-    alone in its complex pair), fewer input rows than channels, output
+    """Every output block boundary, odd block counts (the last block alone
+    in its complex window), fewer input rows than channels, output
     longer or shorter than the input, 'same' and reversed-tap shifts; FP64
     at 1e-10 and FP32 at 1e-5 of the largest output."""
     from torchscience_amd import _lib
@@ -460,3 +460,42 @@ def test_fir_fft_per_filter_matches_oracle(oracle_mod, cpf):
                                 filter_format="fir", broadcast=False)
     refb, _ = oracle_mod.fir_rows(b, xs)     # lfilter(b[i], [1], x[i]) per row
     assert yb.shape == refb.shape and relerr(yb.cpu().numpy(), refb) < 1e-10
+
+
+def test_fir_fft_channels_stay_apart():
+    """The FFT kernel transforms two blocks of ONE channel per complex
+    window, so channels never share rounding or non-finite values: rows at
+    scales 1e6, 1, 1e-3 each match the direct kernel per channel at the
+    FP32 tolerance of their own scale, and a NaN in one channel leaves every
+    other channel finite and reaches only the outputs of its own channel's
+    window (two blocks)."""
+    from torchscience_amd import _lib
+    from torchscience_amd.signal_processing.filter import _fir
+    from torchscience_amd.signal_processing.filter._fir import fir_rows_forward
+    import scipy.signal as ss
+    dev = _dev()
+    K = 101
+    Lb = 4096 - K + 1
+    h = torch.tensor(ss.firwin(K, 0.2), dtype=torch.float32, device=dev)
+    x = torch.randn(4, 8 * Lb, dtype=torch.float32, device=dev)
+    x *= torch.tensor([1e6, 1.0, 1e-3, 1.0], device=dev)[:, None]
+    st, y = _fft_call(h, x, channels=4, shift=0, n_out=x.shape[1])
+    assert st == _lib.OK
+    old = _fir.FFT_MIN_TAPS
+    try:
+        _fir.FFT_MIN_TAPS = 1 << 30
+        y_dir, _ = fir_rows_forward(h.reshape(1, -1), x, None, channels=4, chans_per_filter=4)
+    finally:
+        _fir.FFT_MIN_TAPS = old
+    for c in range(4):
+        scale = y_dir[c].abs().max().item()
+        assert (y[c] - y_dir[c]).abs().max().item() < F32_TOL * scale, c
+    bad = 3 * Lb + 100                       # in block 3 (window of blocks 2 and 3)
+    x[0, bad] = float("nan")
+    st, y = _fft_call(h, x, channels=4, shift=0, n_out=x.shape[1])
+    assert st == _lib.OK
+    assert torch.isfinite(y[1:]).all()
+    nan_at = torch.isnan(y[0]).nonzero().flatten()
+    assert nan_at.numel() > 0
+    assert nan_at.min().item() >= 2 * Lb and nan_at.max().item() < 4 * Lb
+    assert torch.isfinite(y[0, :2 * Lb]).all() and torch.isfinite(y[0, 4 * Lb:]).all()

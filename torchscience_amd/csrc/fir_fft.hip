@@ -7,12 +7,15 @@
 // (overlap-save) with a 4096-point transform held in LDS, so that the cost
 // per output no longer grows with K.
 //
-// Work decomposition: one workgroup of 256 lanes filters one block of
-// L = 4096 - K + 1 consecutive outputs of TWO channels of one filter at
-// once: the first is the real part and the second the imaginary part of one
-// complex input window (a filter with an odd channel count leaves its last
-// channel alone, with a zero imaginary part).  The taps are real, so the circular convolution keeps the
-// two parts apart and no post-processing of a real transform is needed.
+// Work decomposition: one workgroup of 256 lanes filters TWO consecutive
+// blocks of L = 4096 - K + 1 outputs of one channel at once: the first block
+// is the real part and the second the imaginary part of one complex input
+// window (a channel with an odd block count leaves its last block alone,
+// with a zero imaginary part).  The taps are real, so the circular
+// convolution keeps the two parts apart and no post-processing of a real
+// transform is needed.  Both parts belong to one channel, so rounding
+// shared through the transform, and a NaN or Inf in the input, never reach
+// another channel (they reach at most the two blocks of the window).
 //
 // The transform: six radix-4 stages, decimation in frequency (natural order
 // in, base-4 digit-reversed order out), fused in pairs into three radix-16
@@ -230,9 +233,9 @@ struct FftArgs {
   int64_t x_stride;
   int64_t y_stride;
   int64_t shift;
-  int64_t blocks;  // output blocks per channel pair
+  int64_t blocks;  // output blocks per channel
+  int64_t pairs;   // block pairs per channel, ceil(blocks / 2)
   int64_t cpf;     // channels per filter (<= channels)
-  int64_t ppf;     // channel pairs per filter
   int K;
   int L;           // outputs per block, N - K + 1
 };
@@ -242,16 +245,13 @@ __global__ __launch_bounds__(256) void fir_fft_kernel(FftArgs<T> a) {
   using C = typename CxOf<T>::type;
   __shared__ C lds[kFftLds];
   const int t = threadIdx.x;
-  const int64_t pair = blockIdx.x / a.blocks;
-  const int64_t blk = blockIdx.x - pair * a.blocks;
-  const int64_t f = pair / a.ppf;
-  const int64_t c_in = 2 * (pair - f * a.ppf);
-  const int64_t c0 = f * a.cpf + c_in;
-  const bool has1 = c_in + 1 < a.cpf;
+  const int64_t c0 = blockIdx.x / a.pairs;         // channel
+  const int64_t bp = blockIdx.x - c0 * a.pairs;     // its block pair
+  const int64_t f = c0 / a.cpf;
+  const bool has1 = 2 * bp + 1 < a.blocks;
   const C* __restrict__ H = a.H + f * kFftN;
   const T* __restrict__ x0 = a.x + (c0 % a.x_rows) * a.x_stride;
-  const T* __restrict__ x1 = has1 ? a.x + ((c0 + 1) % a.x_rows) * a.x_stride : x0;
-  const int64_t n0 = blk * a.L;
+  const int64_t n0 = 2 * bp * a.L;                  // first output of the real block
   const int64_t src0 = n0 + a.shift - (a.K - 1);
   // the first pass's points t + m1 * 1024 + m2 * 256 are coalesced in
   // memory: load them straight into registers
@@ -261,8 +261,10 @@ __global__ __launch_bounds__(256) void fir_fft_kernel(FftArgs<T> a) {
 #pragma unroll
     for (int m2 = 0; m2 < 4; ++m2) {
       const int64_t s = src0 + point<1024>(t, m1, m2);
+      const int64_t s1 = s + a.L;                 // the same point of the next block
       const bool in = s >= 0 && s < a.x_len;
-      v[m1][m2] = C{in ? x0[s] : T(0), (in && has1) ? x1[s] : T(0)};
+      const bool in1 = has1 && s1 >= 0 && s1 < a.x_len;
+      v[m1][m2] = C{in ? x0[s] : T(0), in1 ? x0[s1] : T(0)};
     }
   fwd16<1024>(v, a.tw, t);
   lds_store<1024>(lds, t, v);
@@ -289,16 +291,15 @@ __global__ __launch_bounds__(256) void fir_fft_kernel(FftArgs<T> a) {
   inv16<1024>(v, a.tw, t);
   // circular outputs i >= K - 1 are the linear ones: y[n0 + i - (K - 1)]
   T* __restrict__ y0 = a.y + c0 * a.y_stride;
-  T* __restrict__ y1 = y0 + a.y_stride;
 #pragma unroll
   for (int m1 = 0; m1 < 4; ++m1)
 #pragma unroll
     for (int m2 = 0; m2 < 4; ++m2) {
       const int i = point<1024>(t, m1, m2);
       const int64_t n = n0 + i - (a.K - 1);
-      if (i >= a.K - 1 && n < a.T_out) {
-        y0[n] = v[m1][m2].x;
-        if (has1) y1[n] = v[m1][m2].y;
+      if (i >= a.K - 1) {
+        if (n < a.T_out) y0[n] = v[m1][m2].x;
+        if (has1 && n + a.L < a.T_out) y0[n + a.L] = v[m1][m2].y;
       }
     }
 }
@@ -346,10 +347,10 @@ tsiir_status launch_fft(const tsiir_fir_desc* d, const void* taps, const void* x
   a.K = d->n_taps;
   a.L = kFftN - d->n_taps + 1;
   a.blocks = (d->n_samples + a.L - 1) / a.L;
+  a.pairs = (a.blocks + 1) / 2;
   const int64_t nf = fft_filters(d);
   a.cpf = d->channels / nf;
-  a.ppf = (a.cpf + 1) / 2;
-  const int64_t grid = nf * a.ppf * a.blocks;
+  const int64_t grid = d->channels * a.pairs;
   if (grid > 0x7fffffff || nf > 0x7fffffff) return TSIIR_EUNSUPPORTED;
   hipLaunchKernelGGL(fft_twiddle_kernel<T>, dim3((kFftTw + 255) / 256), dim3(256), 0, st, tw);
   if (hipGetLastError() != hipSuccess) return TSIIR_ELAUNCH;
