@@ -42,13 +42,15 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs scaled by the same counters over a 1 GiB copy
 # (profiles/sos_fwd32_c2a_pmc.txt)
-TRAFFIC_BYTES = {"c2a": 2156240896}
+# (profiles/sos_fwd32_c2a_pmc.txt, profiles/sos_train_c3_pmc.txt: the C3
+# figure is the forward and adjoint kernels of one training step)
+TRAFFIC_BYTES = {"c2a": 2156240896, "c3": 814700000}
 
 
 # The first launches after the GPU leaves idle run at full speed, then the
 # power management slows them for a while (C2a: dispatches 5-25 at
-# 0.50-0.54 ms against 0.425 ms before and after; profiles/
-# sos_fwd32_c2a_warmup_trace.txt).  Warm-up therefore runs at least W steps
+# 0.49-0.60 ms against 0.41-0.43 ms before and after; profiles/
+# sos_fwd32_c2a_kernel_trace.txt).  Warm-up therefore runs at least W steps
 # and at least WARMUP_MIN_S seconds of them, so the K timed steps see the
 # settled clocks; the JSON reports how many warm-up steps ran.
 WARMUP_MIN_S = 0.6
