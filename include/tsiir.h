@@ -51,8 +51,9 @@ typedef enum tsiir_status {
   TSIIR_EUNSUPPORTED = 2,  /* valid request this build does not implement */
   TSIIR_ELAUNCH = 3,       /* a kernel launch failed */
   TSIIR_EWORKSPACE = 4,    /* workspace missing or too small */
-  TSIIR_ESCRATCH = 5       /* needs more than the fixed scratch of
+  TSIIR_ESCRATCH = 5,      /* needs more than the fixed scratch of
                               tsiir_sosfilt_forward_noalloc */
+  TSIIR_EAGAIN = 6         /* serving queue full: poll, then submit again */
 } tsiir_status;
 
 typedef enum tsiir_dtype {
@@ -394,6 +395,45 @@ tsiir_status tsiir_serve_prepare_filter(const void* sos, int32_t coef_dtype, int
  * (FP32 recurrence) or TSIIR_MODE_COMPENSATED (FP32 storage, FP64 updates). */
 tsiir_status tsiir_serve_batch(const tsiir_serve_item* items, int32_t n_items, int32_t mode,
                                void* stream);
+
+/* ---- persistent serving launch -----------------------------------------
+ * A queue of items (tsiir_serve_item) served by a persistent launch: a fixed
+ * grid of waves that keeps taking submitted items while it runs, so a
+ * serving loop submits chunks as they arrive instead of launching a kernel
+ * per batch.  The queue owns a HIP stream, a ring of `capacity` item slots
+ * and completion words in host memory the device reads and writes
+ * coherently, and a device claim counter: tsiir_serve_create allocates them,
+ * tsiir_serve_destroy frees them.  Each item is walked by one wave from its
+ * FP64 state, which the item overwrites with the final state (as
+ * tsiir_serve_batch); an item's result does not depend on the other items.
+ * The launch drains by itself after idle_us microseconds without work and is
+ * launched again by the next submit or poll that finds work waiting.
+ * Submitted items must have their input ready (the caller synchronises the
+ * work that produced it) and must not share a state slot with an item still
+ * in flight.  One queue per thread, or the caller serialises the calls.
+ * The reference has no serving layer; this serves the streaming use of
+ * sosfilt with zi / zf (_sosfilt.py:98-133). */
+typedef struct tsiir_serve_queue tsiir_serve_queue;
+
+/* capacity: item slots (tickets in flight at once); workgroups: waves of the
+ * persistent launch; mode: TSIIR_MODE_PLAIN or TSIIR_MODE_COMPENSATED. */
+tsiir_status tsiir_serve_create(int32_t capacity, int32_t workgroups, int32_t mode,
+                                int32_t idle_us, tsiir_serve_queue** out);
+
+/* Append n_items items (host array, copied); *first_ticket gets the ticket of
+ * the first (tickets are consecutive).  TSIIR_EAGAIN when the slots they
+ * need still hold unfinished items (nothing is submitted then). */
+tsiir_status tsiir_serve_submit(tsiir_serve_queue* queue, const tsiir_serve_item* items,
+                                int32_t n_items, uint64_t* first_ticket);
+
+/* *all_done = 1 when every ticket of [first_ticket, first_ticket + n_items)
+ * has completed (outputs and states written and visible), else 0.  Never
+ * blocks. */
+tsiir_status tsiir_serve_poll(tsiir_serve_queue* queue, uint64_t first_ticket, int32_t n_items,
+                              int32_t* all_done);
+
+/* Lets the launch finish the submitted items, waits for it, frees the queue. */
+tsiir_status tsiir_serve_destroy(tsiir_serve_queue* queue);
 
 #ifdef __cplusplus
 }

@@ -16,7 +16,7 @@ import torch
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "_lib", "libtsiir.so")
 
-OK, EINVAL, EUNSUPPORTED, ELAUNCH, EWORKSPACE, ESCRATCH = range(6)
+OK, EINVAL, EUNSUPPORTED, ELAUNCH, EWORKSPACE, ESCRATCH, EAGAIN = range(7)
 NOALLOC_SCRATCH_BYTES = 64 << 20
 F32, F64, BF16 = 0, 1, 2
 MODE_PLAIN, MODE_COMPENSATED = 0, 1
@@ -57,6 +57,10 @@ EXPORTED = (
     "tsiir_serve_table_bytes",
     "tsiir_serve_prepare_filter",
     "tsiir_serve_batch",
+    "tsiir_serve_create",
+    "tsiir_serve_submit",
+    "tsiir_serve_poll",
+    "tsiir_serve_destroy",
 )
 
 
@@ -214,6 +218,16 @@ def load(path: str | None = None) -> ctypes.CDLL:
     L.tsiir_serve_prepare_filter.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp, vp]
     L.tsiir_serve_batch.restype = ctypes.c_int
     L.tsiir_serve_batch.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp]
+    L.tsiir_serve_create.restype = ctypes.c_int
+    L.tsiir_serve_create.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                     ctypes.c_int32, ctypes.POINTER(vp)]
+    L.tsiir_serve_submit.restype = ctypes.c_int
+    L.tsiir_serve_submit.argtypes = [vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64)]
+    L.tsiir_serve_poll.restype = ctypes.c_int
+    L.tsiir_serve_poll.argtypes = [vp, ctypes.c_uint64, ctypes.c_int32,
+                                   ctypes.POINTER(ctypes.c_int32)]
+    L.tsiir_serve_destroy.restype = ctypes.c_int
+    L.tsiir_serve_destroy.argtypes = [vp]
     if path is None:
         _lib = L
     return L

@@ -897,6 +897,7 @@ const char* tsiir_status_string(tsiir_status s) {
     case TSIIR_EUNSUPPORTED: return "unsupported request";
     case TSIIR_ELAUNCH: return "kernel launch failed";
     case TSIIR_EWORKSPACE: return "workspace missing or too small";
+    case TSIIR_EAGAIN: return "serving queue full: poll and submit again";
     case TSIIR_ESCRATCH:
       return "request needs more workspace than the fixed scratch of the no-allocation entry point";
   }

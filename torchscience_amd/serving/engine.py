@@ -386,6 +386,89 @@ class HipExecutor:
         return True
 
 
+class PersistentExecutor(HipExecutor):
+    """HipExecutor whose ragged batches go to a persistent launch through a
+    device queue (tsiir_serve_create / submit / poll / destroy): one grid of
+    waves keeps taking the submitted chunk channels while it runs, so a
+    batch costs a host-side submit instead of a kernel launch.  Long chunks
+    of small batches still run the window-parallel forward.  Inputs must be
+    complete when submitted: the caller's stream is waited for (host side)
+    when it still has work queued."""
+
+    def __init__(self, device: torch.device, n_streams: int = 4, precision: str = "fp32c",
+                 capacity: int = 1 << 18, workgroups: int = 2048, idle_us: int = 20000, **kw):
+        super().__init__(device, n_streams, precision, **kw)
+        q = ctypes.c_void_p()
+        with torch.cuda.device(device):
+            _lib.check(self._lib.tsiir_serve_create(capacity, workgroups, self.mode, idle_us,
+                                                    ctypes.byref(q)), "tsiir_serve_create")
+        self._q = q
+
+    def launch(self, batch: list[Request], table: StreamTable, lane: int = 0):
+        if (lane == 1 and self.parallel_long and
+                sum(r.channels for r in batch) <= self.parallel_long_max_channels):
+            return super().launch(batch, table, lane)
+        for r in batch:
+            if r.y is None:
+                r.y = torch.empty_like(r.x)
+        cur = torch.cuda.current_stream(self.device)
+        if not cur.query():
+            cur.synchronize()
+        C = np.fromiter((r.channels for r in batch), dtype=np.int64, count=len(batch))
+        T = np.fromiter((r.samples for r in batch), dtype=np.int64, count=len(batch))
+        S = np.fromiter((table.sections(r.stream_id) for r in batch), dtype=np.int64,
+                        count=len(batch))
+        n = int(C.sum())
+        req = np.repeat(np.arange(len(batch)), C)
+        ch = np.arange(n) - np.repeat(np.cumsum(C) - C, C)
+        items = np.empty(n, dtype=ITEM_DTYPE)
+        xb = np.fromiter((r.x.data_ptr() for r in batch), dtype=np.uint64, count=len(batch))
+        yb = np.fromiter((r.y.data_ptr() for r in batch), dtype=np.uint64, count=len(batch))
+        tb = np.fromiter((table.table(r.stream_id).data_ptr() for r in batch), dtype=np.uint64,
+                         count=len(batch))
+        sb = np.fromiter((table.state(r.stream_id).data_ptr() for r in batch), dtype=np.uint64,
+                         count=len(batch))
+        items["x"] = xb[req] + (4 * ch * T[req]).astype(np.uint64)
+        items["y"] = yb[req] + (4 * ch * T[req]).astype(np.uint64)
+        items["table"] = tb[req]
+        items["state"] = sb[req] + (16 * S[req] * ch).astype(np.uint64)
+        items["n_samples"] = T[req]
+        items["n_sections"] = S[req]
+        items["reserved"] = 0
+        first = ctypes.c_uint64()
+        t0 = time.perf_counter()
+        while True:
+            rc = self._lib.tsiir_serve_submit(self._q, items.ctypes.data, n, ctypes.byref(first))
+            if rc != _lib.EAGAIN or time.perf_counter() - t0 > 30.0:
+                break
+            # a slot this batch needs still holds a running item: let it finish
+            time.sleep(5e-5)
+        _lib.check(rc, "tsiir_serve_submit")
+        return {"queue": (first.value, n), "slot": None, "batch": batch, "commit": None,
+                "table": table}
+
+    def done(self, handle) -> bool:
+        if "queue" not in handle:
+            return super().done(handle)
+        first, n = handle["queue"]
+        flag = ctypes.c_int32()
+        _lib.check(self._lib.tsiir_serve_poll(self._q, first, n, ctypes.byref(flag)),
+                   "tsiir_serve_poll")
+        return bool(flag.value)
+
+    def close(self) -> None:
+        """Finish the submitted chunks and release the queue."""
+        if self._q:
+            _lib.check(self._lib.tsiir_serve_destroy(self._q), "tsiir_serve_destroy")
+            self._q = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 @dataclass
 class ServeStats:
     n_requests: int
