@@ -77,7 +77,8 @@ def parse():
     p.add_argument("--steps", type=int, default=200)
     p.add_argument("--warmup", type=int, default=50)
     p.add_argument("--workload", default="c2a",
-                   choices=["c2a", "c2b", "c2c", "c5", "c5c", "c5bf16", "c3", "c4", "fir",
+                   choices=["c2a", "c2b", "c2c", "c5", "c5c", "c5bf16", "c5bf16c", "c3", "c4",
+                            "fir",
                             "fir64", "fir2k", "fir2k64", "iir8", "design"])
     p.add_argument("--requests", type=int, default=10000)
     p.add_argument("--streams", type=int, default=2000)
@@ -114,12 +115,13 @@ def make_workload(name, dev):
         sos = butterworth_sos(6, [0.05, 0.3], "bandpass")
         desc = "long-sequence sosfilt fp32: x[8,2^24] N(0,1), 6-section bandpass [0.05,0.3]"
         cpf = B
-    elif name in ("c5", "c5c", "c5bf16"):
+    elif name in ("c5", "c5c", "c5bf16", "c5bf16c"):
         B, T = 2048, 1 << 20
         g.manual_seed(5)
         sos = chebyshev1_sos(16, 0.01, 1.0)
-        form = {"c5": "fp32 (FP64 carries)", "c5c": "fp32c (FP64 recurrence updates)",
-                "c5bf16": "bf16 I/O fp32 arithmetic"}[name]
+        form = {"c5": "fp32 (DF2T, FP64 carries)", "c5c": "fp32c (FP64 recurrence updates)",
+                "c5bf16": "bf16 I/O fp32 arithmetic (DF2T, FP64 carries)",
+                "c5bf16c": "bf16c (bf16 I/O, FP64 recurrence updates)"}[name]
         desc = ("precision stress sosfilt: x[2048,2^20], order-16 Chebyshev I 1 dB cutoff 0.01 "
                 "(8 sections), " + form)
         cpf = B
@@ -132,7 +134,7 @@ def make_workload(name, dev):
                 "against 3 sines (f ~ U[0.01,0.4] of Nyquist) + 0.1 N(0,1); grads of x, sos, zi")
         cpf = B
     x = torch.randn(B, T, device=dev, generator=g, dtype=torch.float32)
-    if name == "c5bf16":
+    if name in ("c5bf16", "c5bf16c"):
         x = x.bfloat16()
     return dict(B=B, T=T, sos=sos, x=x, desc=desc, cpf=cpf)
 
@@ -226,7 +228,8 @@ def accuracy(y, x, sos_np, coef_dtype, n):
 
 
 # channels whose output is checked against the FP64 oracle after timing
-ACCURACY_CHANNELS = {"c2a": 256, "c2b": 256, "c2c": 8, "c5": 64, "c5c": 64, "c5bf16": 64}
+ACCURACY_CHANNELS = {"c2a": 256, "c2b": 256, "c2c": 8, "c5": 64, "c5c": 64, "c5bf16": 64,
+                     "c5bf16c": 64}
 
 
 # FP32 / FP64 vector FMA rate of MI355X without packed FP32 (all HIP code
@@ -656,8 +659,8 @@ def main():
     x = w["x"]
     per_channel = sos_np.ndim == 3
     # the compensated form takes FP64 coefficients; the others FP32
-    sos = torch.tensor(sos_np, dtype=torch.float64 if args.workload == "c5c" else torch.float32,
-                       device=dev)
+    sos = torch.tensor(sos_np, dtype=torch.float64 if args.workload in ("c5c", "c5bf16c")
+                       else torch.float32, device=dev)
     io_bytes = 2 if x.dtype == torch.bfloat16 else 4
 
     if args.workload == "c3":
@@ -685,8 +688,7 @@ def main():
             return batched_filter_apply(sos, x, broadcast=False)
         algo_bytes = B * T * io_bytes * 2 + sos.numel() * sos.element_size()
     else:
-        prec = "bf16io" if x.dtype == torch.bfloat16 else ("fp32c" if args.workload == "c5c"
-                                                            else None)
+        prec = {"c5c": "fp32c", "c5bf16": "bf16io", "c5bf16c": "bf16c"}.get(args.workload)
 
         def step():
             return sosfilt(sos, x, precision=prec)
