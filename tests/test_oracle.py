@@ -402,6 +402,80 @@ def test_iir_design_matches_scipy(oracle_mod, family, param):
                 pool.pop(i)
 
 
+def _rule_sections(z, p, k):
+    """Sections from a digital zpk whose zeros are all real, by the rules the
+    reference's pairing states (_zpk_to_sos.py:66-154): one section per
+    pole of positive imaginary part, in list order, then the real poles two
+    at a time, then a lone real pole; each section takes the next two real
+    zeros in list order (one for the lone pole); the gain is split evenly,
+    |k|^(1/S) per section, with only the first b0 negated when k < 0."""
+    zr = list(np.real(z))
+    rows = []
+    for pp in p[p.imag > 1e-9]:
+        z1, z2 = zr.pop(0), zr.pop(0)
+        rows.append([1.0, -(z1 + z2), z1 * z2, 1.0, -2.0 * pp.real, abs(pp) ** 2])
+    real_p = list(np.real(p[np.abs(p.imag) <= 1e-9]))
+    while len(real_p) >= 2:
+        p1, p2 = real_p.pop(0), real_p.pop(0)
+        z1, z2 = zr.pop(0), zr.pop(0)
+        rows.append([1.0, -(z1 + z2), z1 * z2, 1.0, -(p1 + p2), p1 * p2])
+    if real_p:
+        rows.append([1.0, -zr.pop(0), 0.0, 1.0, -real_p[0], 0.0])
+    sos = np.array(rows)
+    sos[:, :3] *= abs(k) ** (1.0 / len(rows))
+    if k < 0:
+        sos[0, 0] = -sos[0, 0]
+    return sos
+
+
+@pytest.mark.parametrize("family,param", [("butterworth", 0.0), ("chebyshev1", 1.0)])
+def test_iir_design_section_order_and_gain_split(oracle_mod, family, param):
+    """The design restatement's section rows, in order, against sections
+    built from scipy's own prototype, band-map and bilinear functions
+    (buttap / cheb1ap, lp2lp_zpk / lp2hp_zpk / lp2bp_zpk, bilinear_zpk at
+    fs = 2 after pre-warping), whose pole and zero lists come in the same
+    order as the reference's formulas (_butterworth_prototype.py:84-91,
+    _chebyshev_type_1_prototype.py:100-108, _lowpass_to_bandpass_zpk.py,
+    zeros appended at -1 by the bilinear map).  Every zero of these designs
+    is real (+1 or -1), so the pairing needs no zero matching and the rows
+    follow from the pole order and the even gain split alone.  scipy's own
+    zpk2sos (nearest pairing, highest-Q section last) and a gain kept whole
+    on the first section both fail this check."""
+    import scipy.signal as ss
+    proto = (lambda N: ss.buttap(N)) if family == "butterworth" else \
+        (lambda N: ss.cheb1ap(N, param))
+    multi = differs = 0
+    for bt, c in [("lowpass", 0.3), ("highpass", 0.15), ("bandpass", [0.2, 0.5]),
+                  ("lowpass", 0.02), ("bandpass", [0.05, 0.3])]:
+        for N in range(1, 13):
+            cc = np.atleast_1d(np.asarray(c, dtype=np.float64))
+            w = 4.0 * np.tan(np.pi * cc / 2.0)
+            z, p, k = proto(N)
+            if bt == "lowpass":
+                z, p, k = ss.lp2lp_zpk(z, p, k, w[0])
+            elif bt == "highpass":
+                z, p, k = ss.lp2hp_zpk(z, p, k, w[0])
+            else:
+                z, p, k = ss.lp2bp_zpk(z, p, k, np.sqrt(w[0] * w[1]), w[1] - w[0])
+            z, p, k = ss.bilinear_zpk(z, p, k, 2.0)
+            want = _rule_sections(z, p, k)
+            got = oracle_mod.iir_design_sos(family, N, c, bt, param)
+            np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-12,
+                                       err_msg=f"{family} {bt} {c} N={N}")
+            S = want.shape[0]
+            if S >= 2:
+                # the planted alternatives differ from the rows the rules give
+                multi += 1
+                near = ss.zpk2sos(z, p, k)
+                differs += int(np.max(np.abs(near[:, 3:] - want[:, 3:])) > 1e-6)
+                whole = want.copy()
+                whole[:, :3] /= abs(k) ** (1.0 / S)
+                whole[0, :3] *= abs(k)
+                assert np.max(np.abs(whole - want)) > 1e-6
+    # nearest pairing orders the sections differently in most of these designs
+    assert differs >= 0.75 * multi, (differs, multi)
+
+
 def test_lfiltic_matches_scipy_and_continues(oracle_mod):
     """lfiltic against scipy.signal.lfiltic (same convention: most recent
     first, zero-padded histories; a[0] = 1), and the continuation it exists
