@@ -1,0 +1,201 @@
+"""Complex filter coefficients on the HIP path (GPU): lfilter and sosfilt
+against the FP64 oracle's complex DF2T (itself pinned against scipy.signal)
+and the reference's own complex-coefficient case (test__lfilter.py:416-428).
+
+The complex filter runs as four real filters on the conjugate-extended
+polynomials (numerator B A~, denominator A A~), so FP64 results agree with
+the serial complex DF2T to rounding, not bitwise: 1e-9 of each row's
+maximum (the time-parallel form of the real filters carries its chunk
+states through an FP64 scan); complex64 within 1e-5 of the complex128
+oracle on the same (complex64-rounded) coefficients.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch.device("cuda:0")
+
+
+def _filt():
+    from torchscience_amd.signal_processing import filter as F
+    return F
+
+
+def _relerr(y, ref):
+    y = np.asarray(y).reshape(-1, np.shape(y)[-1])
+    ref = np.asarray(ref).reshape(y.shape)
+    num = np.max(np.abs(y - ref), axis=-1)
+    den = np.maximum(np.max(np.abs(ref), axis=-1), 1e-300)
+    return float(np.max(num / den))
+
+
+def _stable_a(rng, na, radius=0.9):
+    a = 0.3 * (rng.standard_normal(na) + 1j * rng.standard_normal(na))
+    a[0] = 1.0
+    if na > 1:
+        r = np.max(np.abs(np.roots(a)))
+        if r > radius:
+            a[1:] = a[1:] * (radius / r) ** np.arange(1, na)
+    return a * (1.1 - 0.3j)          # a0 != 1: the normalisation runs
+
+
+def test_reference_complex_coefficient_case():
+    """test__lfilter.py:416-428: complex b and a, real float64 x, against
+    scipy.signal.lfilter at rtol = atol = 1e-10."""
+    import scipy.signal as ss
+    F, dev = _filt(), _dev()
+    b = torch.tensor([0.5 + 0.1j, 0.5 - 0.1j], dtype=torch.complex128)
+    a = torch.tensor([1.0, -0.3 + 0.1j], dtype=torch.complex128)
+    x = torch.randn(100, dtype=torch.float64, generator=torch.Generator().manual_seed(0))
+    y = F.lfilter(b.to(dev), a.to(dev), x.to(dev))
+    ys = ss.lfilter(b.numpy(), a.numpy(), x.numpy())
+    assert y.is_complex() and y.dtype == torch.complex128 and y.shape == x.shape
+    np.testing.assert_allclose(y.cpu().numpy(), ys, rtol=1e-10, atol=1e-10)
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 5, 8])
+def test_complex_lfilter_against_oracle(oracle_mod, N):
+    """Orders 1-8 (real orders up to 16: the section, FIR and general
+    kernels, serial and time-parallel), FIR (a = [a0]) and full
+    denominators, complex and real x, zi per row, shared and absent,
+    T from 1 to 3000, axis 0 and -1."""
+    F, dev = _filt(), _dev()
+    rng = np.random.default_rng(100 + N)
+    worst = 0.0
+    for na in (1, N + 1):
+        for T in (1, 5, 700, 3000):
+            for cx in (False, True):
+                b = rng.standard_normal(N + 1) + 1j * rng.standard_normal(N + 1)
+                a = _stable_a(rng, na)
+                x = rng.standard_normal((2, 3, T))
+                if cx:
+                    x = x + 1j * rng.standard_normal((2, 3, T))
+                zi = rng.standard_normal((2, 3, N)) + 1j * rng.standard_normal((2, 3, N))
+                print(f"N={N} na={na} T={T} complex_x={cx}", flush=True)
+                bt, at = torch.tensor(b, device=dev), torch.tensor(a, device=dev)
+                xt = torch.tensor(x, device=dev)
+                y = F.lfilter(bt, at, xt)
+                assert y.dtype == torch.complex128
+                worst = max(worst, _relerr(y.cpu().numpy(), oracle_mod.lfilter(b, a, x)))
+                y, zf = F.lfilter(bt, at, xt, zi=torch.tensor(zi, device=dev))
+                yo, zfo = oracle_mod.lfilter(b, a, x, zi=zi)
+                worst = max(worst, _relerr(y.cpu().numpy(), yo))
+                assert zf.shape == (2, 3, N) and zf.dtype == torch.complex128
+                scale = max(np.max(np.abs(yo)), 1.0)
+                assert np.max(np.abs(zf.cpu().numpy() - zfo)) < 1e-9 * scale
+                # one state row for every channel
+                z1 = zi[0, 0]
+                y, zf = F.lfilter(bt, at, xt, zi=torch.tensor(z1, device=dev))
+                yo, zfo = oracle_mod.lfilter(b, a, x, zi=np.broadcast_to(z1, (2, 3, N)))
+                worst = max(worst, _relerr(y.cpu().numpy(), yo))
+                assert np.max(np.abs(zf.cpu().numpy() - zfo)) < 1e-9 * scale
+                # another axis
+                xa = np.ascontiguousarray(np.moveaxis(x, -1, 0))
+                y = F.lfilter(bt, at, torch.tensor(xa, device=dev), axis=0)
+                worst = max(worst, _relerr(np.moveaxis(y.cpu().numpy(), 0, -1),
+                                           oracle_mod.lfilter(b, a, x)))
+    print(f"order {N}: worst normwise error {worst:.3e}", flush=True)
+    assert worst < 1e-9
+
+
+def test_complex64_lfilter():
+    """complex64 coefficients and data: float32 kernels, within 1e-5 of the
+    complex128 oracle on the complex64-rounded values."""
+    import oracle.oracle as orc
+    F, dev = _filt(), _dev()
+    rng = np.random.default_rng(7)
+    for N in (1, 2):
+        b = (rng.standard_normal(N + 1) + 1j * rng.standard_normal(N + 1)).astype(np.complex64)
+        a = _stable_a(rng, N + 1, 0.8).astype(np.complex64)
+        x = (rng.standard_normal((4, 5000)) + 1j * rng.standard_normal((4, 5000))).astype(
+            np.complex64)
+        y = F.lfilter(torch.tensor(b, device=dev), torch.tensor(a, device=dev),
+                      torch.tensor(x, device=dev))
+        assert y.dtype == torch.complex64
+        ref = orc.lfilter(b.astype(np.complex128), a.astype(np.complex128),
+                          x.astype(np.complex128))
+        err = _relerr(y.cpu().numpy().astype(np.complex128), ref)
+        print(f"complex64 order {N}: {err:.3e}", flush=True)
+        assert err < 1e-5
+
+
+def test_complex_sosfilt_against_oracle(oracle_mod):
+    """Complex sections (a0 != 1 in one), real and complex x, zi per
+    channel and shared: y against the oracle's cascade; zf has zi's dtype
+    (a real zi keeps the real part of the state, as the reference's zf is
+    allocated like zi)."""
+    F, dev = _filt(), _dev()
+    rng = np.random.default_rng(9)
+    sos = np.array([[1.0, 0.5 + 0.2j, 0.1, 1.0, -0.3 + 0.1j, 0.05],
+                    [0.7, -0.2, 0.3j, 1.2, 0.1 - 0.2j, 0.1],
+                    [0.2 + 0.1j, 0.4, 0.2, 1.0, -0.5, 0.2 + 0.1j]])
+    st = torch.tensor(sos, device=dev)
+    for cx in (False, True):
+        x = rng.standard_normal((4, 2000))
+        if cx:
+            x = x + 1j * rng.standard_normal((4, 2000))
+        print(f"complex sos, complex_x={cx}", flush=True)
+        xt = torch.tensor(x, device=dev)
+        y = F.sosfilt(st, xt)
+        assert y.dtype == torch.complex128
+        assert _relerr(y.cpu().numpy(), oracle_mod.sosfilt(sos, x)) < 1e-9
+        zi = rng.standard_normal((4, 3, 2)) + 1j * rng.standard_normal((4, 3, 2))
+        y, zf = F.sosfilt(st, xt, zi=torch.tensor(zi, device=dev))
+        yo, zfo = oracle_mod.sosfilt(sos, x, zi=zi)
+        assert _relerr(y.cpu().numpy(), yo) < 1e-9
+        assert zf.dtype == torch.complex128
+        assert np.max(np.abs(zf.cpu().numpy() - zfo)) < 1e-9 * max(np.max(np.abs(yo)), 1)
+        z1 = zi[0].real.copy()
+        y, zf = F.sosfilt(st, xt, zi=torch.tensor(z1, device=dev))
+        yo, zfo = oracle_mod.sosfilt(sos, x, zi=np.broadcast_to(z1, (4, 3, 2)))
+        assert _relerr(y.cpu().numpy(), yo) < 1e-9
+        assert zf.dtype == torch.float64 and zf.shape == (4, 3, 2)
+        assert np.max(np.abs(zf.cpu().numpy() - zfo.real)) < 1e-9 * max(np.max(np.abs(yo)), 1)
+
+
+@pytest.mark.parametrize("N,na", [(1, 2), (3, 4), (3, 1)])
+def test_complex_coefficient_gradients(N, na):
+    """gradcheck (FP64, Wirtinger) of y and zf with respect to complex b, a,
+    x and zi through the section (N = 1), general-order (N = 3) and FIR
+    (a = [a0]) kernels."""
+    F, dev = _filt(), _dev()
+    rng = np.random.default_rng(20 + N + na)
+    b = torch.tensor(rng.standard_normal(N + 1) + 1j * rng.standard_normal(N + 1),
+                     device=dev, requires_grad=True)
+    a = torch.tensor(_stable_a(rng, na, 0.8), device=dev, requires_grad=True)
+    x = torch.tensor(rng.standard_normal((2, 9)) + 1j * rng.standard_normal((2, 9)),
+                     device=dev, requires_grad=True)
+    zi = torch.tensor(rng.standard_normal((2, N)) + 1j * rng.standard_normal((2, N)),
+                      device=dev, requires_grad=True)
+    print(f"gradcheck N={N} na={na}", flush=True)
+
+    def f(b, a, x, zi):
+        y, zf = F.lfilter(b, a, x, zi=zi)
+        return y, zf
+
+    assert torch.autograd.gradcheck(f, (b, a, x, zi), eps=1e-6, atol=1e-7, rtol=1e-6)
+
+
+def test_complex_sosfilt_gradients():
+    """gradcheck of the complex cascade with respect to sos, x and zi.  Every
+    a0 differs from 1: at a0 == 1 exactly the reference skips the
+    normalisation, so a0 receives no gradient there, while a finite
+    difference of a0 takes the normalising branch."""
+    F, dev = _filt(), _dev()
+    rng = np.random.default_rng(31)
+    sos = torch.tensor([[1.0, 0.5 + 0.2j, 0.1, 1.1 - 0.2j, -0.3 + 0.1j, 0.05],
+                        [0.7, -0.2, 0.3j, 1.2, 0.1 - 0.2j, 0.1]], device=dev,
+                       dtype=torch.complex128, requires_grad=True)
+    x = torch.tensor(rng.standard_normal((2, 8)), device=dev, requires_grad=True)
+    zi = torch.tensor(rng.standard_normal((2, 2, 2)) + 1j * rng.standard_normal((2, 2, 2)),
+                      device=dev, requires_grad=True)
+    print("gradcheck complex sosfilt", flush=True)
+    assert torch.autograd.gradcheck(lambda s, x, z: F.sosfilt(s, x, zi=z), (sos, x, zi),
+                                    eps=1e-6, atol=1e-7, rtol=1e-6)

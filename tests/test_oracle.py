@@ -165,6 +165,42 @@ def test_mixed_complex(golden, oracle_mod):
                                atol=1e-10)
 
 
+def test_complex_coefficients_match_scipy(oracle_mod):
+    """The oracle's complex-coefficient DF2T (lfilter and the section
+    cascade) against scipy.signal, which filters complex coefficients in the
+    same normalised DF2T (the reference's complex-coefficient test,
+    test__lfilter.py:416-428, compares with it): orders 0-6, FIR and full
+    denominators, a0 != 1, complex and real x, zi per row and T < order;
+    sections with a0 != 1 against scipy on the normalised rows."""
+    import scipy.signal as ss
+    rng = np.random.default_rng(11)
+    for N in range(0, 7):
+        for na in sorted({1, N + 1}):
+            for T in (2, 50):
+                b = rng.standard_normal(N + 1) + 1j * rng.standard_normal(N + 1)
+                a = 0.15 * (rng.standard_normal(na) + 1j * rng.standard_normal(na))
+                a[0] = 1.2 - 0.5j
+                x = rng.standard_normal((3, T)) + 1j * rng.standard_normal((3, T))
+                np.testing.assert_allclose(oracle_mod.lfilter(b, a, x), ss.lfilter(b, a, x),
+                                           rtol=1e-10, atol=1e-10)
+                if N == 0:
+                    continue
+                zi = rng.standard_normal((3, N)) + 1j * rng.standard_normal((3, N))
+                y, zf = oracle_mod.lfilter(b, a, x.real, zi=zi)
+                ys, zfs = ss.lfilter(b, a, x.real, zi=zi)
+                np.testing.assert_allclose(y, ys, rtol=1e-10, atol=1e-10)
+                np.testing.assert_allclose(zf, zfs, rtol=1e-10, atol=1e-10)
+    sos = np.array([[1.0, 0.5 + 0.2j, 0.1, 1.0, -0.3 + 0.1j, 0.05],
+                    [0.7, -0.2, 0.3j, 1.2, 0.1 - 0.2j, 0.1]])
+    sn = sos / sos[:, 3:4]
+    x = rng.standard_normal((3, 60))
+    zi = rng.standard_normal((3, 2, 2)) + 1j * rng.standard_normal((3, 2, 2))
+    y, zf = oracle_mod.sosfilt(sos, x, zi=zi)
+    ys, zfs = ss.sosfilt(sn, x, zi=zi.transpose(1, 0, 2))
+    np.testing.assert_allclose(y, ys, rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(zf, zfs.transpose(1, 0, 2), rtol=1e-10, atol=1e-10)
+
+
 def _ff_args(case):
     padtype = str(case["padtype"])
     padtype = None if padtype == "none" else padtype

@@ -6,9 +6,11 @@ Filters of order <= 2 (max(len(b), len(a)) <= 3) are one second-order
 section and run through the single-pass SOS kernel, with autograd.  Higher
 orders with a single denominator coefficient (FIR, a = [a0]) run the
 direct-form FIR kernel, with autograd.  Other higher orders run the
-general-order DF2T kernel (one lane per channel, serial in time), with an
-autograd whose backward reuses that kernel (reverse-time all-pole pass), the
-FIR kernel and the tap-gradient kernel.
+general-order DF2T kernel (time-parallel chunks up to order 16, one lane
+per channel above), with an autograd whose backward reuses that kernel
+(reverse-time all-pole pass), the FIR kernel and the tap-gradient kernel.
+Complex coefficients run as four of these real filters on the
+conjugate-extended polynomials (see _lfilter_complex_coef).
 
 zi is (batch..., N) with x's batch dimensions in order (the documented
 meaning; the reference's moveaxis of zi, _lfilter.py:137, is not followed),
@@ -48,8 +50,7 @@ def lfilter(
     a = torch.atleast_1d(a)
     order = plan_lfilter(b.shape, a.shape)
     if b.is_complex() or a.is_complex():
-        raise NotImplementedError(
-            "complex filter coefficients are not supported on the MI355X path")
+        return _lfilter_complex_coef(b, a, x, axis, zi, order)
     if order <= 2:
         return _lfilter_as_section(b, a, x, axis, zi, order)
     if a.shape[0] == 1:
@@ -75,6 +76,102 @@ def _complex_split(b, a, x, axis, zi, out):
     if zi is None:
         return y
     return y, torch.complex(zfs[0], zfs[1])
+
+
+def _polymul(p, q):
+    """Coefficients of P(z) Q(z) for rows p (..., n) and one q (m,):
+    (..., n + m - 1), as sums of shifted products (differentiable)."""
+    n, m = p.shape[-1], q.shape[-1]
+    return sum(F.pad(p * q[j], (j, m - 1 - j)) for j in range(m))
+
+
+def _lfilter_complex_coef(b, a, x, axis, zi, order):
+    """lfilter with complex coefficients, through the real kernels.
+
+    With B(z), A(z) the normalised polynomials (a0 = 1) and Z(z) the
+    polynomial of zi's N state values, the DF2T output is
+    Y = (B X + Z) / A.  Multiplying above and below by A~(z), A's
+    polynomial with conjugated coefficients, gives a real denominator
+    A A~ (its coefficient of z^-m, sum_k a_k conj(a_(k-m')), equals its own
+    conjugate) and the numerator (B A~) X + Z A~.  So, with c = B A~ and
+    s = Z A~ split into real and imaginary parts and d = A A~,
+      Re y = lfilter(Re c, d, Re x, zi = Re s) - lfilter(Im c, d, Im x),
+      Im y = lfilter(Re c, d, Im x, zi = Im s) + lfilter(Im c, d, Re x),
+    four real filters of order N + len(a) - 1 (N with an FIR a = [a0]) that
+    run on the HIP kernels, two launches each over stacked channels.  The
+    poles that A~ adds are the conjugates of A's, inside the unit circle
+    when A's are, and are cancelled by A~'s zeros in the numerator to
+    rounding.  zf follows from the DF2T state definition on the last N
+    samples of x and y (_lfilter.py:153-166):
+      zf[k] = sum_(j > k) (b_j x[T-1+k+1-j] - a_j y[T-1+k+1-j]) + zi[k + T].
+    Complex coefficients of order above 32 (real order above 64) raise."""
+    nd = x.dim()
+    ax = axis % nd
+    xm = torch.movedim(x, ax, -1)
+    batch = xm.shape[:-1]
+    T = xm.shape[-1]
+    zb = False
+    if zi is not None:
+        zb = plan_lfilter_zi(zi.shape, batch, order) if order > 0 else False
+    dev = _require_device(x)
+    out = torch.promote_types(torch.promote_types(b.dtype, a.dtype), x.dtype)
+    comp = torch.float64 if out == torch.complex128 else torch.float32
+    na = a.shape[0]
+    if order + na - 1 > 64:
+        raise NotImplementedError(
+            "complex filter coefficients are supported up to order 32 on the MI355X path")
+    bc = b.to(device=dev, dtype=out)
+    ac = a.to(device=dev, dtype=out)
+    a0 = ac[0]
+    if bool(a0 != 1):
+        # the reference's normalisation (_lfilter.py:96-98); a[0] is then 1
+        bc = bc / a0
+        ac = torch.cat([torch.ones(1, dtype=out, device=dev), ac[1:] / a0])
+    bc = F.pad(bc, (0, order + 1 - bc.shape[0]))
+    ab = ac.conj().resolve_conj()
+    d = _polymul(ac, ab).real.to(comp).contiguous()
+    c = _polymul(bc, ab)
+    cr = c.real.to(comp).contiguous()
+    ci = c.imag.to(comp).contiguous()
+    R = max(c.shape[0], d.shape[0]) - 1
+    B = 1
+    for n_ in batch:
+        B *= n_
+    if order == 0:
+        y = torch.movedim(bc[0] * xm.to(out), -1, ax)
+        if zi is None:
+            return y
+        return y, torch.zeros_like(zi)
+    xc = xm.to(out)
+    xr, xi = xc.real.to(comp), xc.imag.to(comp)
+    sr = si = None
+    if zi is not None:
+        zrow = zi.to(device=dev, dtype=out).reshape(-1, order)
+        s = _polymul(zrow, ab)                               # (rows, N + na - 1)
+        s = F.pad(s, (0, R - s.shape[-1]))
+        s = s.expand(B, R) if zb else s
+        sr = s.real.to(comp).reshape(batch + (R,))
+        si = s.imag.to(comp).reshape(batch + (R,))
+    # P = F(Re c) over [Re x, Im x] with the state parts; Q = F(Im c) over [Im x, Re x]
+    xs = torch.stack([xr, xi])
+    zs = None if zi is None else torch.stack([sr, si])
+    P = lfilter(cr, d, xs, axis=-1, zi=zs)
+    P = P[0] if zi is not None else P
+    Q = lfilter(ci, d, torch.stack([xi, xr]), axis=-1)
+    y = torch.complex(P[0] - Q[0], P[1] + Q[1]).to(out)
+    y_out = torch.movedim(y, -1, ax)
+    if zi is None:
+        return y_out
+    # the final DF2T state from the last N samples of x and y
+    N = order
+    ac_p = F.pad(ac, (0, N + 1 - ac.shape[0]))
+    xt = _tail(xc.reshape(B, T), N)
+    yt = _tail(y.reshape(B, T), N)
+    zf = xt @ _tail_toeplitz(bc, N).transpose(0, 1) - yt @ _tail_toeplitz(ac_p, N).transpose(0, 1)
+    if T < N:
+        zrow = zi.to(device=dev, dtype=out).reshape(-1, N).expand(B, N)
+        zf = zf + F.pad(zrow[:, T:], (0, T))
+    return y_out, zf.reshape(batch + (N,))
 
 
 def _state_rows(zi, order, dev, comp):

@@ -49,6 +49,8 @@ def sosfilt(
     bfloat16 I/O).  Float32 recurrences of filters with a pole beyond radius
     0.995 run in the DF2T update order with float64 carries.
     """
+    if sos.is_complex():
+        return _sosfilt_complex_coef(sos, x, axis, zi, precision)
     plan = plan_sosfilt(sos.shape, sos.dtype, x.shape, x.dtype, axis,
                         None if zi is None else zi.shape,
                         None if zi is None else zi.dtype, precision)
@@ -103,6 +105,50 @@ def sosfilt(
     if zi is None:
         return y
     return y, zf.reshape(plan.zf_shape).to(zi.dtype)
+
+
+def _sosfilt_complex_coef(sos, x, axis, zi, precision):
+    """Complex sections: the reference's cascade of lfilter calls, one per
+    section (_sosfilt.py:116-128), each through the complex-coefficient
+    lfilter (real kernels on the conjugate-extended sections).  zf has zi's
+    dtype, as in the reference, whose zf is allocated like zi
+    (_sosfilt.py:114) so a real zi keeps only the real part of the state."""
+    from ._lfilter import lfilter
+    if sos.dim() != 2 or sos.shape[1] != 6:
+        raise ValueError("sos must be shape (n_sections, 6)")
+    if precision is not None:
+        raise NotImplementedError("precision= is for real filter coefficients")
+    S = sos.shape[0]
+    nd = x.dim()
+    if nd == 0:
+        raise ValueError("x must have at least one dimension")
+    ax = axis % nd
+    batch = tuple(x.shape[:ax] + x.shape[ax + 1:])
+    zs = None
+    if zi is not None:
+        if tuple(zi.shape) == (S, 2) and len(batch) > 0:
+            zs = zi.expand(batch + (S, 2))
+        elif tuple(zi.shape) == batch + (S, 2):
+            zs = zi
+        else:
+            raise RuntimeError(
+                f"zi must have shape {batch + (S, 2)} (or ({S}, 2)), got {tuple(zi.shape)}")
+    y = x.clone()
+    zfs = []
+    for i in range(S):
+        if zs is None:
+            y = lfilter(sos[i, :3], sos[i, 3:], y, axis=axis)
+        else:
+            y, zf_i = lfilter(sos[i, :3], sos[i, 3:], y, axis=axis, zi=zs[..., i, :])
+            zfs.append(zf_i)
+    if zi is None:
+        return y
+    if S == 0:
+        return y, torch.zeros(batch + (0, 2), dtype=zi.dtype, device=zi.device)
+    zf = torch.stack(zfs, dim=-2)
+    if not zi.dtype.is_complex:
+        zf = zf.real
+    return y, zf.to(zi.dtype)
 
 
 def _run(sos_c, xr, zr, B, comp, io, want_zf, zi_b, compensated=False):
