@@ -93,6 +93,21 @@ typedef enum tsiir_mode {
 /* zi / zf hold FP64 values whatever io_dtype is (a caller that carries a
  * stream's state between chunks in FP64 with FP32 or BF16 samples). */
 #define TSIIR_STATE_F64 0x200u
+/* Few channels of long rows (fewer than 1024 channels, contiguous rows,
+ * rows of 2^k * 16384 samples with k large enough for 1024 segments, at most
+ * 16 sections, no REVERSE and no saved section outputs) take a segmented
+ * form when every filter forgets its state within TSIIR_SEGMENT_WARMUP
+ * samples: the infinity norm of Phi^4096 below 2^-64, Phi being the 2S x 2S
+ * state-transition matrix of the cascade in DF2T form, checked on the device
+ * in the call.  Each segment then runs as an independent row of the
+ * many-channel kernels, starting from the state that a
+ * TSIIR_SEGMENT_WARMUP-sample pass from zero state over the end of the
+ * segment before it reaches, so no look-back wait links segments.  The
+ * result equals the look-back form's to rounding.  The look-back form is
+ * launched as well and runs only when the check fails.  This flag keeps the
+ * look-back form only (for comparisons). */
+#define TSIIR_NO_SEGMENT 0x400u
+#define TSIIR_SEGMENT_WARMUP 4096
 
 typedef struct tsiir_sosfilt_desc {
   int64_t channels;          /* output channels B                          */

@@ -60,14 +60,51 @@ def test_workspace_size_query_without_gpu():
                          mode=0, flags=0, reserved=0, x_rows=4096, x_row_stride=65536,
                          y_row_stride=65536, chans_per_filter=4096)
     n = L.tsiir_sosfilt_workspace_bytes(ctypes.byref(d))
-    # header + one 896-byte table entry per section (rounded to 256 B) +
+    # 1024-byte header + one 896-byte table entry per section (rounded to 256 B) +
     # 64 B of status per (window, channel, section); windows of 4096 samples;
     # then (256-B aligned) one 1088-byte FP32-kernel entry per section
-    fp64_part = 256 + 5376 + 16 * 4096 * 6 * 64
+    fp64_part = 1024 + 5376 + 16 * 4096 * 6 * 64
     assert n == (fp64_part + 255) // 256 * 256 + 6 * 1088
     d.n_sections = 0
     assert L.tsiir_sosfilt_workspace_bytes(ctypes.byref(d)) == 0
     assert L.tsiir_status_string(2) == b"unsupported request"
+
+
+def test_segmented_workspace_plan_without_gpu():
+    """Few long rows (8 x 2^24, the long-sequence shape) size the workspace
+    for the segmented form as well: 1024 segments of 131072 samples (the
+    longest power of two that still gives 1024 rows), the many-channel
+    tables and status words of the 131072-sample main pass, the
+    4096-sample warm-up outputs and three FP64 state arrays; TSIIR_NO_SEGMENT,
+    REVERSE, 1024 channels or a row length with no such segment size keep
+    the look-back form's size."""
+    L = _lib.load()
+
+    def ws(B, T, flags=0):
+        d = _lib.SosfiltDesc(channels=B, n_samples=T, n_sections=6, io_dtype=_lib.F32,
+                             coef_dtype=_lib.F32, mode=0, flags=flags, reserved=0, x_rows=B,
+                             x_row_stride=T, y_row_stride=T, chans_per_filter=B)
+        return L.tsiir_sosfilt_workspace_bytes(ctypes.byref(d))
+
+    def align(v):
+        return (v + 255) // 256 * 256
+
+    def plain(B, T):
+        # header, one 896-byte table entry per section, 64 B of status per
+        # (4096-sample window, row, section), one 1088-byte FP32 entry per section
+        return align(1024 + 5376 + (-(-T // 4096)) * B * 6 * 64) + 6 * 1088
+
+    B, T = 8, 1 << 24
+    rows, lseg = 1024, 131072
+    base = align(max(plain(rows, 4096), plain(rows, lseg)))
+    states = rows * 6 * 2 * 8
+    seg = align(align(align(base + rows * 4096 * 4) + states) + states)
+    assert ws(B, T, _lib.NO_SEGMENT) == plain(B, T)
+    assert ws(B, T) == max(seg, plain(B, T))
+    assert ws(B, T, _lib.WRITE_ZF) == max(seg + states, plain(B, T))
+    assert ws(B, T, _lib.REVERSE) == plain(B, T)
+    assert ws(1024, 65536) == plain(1024, 65536)
+    assert ws(8, 3 * 5 * 7 * 16384 + 4096) == plain(8, 3 * 5 * 7 * 16384 + 4096)
 
 
 def test_fir_fft_host_checks_without_gpu():

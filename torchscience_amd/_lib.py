@@ -26,6 +26,8 @@ FIR_REVERSE_TAPS = 0x40
 FP32_DF2T = 0x80
 DEBUG_GIVE_UP = 0x100
 STATE_F64 = 0x200
+NO_SEGMENT = 0x400
+SEGMENT_WARMUP = 4096
 
 # Every symbol include/tsiir.h declares.
 EXPORTED = (

@@ -23,14 +23,21 @@ constexpr uint32_t kSpinLimit = 1u << 18;
 // Workspace header (256 bytes).  `epoch` advances before every launch that
 // publishes scan status words (see launch_flag).
 struct WsHeader {
-  uint32_t reserved0;
+  uint32_t seg_gate;   // 1 while the segmented form applies (see seg_check_kernel)
   uint32_t reserved1;
   uint32_t epoch;
   uint32_t error;      // set when a look-back wait gave up (see give_up)
   uint32_t* mailbox;   // caller's device-visible word that also gets it, or null
-  uint32_t pad[58];
+  // the last shared filter the segmented form's device check judged: its
+  // section count, coefficient type and values (as doubles), and the verdict
+  uint32_t seg_cache_valid;
+  int32_t seg_cache_sections;
+  int32_t seg_cache_coef;
+  uint32_t seg_cache_verdict;
+  uint32_t pad[54];
+  double seg_cache_sos[16 * 6];
 };
-static_assert(sizeof(WsHeader) == 256, "header size");
+static_assert(sizeof(WsHeader) == 1024, "header size");
 
 // A look-back wait that gives up (its spin limit passed, or the caller's
 // TSIIR_DEBUG_GIVE_UP): `bit` goes into the header's error word and the
@@ -144,7 +151,16 @@ struct FwdArgs {
   int S;
   uint32_t flags;
   int io_dtype;
+  // When set, the kernel runs only if (*gate != 0) == gate_on: the segmented
+  // form and the look-back form of one call are both launched and a device
+  // check picks which one works (every kernel of the other exits at once).
+  const uint32_t* gate = nullptr;
+  uint32_t gate_on = 0;
 };
+
+__device__ __forceinline__ bool gated_out(const FwdArgs& a) {
+  return a.gate != nullptr && ((*a.gate != 0u) != (a.gate_on != 0u));
+}
 
 // 64-bit status word = (flag << 32) | 32-bit payload.  A double is stored
 // as two words (low half, high half).  Each word is written and read with a

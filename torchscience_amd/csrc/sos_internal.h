@@ -24,18 +24,31 @@ struct FwdGeometry {
 
 bool fwd_geometry(const tsiir_sosfilt_desc* d, FwdGeometry* g);
 
+// Workspace bytes of the forward for this descriptor, including the
+// segmented form's buffers where TSIIR_SEGMENTED applies (`saved`: the
+// section outputs are kept, which the segmented form does not do).
+size_t fwd_workspace_bytes(const tsiir_sosfilt_desc* d, bool saved);
+
 // Coefficient tables (see sos_common.h) for n_rows (filter, section) rows.
 // Also advances the workspace epoch by n_epochs (one per following launch
-// that publishes scan status words).
+// that publishes scan status words).  With a gate (see FwdArgs::gate) it
+// does nothing unless (*gate != 0) == gate_on.
 struct WsHeader;
 void launch_prep(const void* sos, int coef_dtype, int64_t n_rows, int log2w, bool adjoint,
-                 double* tables, WsHeader* hdr, uint32_t n_epochs, hipStream_t stream);
+                 double* tables, WsHeader* hdr, uint32_t n_epochs, hipStream_t stream,
+                 const uint32_t* gate = nullptr, uint32_t gate_on = 0);
 
 // tsiir_sosfilt_forward, optionally also writing the outputs of sections
 // 0..S-2 to `saved` ((S-1, channels, T) in the I/O type).
+// A device word that decides whether a launch runs (see FwdArgs::gate).
+struct FwdGate {
+  const uint32_t* word;
+  uint32_t on;
+};
+
 tsiir_status forward_impl(const tsiir_sosfilt_desc* d, const void* x, void* y, const void* sos,
                           const void* zi, void* zf, void* workspace, size_t workspace_bytes,
-                          void* stream, void* saved);
+                          void* stream, void* saved, const FwdGate* gate = nullptr);
 
 // FP32-recurrence forward (sosfilt_fwd32.hip): builds the FP64 tables of
 // a.tables (for 64-sample lanes) and the Fast32 tables, advances the

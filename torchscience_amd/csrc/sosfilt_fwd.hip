@@ -23,6 +23,7 @@
 // Tickets taken with an atomic counter order the waves (window-major) so a
 // wave only ever waits on windows that were issued before it.
 
+#include <cmath>
 #include <stdlib.h>
 #include <type_traits>
 #include <string.h>
@@ -120,6 +121,7 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
   static_assert((1 << kLog2NW) == NW, "NW must be 1, 2, 4 or 8");
   static_assert(!CHAINED || (NW == 1 && !TAIL), "chained windows are single-wave");
   static_assert(W < kGLen, "the zero-input response table covers 64 samples");
+  if (gated_out(a)) return;
   __shared__ __align__(16) TIO lds[(LDS && !TAIL) ? NW * LdsStage<TIO, W>::kElems : 8];
   __shared__ double wagg[NW][2];  // wave aggregates of the current section
   __shared__ double wsin[2];      // incoming state of the window
@@ -548,7 +550,9 @@ __global__ __launch_bounds__(64) void sos_chan_kernel(FwdArgs a,
 template <typename TC>
 __global__ __launch_bounds__(256) void sos_prep_kernel(const TC* sos, int64_t n_rows, int log2w,
                                                        int adjoint, double* tables,
-                                                       WsHeader* hdr, uint32_t n_epochs) {
+                                                       WsHeader* hdr, uint32_t n_epochs,
+                                                       const uint32_t* gate, uint32_t gate_on) {
+  if (gate != nullptr && ((*gate != 0u) != (gate_on != 0u))) return;
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i == 0 && hdr != nullptr) hdr->epoch += n_epochs;
   if (i >= n_rows) return;
@@ -595,16 +599,17 @@ __global__ __launch_bounds__(256) void sos_prep_kernel(const TC* sos, int64_t n_
 }
 
 void launch_prep(const void* sos, int coef_dtype, int64_t n_rows, int log2w, bool adjoint,
-                 double* tables, WsHeader* hdr, uint32_t n_epochs, hipStream_t stream) {
+                 double* tables, WsHeader* hdr, uint32_t n_epochs, hipStream_t stream,
+                 const uint32_t* gate, uint32_t gate_on) {
   const unsigned blocks = static_cast<unsigned>((n_rows + 255) / 256);
   if (coef_dtype == TSIIR_F64)
     hipLaunchKernelGGL(sos_prep_kernel<double>, dim3(blocks), dim3(256), 0, stream,
                        static_cast<const double*>(sos), n_rows, log2w, adjoint ? 1 : 0, tables,
-                       hdr, n_epochs);
+                       hdr, n_epochs, gate, gate_on);
   else
     hipLaunchKernelGGL(sos_prep_kernel<float>, dim3(blocks), dim3(256), 0, stream,
                        static_cast<const float*>(sos), n_rows, log2w, adjoint ? 1 : 0, tables,
-                       hdr, n_epochs);
+                       hdr, n_epochs, gate, gate_on);
 }
 
 // zf = zi (or zeros) for an empty time axis.
@@ -615,6 +620,143 @@ __global__ void copy_state_kernel(const TACC* zi, TACC* zf, int64_t B, int S, in
   const int64_t within = i % (S * 2);
   const int64_t row = i / (S * 2);
   zf[i] = zi ? zi[(bcast ? 0 : row) * S * 2 + within] : TACC(0);
+}
+
+// Segmented rows: row r = c * nseg + k is segment k of channel c.  Its
+// incoming state is channel c's zi (or zeros) for k = 0, else the warm-up end
+// state of row r - 1 (FP64 throughout).  Runs only while the gate is set.
+template <typename TS>
+__global__ void seg_state_in_kernel(const uint32_t* gate, const double* __restrict__ warm,
+                                    const TS* __restrict__ zi, int bcast,
+                                    double* __restrict__ zin, int64_t rows, int64_t nseg, int S) {
+  if (*gate == 0u) return;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t per = static_cast<int64_t>(S) * 2;
+  if (i >= rows * per) return;
+  const int64_t r = i / per, within = i % per;
+  const int64_t k = r % nseg, c = r / nseg;
+  double v;
+  if (k != 0)
+    v = warm[(r - 1) * per + within];
+  else
+    v = zi ? static_cast<double>(zi[(bcast ? 0 : c) * per + within]) : 0.0;
+  zin[i] = v;
+}
+
+// zf of channel c = the final state of its last segment, in the caller's
+// state type.
+template <typename TS>
+__global__ void seg_state_out_kernel(const uint32_t* gate, const double* __restrict__ zall,
+                                     TS* __restrict__ zf, int64_t channels, int64_t nseg, int S) {
+  if (*gate == 0u) return;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t per = static_cast<int64_t>(S) * 2;
+  if (i >= channels * per) return;
+  const int64_t c = i / per, within = i % per;
+  zf[i] = static_cast<TS>(zall[(c * nseg + nseg - 1) * per + within]);
+}
+
+__global__ void seg_gate_init_kernel(WsHeader* hdr) { hdr->seg_gate = 1u; }
+
+// One wave per filter: Phi, the 2S x 2S state-transition matrix of the
+// cascade in DF2T form (one zero-input step), squared `squarings` times in
+// FP64.  The filter forgets its state within 2^squarings samples when the
+// infinity norm of some Phi^(2^q) falls below 2^-64 (every higher power is
+// then smaller still); any other outcome (a norm that stays larger, is not
+// finite, or a zero a0) is a no.  S <= 16.
+//
+// `single` (one filter for every channel, one workgroup): the header keeps
+// the last filter judged (its section count, coefficient type and values,
+// compared bit for bit) with its verdict, so a call with the same filter
+// reads the verdict instead of recomputing it, and the kernel sets the gate
+// itself.  Otherwise (one workgroup per filter) a no clears the gate, which
+// seg_gate_init_kernel set beforehand.
+constexpr int kSegCheckThreads = kWave;
+template <typename TC>
+__global__ __launch_bounds__(kSegCheckThreads) void seg_check_kernel(const TC* __restrict__ sos,
+                                                                   int S, int squarings,
+                                                                   int coef_code, int single,
+                                                                   WsHeader* hdr) {
+  __shared__ double P[32 * 32];
+  __shared__ double Q[32 * 32];
+  const int n = 2 * S;
+  const int t = threadIdx.x;
+  const TC* sf = sos + static_cast<int64_t>(blockIdx.x) * S * 6;
+  if (single) {
+    bool same = hdr->seg_cache_valid != 0u && hdr->seg_cache_sections == S &&
+                hdr->seg_cache_coef == coef_code;
+    for (int i = t; i < 6 * S; i += kSegCheckThreads)
+      same = same && __double_as_longlong(hdr->seg_cache_sos[i]) ==
+                         __double_as_longlong(static_cast<double>(sf[i]));
+    if (__syncthreads_and(same)) {
+      if (t == 0) hdr->seg_gate = hdr->seg_cache_verdict;
+      return;
+    }
+  }
+  // lane j < n: the cascade's zero-input step from unit state e_j, its
+  // state vector kept in row j of Q
+  bool ok = true;
+  if (t < n) {
+    for (int i = 0; i < n; ++i) Q[t * n + i] = (i == t) ? 1.0 : 0.0;
+    double u = 0.0;
+    for (int s = 0; s < S; ++s) {
+      const double a0 = static_cast<double>(sf[6 * s + 3]);
+      if (!(a0 != 0.0) || !isfinite(a0)) ok = false;
+      const double b0 = static_cast<double>(sf[6 * s]) / a0;
+      const double b1 = static_cast<double>(sf[6 * s + 1]) / a0;
+      const double b2 = static_cast<double>(sf[6 * s + 2]) / a0;
+      const double a1 = static_cast<double>(sf[6 * s + 4]) / a0;
+      const double a2 = static_cast<double>(sf[6 * s + 5]) / a0;
+      const double yv = b0 * u + Q[t * n + 2 * s];
+      const double n0 = b1 * u - a1 * yv + Q[t * n + 2 * s + 1];
+      const double n1 = b2 * u - a2 * yv;
+      Q[t * n + 2 * s] = n0;
+      Q[t * n + 2 * s + 1] = n1;
+      u = yv;
+    }
+  }
+  int state = __syncthreads_or(!ok) ? 2 : 0;  // 0 running, 1 small, 2 failed
+  for (int e = t; e < n * n; e += kSegCheckThreads) P[e] = Q[(e % n) * n + e / n];
+  __syncthreads();
+  for (int q = 0; q <= squarings && state == 0; ++q) {
+    double row = 0.0;
+    if (t < n)
+      for (int j = 0; j < n; ++j) row += fabs(P[t * n + j]);
+    // the infinity norm: the largest row sum (a wave reduction; a NaN row
+    // sum makes the norm test fail through `fin`)
+    double m = row;
+    int fin = isfinite(row) ? 1 : 0;
+    for (int off = 32; off >= 1; off >>= 1) {
+      m = fmax(m, __shfl_xor(m, off));
+      fin &= __shfl_xor(fin, off);
+    }
+    if (!fin) state = 2;
+    else if (m < 0x1p-64) state = 1;
+    if (state != 0 || q == squarings) break;
+    for (int e = t; e < n * n; e += kSegCheckThreads) {
+      const int i = e / n, j = e % n;
+      double acc = 0.0;
+      for (int k = 0; k < n; ++k) acc = fma(P[i * n + k], P[k * n + j], acc);
+      Q[e] = acc;
+    }
+    __syncthreads();
+    for (int e = t; e < n * n; e += kSegCheckThreads) P[e] = Q[e];
+    __syncthreads();
+  }
+  const uint32_t verdict = state == 1 ? 1u : 0u;
+  if (single) {
+    for (int i = t; i < 6 * S; i += kSegCheckThreads)
+      hdr->seg_cache_sos[i] = static_cast<double>(sf[i]);
+    if (t == 0) {
+      hdr->seg_cache_sections = S;
+      hdr->seg_cache_coef = coef_code;
+      hdr->seg_cache_verdict = verdict;
+      hdr->seg_cache_valid = 1u;
+      hdr->seg_gate = verdict;
+    }
+  } else if (t == 0 && verdict == 0u) {
+    __hip_atomic_fetch_and(&hdr->seg_gate, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 __global__ void set_mailbox_kernel(WsHeader* hdr, uint32_t* mailbox) { hdr->mailbox = mailbox; }
@@ -673,7 +815,7 @@ static bool fwd_lds_io() {
 template <typename TIO, typename TACC, typename TR, int W>
 static tsiir_status launch_fwd(const tsiir_sosfilt_desc* d, const FwdGeometry& g, const void* x,
                                void* y, const void* sos, const void* zi, void* zf, char* ws,
-                               hipStream_t stream, void* saved = nullptr) {
+                               hipStream_t stream, void* saved, const FwdGate* gate) {
   WsHeader* hdr = reinterpret_cast<WsHeader*>(ws);
   double* tables = reinterpret_cast<double*>(ws + sizeof(WsHeader));
   uint64_t* status = reinterpret_cast<uint64_t*>(ws + sizeof(WsHeader) + g.table_bytes);
@@ -698,6 +840,8 @@ static tsiir_status launch_fwd(const tsiir_sosfilt_desc* d, const FwdGeometry& g
   a.flags = d->flags;
   a.saved = saved;
   a.io_dtype = d->io_dtype;
+  a.gate = gate ? gate->word : nullptr;
+  a.gate_on = gate ? gate->on : 0u;
   const FwdPolicy pol = fwd_policy(d);
   if constexpr (std::is_same<TR, float>::value && W == 64) {
     if (use_fwd32(d)) {
@@ -714,7 +858,8 @@ static tsiir_status launch_fwd(const tsiir_sosfilt_desc* d, const FwdGeometry& g
       return hipGetLastError() == hipSuccess ? TSIIR_OK : TSIIR_ELAUNCH;
     }
   }
-  launch_prep(sos, d->coef_dtype, n_rows, log2w, false, tables, hdr, 1, stream);
+  launch_prep(sos, d->coef_dtype, n_rows, log2w, false, tables, hdr, 1, stream, a.gate,
+              a.gate_on);
   const int64_t win = static_cast<int64_t>(kWave) * W * pol.nw;
   const int64_t full = d->n_samples / win;
   const int64_t groups = full * d->channels;
@@ -808,9 +953,151 @@ static int chan_w_f32() {
   return w;
 }
 
+// ---- segmented form ----------------------------------------------------------
+//
+// Few channels of long rows wait at every section of every group for the
+// look-back.  A filter that forgets its state within TSIIR_SEGMENT_WARMUP
+// samples does not need it: each segment of a row can start from the state
+// that a warm-up pass from zero state over the last TSIIR_SEGMENT_WARMUP
+// samples before it reaches (the neglected part is Phi^4096 times the true
+// state, below 2^-64 of it), and then every segment is an independent row of
+// the many-channel kernels, chained only within itself.  Whether the filter
+// forgets is decided on the device (seg_check_kernel), so both forms are
+// launched and the gate lets exactly one of them run.
+
+constexpr int64_t kSegRows = 1024;      // rows from which the forward chains its windows
+constexpr int64_t kSegMinLen = 16384;   // a four-window group of the FP32 kernel
+constexpr int kSegSquarings = 12;       // 2^12 = TSIIR_SEGMENT_WARMUP
+static_assert((int64_t{1} << kSegSquarings) == TSIIR_SEGMENT_WARMUP, "warm-up length");
+
+struct SegPlan {
+  int64_t lseg = 0, nseg = 0, rows = 0;
+  tsiir_sosfilt_desc warm{}, main{};
+  size_t base = 0, ywarm_off = 0, warm_off = 0, zin_off = 0, zall_off = 0, total = 0;
+};
+
+static size_t io_bytes(int32_t dt) { return dt == TSIIR_F64 ? 8 : dt == TSIIR_BF16 ? 2 : 4; }
+
+// The longest power-of-two segment that divides the row and still gives the
+// many-channel kernels at least kSegRows rows; false where the form does not
+// apply.
+static bool seg_plan(const tsiir_sosfilt_desc* d, bool saved, SegPlan* p) {
+  if ((d->flags & TSIIR_NO_SEGMENT) || saved || use_chan(d, nullptr)) return false;
+  if (d->flags & (TSIIR_REVERSE | TSIIR_PATH_CHANNEL | TSIIR_DEBUG_GIVE_UP)) return false;
+  if (d->n_sections > 16) return false;
+  const int64_t B = d->channels, T = d->n_samples;
+  if (B < 1 || B >= kSegRows || T < 2 * kSegMinLen) return false;
+  if (d->x_rows != B || (B > 1 && (d->x_row_stride != T || d->y_row_stride != T))) return false;
+  const int64_t nmin = (kSegRows + B - 1) / B;
+  int64_t lseg = 0;
+  for (int64_t l = kSegMinLen; l <= T / 2; l *= 2)
+    if (T % l == 0 && T / l >= nmin) lseg = l;
+  if (lseg == 0) return false;
+  p->lseg = lseg;
+  p->nseg = T / lseg;
+  p->rows = B * p->nseg;
+  const uint32_t keep = d->flags & (TSIIR_FP32_DF2T | TSIIR_PATH_WINDOW);
+  tsiir_sosfilt_desc w = *d;
+  w.channels = p->rows;
+  w.n_samples = TSIIR_SEGMENT_WARMUP;
+  w.x_rows = p->rows;
+  w.x_row_stride = lseg;
+  w.y_row_stride = TSIIR_SEGMENT_WARMUP;
+  w.chans_per_filter = d->chans_per_filter >= B ? p->rows : d->chans_per_filter * p->nseg;
+  w.flags = keep | TSIIR_WRITE_ZF | TSIIR_STATE_F64;
+  tsiir_sosfilt_desc m = w;
+  m.n_samples = lseg;
+  m.y_row_stride = lseg;
+  m.flags = keep | TSIIR_HAS_ZI | TSIIR_STATE_F64 | (d->flags & TSIIR_WRITE_ZF);
+  p->warm = w;
+  p->main = m;
+  FwdGeometry gw, gm;
+  if (!fwd_geometry(&w, &gw) || !fwd_geometry(&m, &gm)) return false;
+  const size_t states = static_cast<size_t>(p->rows) * d->n_sections * 2 * sizeof(double);
+  p->base = align_up(gw.total > gm.total ? gw.total : gm.total, 256);
+  p->ywarm_off = p->base;
+  p->warm_off = align_up(p->ywarm_off + static_cast<size_t>(p->rows) * TSIIR_SEGMENT_WARMUP *
+                                            io_bytes(d->io_dtype), 256);
+  p->zin_off = align_up(p->warm_off + states, 256);
+  p->zall_off = align_up(p->zin_off + states, 256);
+  p->total = p->zall_off + ((d->flags & TSIIR_WRITE_ZF) ? states : 0);
+  return true;
+}
+
+// Workspace bytes of the forward for this descriptor: where the segmented
+// form applies, enough for it and for the look-back form (one of them runs,
+// one after the other on the stream).
+size_t fwd_workspace_bytes(const tsiir_sosfilt_desc* d, bool saved) {
+  FwdGeometry g;
+  if (!fwd_geometry(d, &g)) return 0;
+  SegPlan p;
+  if (seg_plan(d, saved, &p)) return p.total > g.total ? p.total : g.total;
+  return g.total;
+}
+
+// The device check (gate), the warm-up pass over the last
+// TSIIR_SEGMENT_WARMUP samples of every segment (its end states are the next
+// segments' incoming states), the incoming states assembled, the main pass
+// over all segments as rows, and zf taken from the last segments; every
+// launch after the check runs only while the gate is set.
+static tsiir_status forward_segmented(const tsiir_sosfilt_desc* d, const SegPlan& p,
+                                      const void* x, void* y, const void* sos, const void* zi,
+                                      void* zf, char* ws, hipStream_t st, int64_t n_filters) {
+  const int S = d->n_sections;
+  WsHeader* hdr = reinterpret_cast<WsHeader*>(ws);
+  const uint32_t* gate = &hdr->seg_gate;
+  const int single = n_filters == 1 ? 1 : 0;
+  if (!single) hipLaunchKernelGGL(seg_gate_init_kernel, dim3(1), dim3(1), 0, st, hdr);
+  if (d->coef_dtype == TSIIR_F64)
+    hipLaunchKernelGGL(seg_check_kernel<double>, dim3(static_cast<unsigned>(n_filters)),
+                       dim3(kSegCheckThreads), 0, st, static_cast<const double*>(sos), S,
+                       kSegSquarings, d->coef_dtype, single, hdr);
+  else
+    hipLaunchKernelGGL(seg_check_kernel<float>, dim3(static_cast<unsigned>(n_filters)),
+                       dim3(kSegCheckThreads), 0, st, static_cast<const float*>(sos), S,
+                       kSegSquarings, d->coef_dtype, single, hdr);
+  if (hipGetLastError() != hipSuccess) return TSIIR_ELAUNCH;
+  const FwdGate on{gate, 1u};
+  double* warm = reinterpret_cast<double*>(ws + p.warm_off);
+  double* zin = reinterpret_cast<double*>(ws + p.zin_off);
+  double* zall = reinterpret_cast<double*>(ws + p.zall_off);
+  const char* xw = static_cast<const char*>(x) +
+                   (p.lseg - TSIIR_SEGMENT_WARMUP) * static_cast<int64_t>(io_bytes(d->io_dtype));
+  tsiir_status rc = forward_impl(&p.warm, xw, ws + p.ywarm_off, sos, nullptr, warm, ws, p.base,
+                                 st, nullptr, &on);
+  if (rc != TSIIR_OK) return rc;
+  const bool has_zi = (d->flags & TSIIR_HAS_ZI) != 0;
+  const int bc = (d->flags & TSIIR_ZI_BROADCAST) ? 1 : 0;
+  const bool sd = d->io_dtype == TSIIR_F64 || (d->flags & TSIIR_STATE_F64);
+  const int64_t n_in = p.rows * S * 2;
+  const unsigned bi = static_cast<unsigned>((n_in + 255) / 256);
+  if (sd)
+    hipLaunchKernelGGL(seg_state_in_kernel<double>, dim3(bi), dim3(256), 0, st, gate, warm,
+                       has_zi ? static_cast<const double*>(zi) : nullptr, bc, zin, p.rows,
+                       p.nseg, S);
+  else
+    hipLaunchKernelGGL(seg_state_in_kernel<float>, dim3(bi), dim3(256), 0, st, gate, warm,
+                       has_zi ? static_cast<const float*>(zi) : nullptr, bc, zin, p.rows,
+                       p.nseg, S);
+  if (hipGetLastError() != hipSuccess) return TSIIR_ELAUNCH;
+  const bool want_zf = (d->flags & TSIIR_WRITE_ZF) != 0;
+  rc = forward_impl(&p.main, x, y, sos, zin, want_zf ? zall : nullptr, ws, p.base, st, nullptr,
+                    &on);
+  if (rc != TSIIR_OK || !want_zf) return rc;
+  const int64_t n_out = d->channels * S * 2;
+  const unsigned bo = static_cast<unsigned>((n_out + 255) / 256);
+  if (sd)
+    hipLaunchKernelGGL(seg_state_out_kernel<double>, dim3(bo), dim3(256), 0, st, gate, zall,
+                       static_cast<double*>(zf), d->channels, p.nseg, S);
+  else
+    hipLaunchKernelGGL(seg_state_out_kernel<float>, dim3(bo), dim3(256), 0, st, gate, zall,
+                       static_cast<float*>(zf), d->channels, p.nseg, S);
+  return hipGetLastError() == hipSuccess ? TSIIR_OK : TSIIR_ELAUNCH;
+}
+
 tsiir_status forward_impl(const tsiir_sosfilt_desc* d, const void* x, void* y, const void* sos,
                           const void* zi, void* zf, void* workspace, size_t workspace_bytes,
-                          void* stream, void* saved) {
+                          void* stream, void* saved, const FwdGate* gate) {
   FwdGeometry g;
   if (!fwd_geometry(d, &g)) return TSIIR_EINVAL;
   if (d->io_dtype != TSIIR_F32 && d->io_dtype != TSIIR_F64 && d->io_dtype != TSIIR_BF16)
@@ -848,6 +1135,17 @@ tsiir_status forward_impl(const tsiir_sosfilt_desc* d, const void* x, void* y, c
   if (!workspace || workspace_bytes < g.total) return TSIIR_EWORKSPACE;
   if (g.n_windows * d->channels > 0x7fffffffLL) return TSIIR_EUNSUPPORTED;
   char* ws = static_cast<char*>(workspace);
+  FwdGate off{nullptr, 0u};
+  SegPlan sp;
+  if (gate == nullptr && seg_plan(d, saved != nullptr, &sp)) {
+    if (workspace_bytes < sp.total) return TSIIR_EWORKSPACE;
+    const tsiir_status rc =
+        forward_segmented(d, sp, x, y, sos, zi, zf, ws, st, g.n_filters);
+    if (rc != TSIIR_OK) return rc;
+    // the look-back form runs only where the check cleared the gate
+    off = FwdGate{&reinterpret_cast<WsHeader*>(ws)->seg_gate, 0u};
+    gate = &off;
+  }
   const bool comp = d->mode == TSIIR_MODE_COMPENSATED;
   if (use_chan(d, saved)) {
     const bool w64 = chan_w_f32() == 64;
@@ -866,18 +1164,18 @@ tsiir_status forward_impl(const tsiir_sosfilt_desc* d, const void* x, void* y, c
   }
   switch (d->io_dtype) {
     case TSIIR_F32:
-      if (!comp) return launch_fwd<float, float, float, 64>(d, g, x, y, sos, zi, zf, ws, st, saved);
+      if (!comp) return launch_fwd<float, float, float, 64>(d, g, x, y, sos, zi, zf, ws, st, saved, gate);
       return g.W == 64
-                 ? launch_fwd<float, float, double, 64>(d, g, x, y, sos, zi, zf, ws, st, saved)
-                 : launch_fwd<float, float, double, 32>(d, g, x, y, sos, zi, zf, ws, st, saved);
+                 ? launch_fwd<float, float, double, 64>(d, g, x, y, sos, zi, zf, ws, st, saved, gate)
+                 : launch_fwd<float, float, double, 32>(d, g, x, y, sos, zi, zf, ws, st, saved, gate);
     case TSIIR_BF16:
       if (!comp)
-        return launch_fwd<bf16_t, float, float, 64>(d, g, x, y, sos, zi, zf, ws, st, saved);
+        return launch_fwd<bf16_t, float, float, 64>(d, g, x, y, sos, zi, zf, ws, st, saved, gate);
       return g.W == 64
-                 ? launch_fwd<bf16_t, float, double, 64>(d, g, x, y, sos, zi, zf, ws, st, saved)
-                 : launch_fwd<bf16_t, float, double, 32>(d, g, x, y, sos, zi, zf, ws, st, saved);
+                 ? launch_fwd<bf16_t, float, double, 64>(d, g, x, y, sos, zi, zf, ws, st, saved, gate)
+                 : launch_fwd<bf16_t, float, double, 32>(d, g, x, y, sos, zi, zf, ws, st, saved, gate);
     default:
-      return launch_fwd<double, double, double, 32>(d, g, x, y, sos, zi, zf, ws, st, saved);
+      return launch_fwd<double, double, double, 32>(d, g, x, y, sos, zi, zf, ws, st, saved, gate);
   }
 }
 
@@ -905,9 +1203,7 @@ const char* tsiir_status_string(tsiir_status s) {
 }
 
 size_t tsiir_sosfilt_workspace_bytes(const tsiir_sosfilt_desc* d) {
-  FwdGeometry g;
-  if (!fwd_geometry(d, &g)) return 0;
-  return g.total;
+  return fwd_workspace_bytes(d, false);
 }
 
 tsiir_status tsiir_workspace_init(void* workspace, size_t bytes, void* stream) {

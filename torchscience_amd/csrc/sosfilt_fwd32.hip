@@ -729,6 +729,7 @@ __global__ __launch_bounds__(64 * NWV) void sos_fwd32_kernel(FwdArgs a,
   constexpr int kSmemV = 10240 * NWV / 16;
   constexpr int kAuxV = (2 * kWave * 8 + kWave * 4 + 2 * NWV * 2 * 8 + 2 * 2 * 8) / 16;
   static_assert(NWV * kStageV + kAuxV <= kSmemV, "shared memory layout");
+  if (gated_out(a)) return;
   __shared__ u32x4 smem[kSmemV];
   const int S = a.S;
   Win32<NWV> w;
@@ -779,7 +780,9 @@ __global__ __launch_bounds__(256) void sos_prep32_kernel(const TC* __restrict__ 
                                                          int64_t n_rows, int S,
                                                          double* __restrict__ tables,
                                                          Fast32* __restrict__ fast,
-                                                         WsHeader* hdr) {
+                                                         WsHeader* hdr, const uint32_t* gate,
+                                                         uint32_t gate_on) {
+  if (gate != nullptr && ((*gate != 0u) != (gate_on != 0u))) return;
   const int64_t i =
       static_cast<int64_t>(blockIdx.x) * kPrepRowsPerBlock + (threadIdx.x / kPrepLanes);
   const int l = threadIdx.x % kPrepLanes;
@@ -945,10 +948,12 @@ tsiir_status launch_fwd32_full(const FwdArgs& a, const void* sos, int coef_dtype
   double* tables = const_cast<double*>(a.tables);
   if (coef_dtype == TSIIR_F64)
     hipLaunchKernelGGL(sos_prep32_kernel<double>, dim3(pblocks), dim3(256), 0, stream,
-                       static_cast<const double*>(sos), n_rows, a.S, tables, fast, a.hdr);
+                       static_cast<const double*>(sos), n_rows, a.S, tables, fast, a.hdr,
+                       a.gate, a.gate_on);
   else
     hipLaunchKernelGGL(sos_prep32_kernel<float>, dim3(pblocks), dim3(256), 0, stream,
-                       static_cast<const float*>(sos), n_rows, a.S, tables, fast, a.hdr);
+                       static_cast<const float*>(sos), n_rows, a.S, tables, fast, a.hdr,
+                       a.gate, a.gate_on);
   const bool rev = (a.flags & TSIIR_REVERSE) != 0;
   if (a.io_dtype == TSIIR_BF16) {
     if (rev)

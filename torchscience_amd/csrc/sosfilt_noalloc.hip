@@ -85,7 +85,7 @@ tsiir_status tsiir_sosfilt_forward_noalloc(const tsiir_sosfilt_desc* desc, const
                                            void* stream) {
   FwdGeometry g;
   if (!fwd_geometry(desc, &g)) return TSIIR_EINVAL;
-  if (g.total > kScratchBytes) return TSIIR_ESCRATCH;
+  if (fwd_workspace_bytes(desc, false) > kScratchBytes) return TSIIR_ESCRATCH;
   void* ws = nullptr;
   const tsiir_status st = scratch_for_device(static_cast<hipStream_t>(stream), &ws);
   if (st != TSIIR_OK) return st;

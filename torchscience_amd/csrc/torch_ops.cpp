@@ -169,8 +169,8 @@ bool needs_fp32_df2t(const at::Tensor& sos) {
 
 using Fwd = std::tuple<at::Tensor, at::Tensor, at::Tensor>;
 
-constexpr uint32_t kCallerFlags =
-    TSIIR_PATH_WINDOW | TSIIR_PATH_CHANNEL | TSIIR_REVERSE | TSIIR_DEBUG_GIVE_UP;
+constexpr uint32_t kCallerFlags = TSIIR_PATH_WINDOW | TSIIR_PATH_CHANNEL | TSIIR_REVERSE |
+                                  TSIIR_DEBUG_GIVE_UP | TSIIR_NO_SEGMENT;
 
 void check_rows(const at::Tensor& x, const at::Tensor& sos, int64_t channels,
                 int64_t chans_per_filter) {

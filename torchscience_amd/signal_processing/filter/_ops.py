@@ -64,7 +64,8 @@ def sosfilt_rows_forward(sos: torch.Tensor, x: torch.Tensor, zi: torch.Tensor | 
 
     Channel c reads x row c % x.shape[0] and filter c // chans_per_filter.
     path: None (chosen from the shape), "window" or "channel" (forces the
-    work decomposition; results agree to rounding).  keep_sections: also
+    work decomposition; results agree to rounding), or "lookback" (the
+    window decomposition without the segmented form for few long rows).  keep_sections: also
     return the outputs of sections 0..S-2, (S-1, channels, T), for the
     backward (else None).  reverse: filter along decreasing time (input read
     and output written from the end; zi is the state at the end).
@@ -75,7 +76,8 @@ def sosfilt_rows_forward(sos: torch.Tensor, x: torch.Tensor, zi: torch.Tensor | 
         x = x.to(io)
     flags = 0
     if path is not None:
-        flags |= {"window": _lib.PATH_WINDOW, "channel": _lib.PATH_CHANNEL}[path]
+        flags |= {"window": _lib.PATH_WINDOW, "channel": _lib.PATH_CHANNEL,
+                  "lookback": _lib.PATH_WINDOW | _lib.NO_SEGMENT}[path]
     if reverse:
         flags |= _lib.REVERSE
     if _debug_give_up:
