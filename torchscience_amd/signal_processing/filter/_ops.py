@@ -115,6 +115,8 @@ class SosfiltRows(torch.autograd.Function):
                                             zi_broadcast=zi_broadcast, keep_sections=True)
         ctx.save_for_backward(x, sos, zi, y, saved)
         ctx.meta = (channels, chans_per_filter, zi_broadcast, want_zf)
+        # an unused zf passes no gradient (None) rather than a filled zero tensor
+        ctx.set_materialize_grads(False)
         if want_zf:
             return y, zf
         return y, None
