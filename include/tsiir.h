@@ -93,9 +93,9 @@ typedef enum tsiir_mode {
 /* zi / zf hold FP64 values whatever io_dtype is (a caller that carries a
  * stream's state between chunks in FP64 with FP32 or BF16 samples). */
 #define TSIIR_STATE_F64 0x200u
-/* Few channels of long rows (fewer than 1024 channels, contiguous rows,
- * rows of 2^k * 16384 samples with k large enough for 1024 segments, at most
- * 16 sections, no REVERSE and no saved section outputs) take a segmented
+/* Few channels of long rows (fewer than 1024 channels, one input row per
+ * channel, rows of at least 1024 / channels segments of 16384 samples, at
+ * most 16 sections, no saved section outputs) take a segmented
  * form when every filter forgets its state within TSIIR_SEGMENT_WARMUP
  * samples: the infinity norm of Phi^4096 below 2^-64, Phi being the 2S x 2S
  * state-transition matrix of the cascade in DF2T form, checked on the device

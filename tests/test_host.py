@@ -73,11 +73,11 @@ def test_workspace_size_query_without_gpu():
 def test_segmented_workspace_plan_without_gpu():
     """Few long rows (8 x 2^24, the long-sequence shape) size the workspace
     for the segmented form as well: 1024 segments of 131072 samples (the
-    longest power of two that still gives 1024 rows), the many-channel
-    tables and status words of the 131072-sample main pass, the
-    4096-sample warm-up outputs and three FP64 state arrays; TSIIR_NO_SEGMENT,
-    REVERSE, 1024 channels or a row length with no such segment size keep
-    the look-back form's size."""
+    longest power of two that still gives 1024 rows), the largest of the
+    warm-up, main and remainder passes' own workspaces, the 4096-sample
+    warm-up outputs, three FP64 state arrays over the segments and two over
+    the channels; TSIIR_NO_SEGMENT, 1024 channels or rows too short for
+    1024 segments keep the look-back form's size."""
     L = _lib.load()
 
     def ws(B, T, flags=0):
@@ -94,17 +94,25 @@ def test_segmented_workspace_plan_without_gpu():
         # (4096-sample window, row, section), one 1088-byte FP32 entry per section
         return align(1024 + 5376 + (-(-T // 4096)) * B * 6 * 64) + 6 * 1088
 
+    def seg(B, T, rows, lseg, wrows, rem):
+        base = max(plain(wrows, 4096), plain(rows, lseg), plain(B, rem) if rem else 0)
+        states, cstates = rows * 6 * 2 * 8, B * 6 * 2 * 8
+        off = align(align(base) + rows * 4096 * 4)       # warm-up outputs
+        off = align(align(align(off + states) + states) + states)  # warm, zin, zall
+        return align(off + cstates) + cstates             # remainder's zi, zf
+
     B, T = 8, 1 << 24
-    rows, lseg = 1024, 131072
-    base = align(max(plain(rows, 4096), plain(rows, lseg)))
-    states = rows * 6 * 2 * 8
-    seg = align(align(align(base + rows * 4096 * 4) + states) + states)
     assert ws(B, T, _lib.NO_SEGMENT) == plain(B, T)
-    assert ws(B, T) == max(seg, plain(B, T))
-    assert ws(B, T, _lib.WRITE_ZF) == max(seg + states, plain(B, T))
-    assert ws(B, T, _lib.REVERSE) == plain(B, T)
+    assert ws(B, T) == max(seg(B, T, 1024, 131072, 1024, 0), plain(B, T))
+    assert ws(B, T, _lib.WRITE_ZF) == ws(B, T)
+    # REVERSE: no warm-up after the very last segment
+    assert ws(B, T, _lib.REVERSE) == max(seg(B, T, 1024, 131072, 1023, 0), plain(B, T))
+    # a remainder: 8 x (2^21 + 78) splits into 128 segments of 16384 and 78 samples
+    T2 = (1 << 21) + 78
+    assert ws(B, T2) == max(seg(B, T2, 1024, 16384, 1024, 78), plain(B, T2))
     assert ws(1024, 65536) == plain(1024, 65536)
-    assert ws(8, 3 * 5 * 7 * 16384 + 4096) == plain(8, 3 * 5 * 7 * 16384 + 4096)
+    # too short for 1024 segments of at least 16384 samples
+    assert ws(8, 105 * 16384 + 4096) == plain(8, 105 * 16384 + 4096)
 
 
 def test_fir_fft_host_checks_without_gpu():

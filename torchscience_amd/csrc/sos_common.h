@@ -156,7 +156,19 @@ struct FwdArgs {
   // check picks which one works (every kernel of the other exits at once).
   const uint32_t* gate = nullptr;
   uint32_t gate_on = 0;
+  // Two-level rows (the segmented form): with seg_n > 0, row r is segment
+  // r % seg_n of channel r / seg_n, at (r / seg_n) * stride + (r % seg_n) *
+  // seg_len elements; a zero seg_len keeps plain rows (r * stride).
+  int64_t seg_n = 0;
+  int64_t x_seg_len = 0;
+  int64_t y_seg_len = 0;
 };
+
+__device__ __forceinline__ int64_t row_offset(const FwdArgs& a, int64_t r, int64_t stride,
+                                              int64_t seg_len) {
+  return (a.seg_n > 0 && seg_len > 0) ? (r / a.seg_n) * stride + (r % a.seg_n) * seg_len
+                                      : r * stride;
+}
 
 __device__ __forceinline__ bool gated_out(const FwdArgs& a) {
   return a.gate != nullptr && ((*a.gate != 0u) != (a.gate_on != 0u));

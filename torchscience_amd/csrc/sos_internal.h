@@ -40,10 +40,15 @@ void launch_prep(const void* sos, int coef_dtype, int64_t n_rows, int log2w, boo
 
 // tsiir_sosfilt_forward, optionally also writing the outputs of sections
 // 0..S-2 to `saved` ((S-1, channels, T) in the I/O type).
-// A device word that decides whether a launch runs (see FwdArgs::gate).
+// Launch details of a forward that the descriptor does not carry: a device
+// word that decides whether its kernels run (see FwdArgs::gate), and the
+// two-level row addressing of the segmented form (FwdArgs::seg_n).
 struct FwdGate {
   const uint32_t* word;
   uint32_t on;
+  int64_t seg_n = 0;
+  int64_t x_seg_len = 0;
+  int64_t y_seg_len = 0;
 };
 
 tsiir_status forward_impl(const tsiir_sosfilt_desc* d, const void* x, void* y, const void* sos,

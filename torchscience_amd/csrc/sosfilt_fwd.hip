@@ -140,8 +140,9 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
     m = rem <= 0 ? 0 : (rem >= W ? W : static_cast<int>(rem));
   }
 
-  const TIO* xr = static_cast<const TIO*>(a.x) + (c % a.x_rows) * a.x_stride;
-  TIO* yr = static_cast<TIO*>(a.y) + c * a.y_stride;
+  const TIO* xr =
+      static_cast<const TIO*>(a.x) + row_offset(a, c % a.x_rows, a.x_stride, a.x_seg_len);
+  TIO* yr = static_cast<TIO*>(a.y) + row_offset(a, c, a.y_stride, a.y_seg_len);
   // REVERSE: processing index t maps to time T-1-t (input and output), so
   // the wave's samples are the time-ordered run ending at T-1-t0w.
   const bool rev = (a.flags & TSIIR_REVERSE) != 0;
@@ -622,13 +623,17 @@ __global__ void copy_state_kernel(const TACC* zi, TACC* zf, int64_t B, int S, in
   zf[i] = zi ? zi[(bcast ? 0 : row) * S * 2 + within] : TACC(0);
 }
 
-// Segmented rows: row r = c * nseg + k is segment k of channel c.  Its
-// incoming state is channel c's zi (or zeros) for k = 0, else the warm-up end
-// state of row r - 1 (FP64 throughout).  Runs only while the gate is set.
+// Segmented rows: row r = c * nseg + k is segment k (in time) of channel c.
+// Its incoming state is channel c's zi (or zeros) for the segment processed
+// first (k = 0; with `rev`, along decreasing time, k = nseg - 1), else the
+// warm-up end state computed for it: row r - 1's entry (rev: row r's, whose
+// warm-up ran over the start of segment r + 1).  FP64 throughout; runs only
+// while the gate is set.
 template <typename TS>
 __global__ void seg_state_in_kernel(const uint32_t* gate, const double* __restrict__ warm,
                                     const TS* __restrict__ zi, int bcast,
-                                    double* __restrict__ zin, int64_t rows, int64_t nseg, int S) {
+                                    double* __restrict__ zin, int64_t rows, int64_t nseg, int S,
+                                    int rev) {
   if (*gate == 0u) return;
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t per = static_cast<int64_t>(S) * 2;
@@ -636,24 +641,25 @@ __global__ void seg_state_in_kernel(const uint32_t* gate, const double* __restri
   const int64_t r = i / per, within = i % per;
   const int64_t k = r % nseg, c = r / nseg;
   double v;
-  if (k != 0)
-    v = warm[(r - 1) * per + within];
+  if (k != (rev ? nseg - 1 : 0))
+    v = warm[(rev ? r : r - 1) * per + within];
   else
     v = zi ? static_cast<double>(zi[(bcast ? 0 : c) * per + within]) : 0.0;
   zin[i] = v;
 }
 
-// zf of channel c = the final state of its last segment, in the caller's
-// state type.
+// zf of channel c = the final state of its segment processed last (the
+// last in time; with `rev` the first), in the caller's state type.
 template <typename TS>
 __global__ void seg_state_out_kernel(const uint32_t* gate, const double* __restrict__ zall,
-                                     TS* __restrict__ zf, int64_t channels, int64_t nseg, int S) {
+                                     TS* __restrict__ zf, int64_t channels, int64_t nseg, int S,
+                                     int rev) {
   if (*gate == 0u) return;
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t per = static_cast<int64_t>(S) * 2;
   if (i >= channels * per) return;
   const int64_t c = i / per, within = i % per;
-  zf[i] = static_cast<TS>(zall[(c * nseg + nseg - 1) * per + within]);
+  zf[i] = static_cast<TS>(zall[(c * nseg + (rev ? 0 : nseg - 1)) * per + within]);
 }
 
 __global__ void seg_gate_init_kernel(WsHeader* hdr) { hdr->seg_gate = 1u; }
@@ -842,6 +848,9 @@ static tsiir_status launch_fwd(const tsiir_sosfilt_desc* d, const FwdGeometry& g
   a.io_dtype = d->io_dtype;
   a.gate = gate ? gate->word : nullptr;
   a.gate_on = gate ? gate->on : 0u;
+  a.seg_n = gate ? gate->seg_n : 0;
+  a.x_seg_len = gate ? gate->x_seg_len : 0;
+  a.y_seg_len = gate ? gate->y_seg_len : 0;
   const FwdPolicy pol = fwd_policy(d);
   if constexpr (std::is_same<TR, float>::value && W == 64) {
     if (use_fwd32(d)) {
@@ -971,56 +980,85 @@ constexpr int kSegSquarings = 12;       // 2^12 = TSIIR_SEGMENT_WARMUP
 static_assert((int64_t{1} << kSegSquarings) == TSIIR_SEGMENT_WARMUP, "warm-up length");
 
 struct SegPlan {
-  int64_t lseg = 0, nseg = 0, rows = 0;
-  tsiir_sosfilt_desc warm{}, main{};
-  size_t base = 0, ywarm_off = 0, warm_off = 0, zin_off = 0, zall_off = 0, total = 0;
+  int64_t lseg = 0, nseg = 0, rows = 0, rem = 0;
+  bool rev = false;
+  tsiir_sosfilt_desc warm{}, main{}, tail{};
+  size_t base = 0, ywarm_off = 0, warm_off = 0, zin_off = 0, zall_off = 0, ztin_off = 0,
+         ztout_off = 0, total = 0;
 };
 
 static size_t io_bytes(int32_t dt) { return dt == TSIIR_F64 ? 8 : dt == TSIIR_BF16 ? 2 : 4; }
 
-// The longest power-of-two segment that divides the row and still gives the
-// many-channel kernels at least kSegRows rows; false where the form does not
-// apply.
+// Each channel's row splits into nseg segments of lseg samples (the longest
+// power of two that still gives the many-channel kernels at least kSegRows
+// of them) and a remainder of rem < lseg samples, which runs last (after
+// the last segment; with REVERSE, time runs backwards and the remainder is
+// the start of the row) from the state the segments end in.  False where
+// the form does not apply.
 static bool seg_plan(const tsiir_sosfilt_desc* d, bool saved, SegPlan* p) {
   if ((d->flags & TSIIR_NO_SEGMENT) || saved || use_chan(d, nullptr)) return false;
-  if (d->flags & (TSIIR_REVERSE | TSIIR_PATH_CHANNEL | TSIIR_DEBUG_GIVE_UP)) return false;
+  if (d->flags & (TSIIR_PATH_CHANNEL | TSIIR_DEBUG_GIVE_UP)) return false;
   if (d->n_sections > 16) return false;
   const int64_t B = d->channels, T = d->n_samples;
-  if (B < 1 || B >= kSegRows || T < 2 * kSegMinLen) return false;
-  if (d->x_rows != B || (B > 1 && (d->x_row_stride != T || d->y_row_stride != T))) return false;
+  if (B < 1 || B >= kSegRows || T < 2 * kSegMinLen || d->x_rows != B) return false;
   const int64_t nmin = (kSegRows + B - 1) / B;
   int64_t lseg = 0;
   for (int64_t l = kSegMinLen; l <= T / 2; l *= 2)
-    if (T % l == 0 && T / l >= nmin) lseg = l;
+    if (T / l >= nmin) lseg = l;
   if (lseg == 0) return false;
   p->lseg = lseg;
   p->nseg = T / lseg;
+  p->rem = T - p->nseg * lseg;
   p->rows = B * p->nseg;
-  const uint32_t keep = d->flags & (TSIIR_FP32_DF2T | TSIIR_PATH_WINDOW);
+  p->rev = (d->flags & TSIIR_REVERSE) != 0;
+  const bool want_zf = (d->flags & TSIIR_WRITE_ZF) != 0;
+  const bool shared = d->chans_per_filter >= B;
+  // channel strides (any value serves one channel)
+  const int64_t xs = B > 1 ? d->x_row_stride : T, ys = B > 1 ? d->y_row_stride : T;
+  const uint32_t keep = d->flags & (TSIIR_FP32_DF2T | TSIIR_PATH_WINDOW | TSIIR_REVERSE);
+  // the warm-up rows: one per segment, over the TSIIR_SEGMENT_WARMUP samples
+  // before it (rev: after it, so there is none after the very last segment)
+  const int64_t wrows = p->rev ? p->rows - 1 : p->rows;
   tsiir_sosfilt_desc w = *d;
-  w.channels = p->rows;
+  w.channels = wrows;
   w.n_samples = TSIIR_SEGMENT_WARMUP;
-  w.x_rows = p->rows;
-  w.x_row_stride = lseg;
+  w.x_rows = wrows;
+  w.x_row_stride = xs;
   w.y_row_stride = TSIIR_SEGMENT_WARMUP;
-  w.chans_per_filter = d->chans_per_filter >= B ? p->rows : d->chans_per_filter * p->nseg;
+  w.chans_per_filter = shared ? wrows : d->chans_per_filter * p->nseg;
   w.flags = keep | TSIIR_WRITE_ZF | TSIIR_STATE_F64;
-  tsiir_sosfilt_desc m = w;
+  tsiir_sosfilt_desc m = *d;
+  m.channels = p->rows;
   m.n_samples = lseg;
-  m.y_row_stride = lseg;
-  m.flags = keep | TSIIR_HAS_ZI | TSIIR_STATE_F64 | (d->flags & TSIIR_WRITE_ZF);
+  m.x_rows = p->rows;
+  m.x_row_stride = xs;
+  m.y_row_stride = ys;
+  m.chans_per_filter = shared ? p->rows : d->chans_per_filter * p->nseg;
+  m.flags = keep | TSIIR_HAS_ZI | TSIIR_STATE_F64 |
+            ((want_zf || p->rem > 0) ? TSIIR_WRITE_ZF : 0u);
+  tsiir_sosfilt_desc t = *d;
+  t.n_samples = p->rem;
+  t.x_row_stride = xs;
+  t.y_row_stride = ys;
+  t.flags = keep | TSIIR_HAS_ZI | TSIIR_STATE_F64 | (want_zf ? TSIIR_WRITE_ZF : 0u);
   p->warm = w;
   p->main = m;
-  FwdGeometry gw, gm;
-  if (!fwd_geometry(&w, &gw) || !fwd_geometry(&m, &gm)) return false;
+  p->tail = t;
+  FwdGeometry gw, gm, gt;
+  if (!fwd_geometry(&w, &gw) || !fwd_geometry(&m, &gm) || !fwd_geometry(&t, &gt)) return false;
+  size_t base = gw.total > gm.total ? gw.total : gm.total;
+  if (p->rem > 0 && gt.total > base) base = gt.total;
   const size_t states = static_cast<size_t>(p->rows) * d->n_sections * 2 * sizeof(double);
-  p->base = align_up(gw.total > gm.total ? gw.total : gm.total, 256);
+  const size_t cstates = static_cast<size_t>(B) * d->n_sections * 2 * sizeof(double);
+  p->base = align_up(base, 256);
   p->ywarm_off = p->base;
   p->warm_off = align_up(p->ywarm_off + static_cast<size_t>(p->rows) * TSIIR_SEGMENT_WARMUP *
                                             io_bytes(d->io_dtype), 256);
   p->zin_off = align_up(p->warm_off + states, 256);
   p->zall_off = align_up(p->zin_off + states, 256);
-  p->total = p->zall_off + ((d->flags & TSIIR_WRITE_ZF) ? states : 0);
+  p->ztin_off = align_up(p->zall_off + states, 256);
+  p->ztout_off = align_up(p->ztin_off + cstates, 256);
+  p->total = p->ztout_off + cstates;
   return true;
 }
 
@@ -1035,11 +1073,12 @@ size_t fwd_workspace_bytes(const tsiir_sosfilt_desc* d, bool saved) {
   return g.total;
 }
 
-// The device check (gate), the warm-up pass over the last
-// TSIIR_SEGMENT_WARMUP samples of every segment (its end states are the next
-// segments' incoming states), the incoming states assembled, the main pass
-// over all segments as rows, and zf taken from the last segments; every
-// launch after the check runs only while the gate is set.
+// The device check (gate); the warm-up pass over the TSIIR_SEGMENT_WARMUP
+// samples before every segment (its end states are the segments' incoming
+// states); the incoming states assembled; the main pass over all segments
+// as two-level rows; then the remainder of each row from the state its
+// segments end in, and zf.  Every launch after the check runs only while
+// the gate is set.
 static tsiir_status forward_segmented(const tsiir_sosfilt_desc* d, const SegPlan& p,
                                       const void* x, void* y, const void* sos, const void* zi,
                                       void* zf, char* ws, hipStream_t st, int64_t n_filters) {
@@ -1057,41 +1096,69 @@ static tsiir_status forward_segmented(const tsiir_sosfilt_desc* d, const SegPlan
                        dim3(kSegCheckThreads), 0, st, static_cast<const float*>(sos), S,
                        kSegSquarings, d->coef_dtype, single, hdr);
   if (hipGetLastError() != hipSuccess) return TSIIR_ELAUNCH;
-  const FwdGate on{gate, 1u};
+  const int64_t esz = static_cast<int64_t>(io_bytes(d->io_dtype));
   double* warm = reinterpret_cast<double*>(ws + p.warm_off);
   double* zin = reinterpret_cast<double*>(ws + p.zin_off);
   double* zall = reinterpret_cast<double*>(ws + p.zall_off);
-  const char* xw = static_cast<const char*>(x) +
-                   (p.lseg - TSIIR_SEGMENT_WARMUP) * static_cast<int64_t>(io_bytes(d->io_dtype));
-  tsiir_status rc = forward_impl(&p.warm, xw, ws + p.ywarm_off, sos, nullptr, warm, ws, p.base,
-                                 st, nullptr, &on);
+  double* ztin = reinterpret_cast<double*>(ws + p.ztin_off);
+  double* ztout = reinterpret_cast<double*>(ws + p.ztout_off);
+  const char* xc = static_cast<const char*>(x);
+  char* yc = static_cast<char*>(y);
+  // the segments start at sample 0 (rev: at the remainder's end)
+  const int64_t seg0 = p.rev ? p.rem : 0;
+  // warm-up rows start TSIIR_SEGMENT_WARMUP samples before each segment's
+  // end (rev: at the start of the segment after it)
+  const int64_t wofs = seg0 + (p.rev ? p.lseg : p.lseg - TSIIR_SEGMENT_WARMUP);
+  const FwdGate on_warm{gate, 1u, p.nseg, p.lseg, 0};
+  tsiir_status rc = forward_impl(&p.warm, xc + wofs * esz, ws + p.ywarm_off, sos, nullptr, warm,
+                                 ws, p.base, st, nullptr, &on_warm);
   if (rc != TSIIR_OK) return rc;
   const bool has_zi = (d->flags & TSIIR_HAS_ZI) != 0;
   const int bc = (d->flags & TSIIR_ZI_BROADCAST) ? 1 : 0;
+  const int rev = p.rev ? 1 : 0;
   const bool sd = d->io_dtype == TSIIR_F64 || (d->flags & TSIIR_STATE_F64);
   const int64_t n_in = p.rows * S * 2;
   const unsigned bi = static_cast<unsigned>((n_in + 255) / 256);
   if (sd)
     hipLaunchKernelGGL(seg_state_in_kernel<double>, dim3(bi), dim3(256), 0, st, gate, warm,
                        has_zi ? static_cast<const double*>(zi) : nullptr, bc, zin, p.rows,
-                       p.nseg, S);
+                       p.nseg, S, rev);
   else
     hipLaunchKernelGGL(seg_state_in_kernel<float>, dim3(bi), dim3(256), 0, st, gate, warm,
                        has_zi ? static_cast<const float*>(zi) : nullptr, bc, zin, p.rows,
-                       p.nseg, S);
+                       p.nseg, S, rev);
   if (hipGetLastError() != hipSuccess) return TSIIR_ELAUNCH;
   const bool want_zf = (d->flags & TSIIR_WRITE_ZF) != 0;
-  rc = forward_impl(&p.main, x, y, sos, zin, want_zf ? zall : nullptr, ws, p.base, st, nullptr,
-                    &on);
-  if (rc != TSIIR_OK || !want_zf) return rc;
+  const FwdGate on_main{gate, 1u, p.nseg, p.lseg, p.lseg};
+  rc = forward_impl(&p.main, xc + seg0 * esz, yc + seg0 * esz, sos, zin,
+                    (want_zf || p.rem > 0) ? zall : nullptr, ws, p.base, st, nullptr, &on_main);
+  if (rc != TSIIR_OK) return rc;
   const int64_t n_out = d->channels * S * 2;
   const unsigned bo = static_cast<unsigned>((n_out + 255) / 256);
-  if (sd)
+  const double* zlast = zall;  // the state the segments end in, per channel
+  int64_t zn = p.nseg;
+  int zrev = rev;
+  if (p.rem > 0) {
+    // the remainder: after the segments (rev: the start of the row, last)
     hipLaunchKernelGGL(seg_state_out_kernel<double>, dim3(bo), dim3(256), 0, st, gate, zall,
-                       static_cast<double*>(zf), d->channels, p.nseg, S);
+                       ztin, d->channels, p.nseg, S, rev);
+    if (hipGetLastError() != hipSuccess) return TSIIR_ELAUNCH;
+    const int64_t t0 = p.rev ? 0 : p.nseg * p.lseg;
+    const FwdGate on_tail{gate, 1u, 0, 0, 0};
+    rc = forward_impl(&p.tail, xc + t0 * esz, yc + t0 * esz, sos, ztin, want_zf ? ztout : nullptr,
+                      ws, p.base, st, nullptr, &on_tail);
+    if (rc != TSIIR_OK) return rc;
+    zlast = ztout;
+    zn = 1;
+    zrev = 0;
+  }
+  if (!want_zf) return TSIIR_OK;
+  if (sd)
+    hipLaunchKernelGGL(seg_state_out_kernel<double>, dim3(bo), dim3(256), 0, st, gate, zlast,
+                       static_cast<double*>(zf), d->channels, zn, S, zrev);
   else
-    hipLaunchKernelGGL(seg_state_out_kernel<float>, dim3(bo), dim3(256), 0, st, gate, zall,
-                       static_cast<float*>(zf), d->channels, p.nseg, S);
+    hipLaunchKernelGGL(seg_state_out_kernel<float>, dim3(bo), dim3(256), 0, st, gate, zlast,
+                       static_cast<float*>(zf), d->channels, zn, S, zrev);
   return hipGetLastError() == hipSuccess ? TSIIR_OK : TSIIR_ELAUNCH;
 }
 

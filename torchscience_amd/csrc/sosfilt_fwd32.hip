@@ -643,8 +643,9 @@ __device__ __forceinline__ void finish_cascade(const FwdArgs& a, const Win32<NWV
 template <typename TIO, bool REV, bool SAVED, bool UNIT, int NWV, bool LB>
 __device__ __forceinline__ void window32(const FwdArgs& a, Win32<NWV>& w, u32x4* stage) {
   const int lane = lane_now();
-  const TIO* xr = static_cast<const TIO*>(a.x) + (w.c % a.x_rows) * a.x_stride;
-  TIO* yr = static_cast<TIO*>(a.y) + w.c * a.y_stride;
+  const TIO* xr =
+      static_cast<const TIO*>(a.x) + row_offset(a, w.c % a.x_rows, a.x_stride, a.x_seg_len);
+  TIO* yr = static_cast<TIO*>(a.y) + row_offset(a, w.c, a.y_stride, a.y_seg_len);
   // REV: processing index t is time T-1-t, so the window is the time-ordered
   // run [T - t0w - 4096, T - t0w)
   const bool tvec = !REV || (a.T % IOTraits<TIO>::kVec) == 0;
