@@ -90,6 +90,23 @@ def test_reference_batched_cases(oracle_mod):
     assert s.shape == (2, 2, 6)
 
 
+def test_compiled_batched_design():
+    """test__batched.py:447-457: batched_butterworth_design under
+    torch.compile (the native call runs outside the traced graph) gives the
+    eager result, shape (5, 2, 6) for order 4 and five cutoffs."""
+    from torchscience_amd.signal_processing.filter import batched_butterworth_design
+    dev = _dev()
+
+    @torch.compile
+    def design_filters(cutoffs):
+        return batched_butterworth_design(4, cutoffs)
+
+    cutoffs = torch.linspace(0.1, 0.4, 5, device=dev)
+    sos = design_filters(cutoffs)
+    assert sos.shape == (5, 2, 6)
+    assert torch.equal(sos, batched_butterworth_design(4, cutoffs))
+
+
 def test_design_errors():
     F, _ = _F(), _dev()
     with pytest.raises(ValueError, match="cutoffs tensor cannot be empty"):

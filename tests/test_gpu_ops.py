@@ -59,3 +59,29 @@ def test_opcheck_operators():
     bargs = (x, y, saved, sos, zi, dy, dzf, 16, False)
     torch.library.opcheck(ops.sosfilt_backward.default, bargs,
                           test_utils=("test_schema", "test_faketensor"))
+
+
+def test_compiled_ctypes_bound_paths_equal_eager():
+    """The paths bound through ctypes (general-order lfilter, FIR lfilter,
+    frequency_response_sos) run under torch.compile with a graph break at
+    the native call and give the eager results."""
+    from torchscience_amd.signal_processing.filter import frequency_response_sos, lfilter
+    from torchscience_amd.signal_processing.filter._design import butterworth_sos
+    dev = _dev()
+    g = torch.Generator(device=dev)
+    g.manual_seed(3)
+    x = torch.randn(4, 3000, generator=g, device=dev)
+    b = torch.tensor([0.02, 0.08, 0.12, 0.08, 0.02], device=dev)
+    a = torch.tensor([1.0, -1.6, 1.3, -0.5, 0.08], device=dev)
+    taps = torch.hamming_window(64, device=dev) / 32
+
+    @torch.compile
+    def run(x):
+        return lfilter(b, a, x), lfilter(taps, torch.ones(1, device=dev), x)
+
+    y1, y2 = run(x)
+    assert torch.equal(y1, lfilter(b, a, x)) and torch.equal(y2, lfilter(taps, torch.ones(1, device=dev), x))
+    sos = torch.tensor(butterworth_sos(6, 0.2), device=dev)
+    w, h = torch.compile(lambda s: frequency_response_sos(s, 256))(sos)
+    w0, h0 = frequency_response_sos(sos, 256)
+    assert torch.equal(h, h0) and torch.equal(w, w0)

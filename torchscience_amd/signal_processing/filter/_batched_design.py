@@ -18,6 +18,9 @@ with the incoming gradient.
 Difference: every step is FP64 (the reference builds its prototypes at
 torch's default dtype unless dtype is given, which leaves float32-accurate
 poles in the float64 result).
+
+Under torch.compile the native call runs outside the traced graph (a graph
+break), as it does for the other ctypes-bound kernels of this package.
 """
 
 from __future__ import annotations
@@ -71,6 +74,7 @@ def _check_range(c: Tensor, band: bool) -> None:
             f"Cutoff frequency must be between 0 and 1 (Nyquist), got {c[i].item()}")
 
 
+@torch.compiler.disable
 def _design(family: str, order: int, param: float, cutoffs: Tensor, filter_type: str,
             output: str, sampling_frequency: Optional[float], dtype, device) -> Tensor:
     if output != "sos":

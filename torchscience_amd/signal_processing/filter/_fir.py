@@ -56,6 +56,7 @@ def _fir_desc(*, channels, n_samples, x_len, n_taps, dtype, shift, flags, x_rows
     return d
 
 
+@torch.compiler.disable
 def fir_rows_forward(taps: Tensor, x: Tensor, zi: Tensor | None, *, channels: int,
                      chans_per_filter: int, shift: int = 0, want_zf: bool = False,
                      zi_broadcast: bool = False, reverse_taps: bool = False,
@@ -97,6 +98,7 @@ def fir_rows_forward(taps: Tensor, x: Tensor, zi: Tensor | None, *, channels: in
     return y, zf
 
 
+@torch.compiler.disable
 def fir_rows_backward_taps(taps_shape, x: Tensor, g: Tensor, *, channels: int,
                            chans_per_filter: int, shift: int = 0) -> Tensor:
     """dtaps[f][j] = sum over channels c of filter f, n < G of

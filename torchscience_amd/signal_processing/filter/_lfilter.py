@@ -240,6 +240,7 @@ def _lfilter_as_section(b, a, x, axis, zi, order):
     return y, zf[..., :order].reshape(zi.shape).to(out)
 
 
+@torch.compiler.disable
 def _general_rows(coef, rows, zi_r, want_zf, zi_broadcast):
     """DF2T of order N = len(coef) // 2 - 1 (coef = b then a, padded) over
     rows (B, T) through tsiir_lfilter_forward_ws (time-parallel for orders up

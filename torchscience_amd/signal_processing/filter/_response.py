@@ -133,6 +133,7 @@ def _grid(frequencies, whole, sampling_frequency, dev):
     return freq_points, w
 
 
+@torch.compiler.disable
 def frequency_response_sos(sos: Tensor, frequencies: Union[Tensor, int] = 512,
                            whole: bool = False, sampling_frequency: Optional[float] = None,
                            validate: bool = True, *, dtype: Optional[torch.dtype] = None,
@@ -173,6 +174,7 @@ def frequency_response_sos(sos: Tensor, frequencies: Union[Tensor, int] = 512,
     return freq_points.to(sos.dtype).to(device), h.to(dtype).to(device)
 
 
+@torch.compiler.disable
 def frequency_response(numerator: Tensor, denominator: Tensor,
                        frequencies: Union[Tensor, int] = 512, whole: bool = False,
                        sampling_frequency: Optional[float] = None, *,
