@@ -199,3 +199,23 @@ def test_complex_sosfilt_gradients():
     print("gradcheck complex sosfilt", flush=True)
     assert torch.autograd.gradcheck(lambda s, x, z: F.sosfilt(s, x, zi=z), (sos, x, zi),
                                     eps=1e-6, atol=1e-7, rtol=1e-6)
+
+
+def test_complex_coefficients_empty_shapes():
+    """No samples, or no channels: shapes and dtypes as scipy's; for an
+    empty time axis zf = zi, as the reference's loop (_lfilter.py:153-175)
+    runs no step (scipy's own zf is not defined there)."""
+    import scipy.signal as ss
+    F, dev = _filt(), _dev()
+    b = np.array([0.5 + 0.1j, 0.5 - 0.1j, 0.2j])
+    a = np.array([1.0, -0.3 + 0.1j, 0.05])
+    bt, at = torch.tensor(b, device=dev), torch.tensor(a, device=dev)
+    for shape in [(3, 0), (0, 10)]:
+        print("empty shape", shape, flush=True)
+        x = np.zeros(shape)
+        zi = np.arange(shape[0] * 2).reshape(shape[0], 2) * (1 + 0.5j)
+        y, zf = F.lfilter(bt, at, torch.tensor(x, device=dev), zi=torch.tensor(zi, device=dev))
+        ys, zfs = ss.lfilter(b, a, x, zi=zi)
+        assert y.shape == ys.shape and zf.shape == zfs.shape and y.dtype == torch.complex128
+        if shape[1] == 0:
+            np.testing.assert_allclose(zf.cpu().numpy(), zi, atol=1e-12)

@@ -281,6 +281,9 @@ def _tail_toeplitz(c, N):
 def _tail(v, N):
     """The last N samples of rows v (B, T), left-padded with zeros if T < N."""
     T = v.shape[-1]
+    if T == 0:
+        # (padding an empty tensor is not relied on to write the pad values)
+        return v.new_zeros((v.shape[0], N))
     return v[:, T - N:] if T >= N else F.pad(v, (N - T, 0))
 
 
