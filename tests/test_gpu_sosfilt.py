@@ -192,6 +192,22 @@ def test_zf_continuation_and_steady_state():
     torch.testing.assert_close(y, torch.ones_like(y), rtol=1e-10, atol=1e-10)
 
 
+def test_lfilter_empty_time_axis_keeps_state():
+    """No samples: zf = zi on every lfilter path (one section, FIR, general
+    order), as the reference's loop runs no step (_lfilter.py:153-175)."""
+    F, dev = _filt(), _dev()
+    x = torch.empty(3, 0, dtype=torch.float64, device=dev)
+    for b, a in (([0.2, 0.3, 0.1], [1.0, -0.4, 0.1]),            # one section
+                 ([0.1] * 9, [1.0]),                               # FIR
+                 ([0.1, 0.2, 0.1, 0.05, 0.02], [1.0, -0.5, 0.2, -0.05, 0.01])):  # order 4
+        N = max(len(b), len(a)) - 1
+        zi = torch.randn(3, N, dtype=torch.float64, device=dev)
+        y, zf = F.lfilter(torch.tensor(b, dtype=torch.float64, device=dev),
+                          torch.tensor(a, dtype=torch.float64, device=dev), x, zi=zi)
+        assert y.shape == (3, 0) and zf.shape == (3, N)
+        torch.testing.assert_close(zf, zi, rtol=0, atol=0)
+
+
 def test_edge_shapes_and_dtypes():
     F, dev = _filt(), _dev()
     import scipy.signal as ss
