@@ -14,11 +14,12 @@ bitwise: a window's incoming state differs between them by less than 2^-64
 of it, which rounds away in FP32.  So which form ran is read from the
 workspace header of a C-ABI call with a caller-owned workspace: its first
 word is the device check's verdict (1: the filter forgets its state, the
-segmented form ran) and its third the launch epoch, which the table
-preparation of every pass that runs advances by one (warm-up and main pass:
-2, and 3 with a remainder pass; the look-back form: 1, also when the check
-sent the call there).  A lowpass at cutoff
-0.002 (||Phi^4096|| ~ 1e-7) must fail the check and keep the look-back.
+segmented form ran) and its third the launch epoch, which every table
+preparation that runs advances by one.  FP32 recurrences prepare the tables
+once for the segmented main pass (which runs each segment's warm-up itself)
+and the look-back form alike: 1, and 2 with a remainder pass, which
+prepares its own.  A lowpass at cutoff 0.002 (||Phi^4096|| ~ 1e-7) must
+fail the check and keep the look-back.
 """
 
 import ctypes
@@ -82,13 +83,13 @@ def test_device_check_picks_the_form(oracle_mod):
         print("device check:", name, flush=True)
         st = torch.tensor(sos.astype(np.float32)[None], device=dev)
         y, hdr = _abi_header(st, x)
-        assert hdr[0] == (1 if forgets else 0) and hdr[2] == (2 if forgets else 1), (name, hdr)
+        assert hdr[0] == (1 if forgets else 0) and hdr[2] == 1, (name, hdr)
         y_lb, hdr_lb = _abi_header(st, x, _lib.NO_SEGMENT)
         assert hdr_lb[2] == 1, hdr_lb
         # with a remainder after the segments (128 segments of 16384 and 78
-        # samples), its pass is the third
+        # samples), its pass prepares the tables a second time
         _, hdr_r = _abi_header(st, torch.cat([x, x[:, :78]], dim=1))
-        assert hdr_r[0] == (1 if forgets else 0) and hdr_r[2] == (3 if forgets else 1), hdr_r
+        assert hdr_r[0] == (1 if forgets else 0) and hdr_r[2] == (2 if forgets else 1), hdr_r
         if not forgets:
             # the look-back form's output (its accuracy on this filter, whose
             # FP32 recurrence the drop-in routes to DF2T, is tested elsewhere)

@@ -162,7 +162,29 @@ struct FwdArgs {
   int64_t seg_n = 0;
   int64_t x_seg_len = 0;
   int64_t y_seg_len = 0;
+  // Status words carry the epoch epoch_back launches before the current one
+  // (launches after one shared table preparation).
+  uint32_t epoch_back = 0;
+  // Segmented form, FP32-recurrence chained kernels: the first group of a
+  // segment other than the first one processed computes its incoming states
+  // itself (a zero-state pass over the TSIIR_SEGMENT_WARMUP samples before
+  // it, REVERSE: after it); the first segment reads zi row (row / seg_n).
+  int warm_fused = 0;
+  // zi's storage type: -1 as TSIIR_STATE_F64 says, else 1 = FP64, 0 = the
+  // compute type (TSIIR_STATE_F64 then applies to zf alone).
+  int zi_f64 = -1;
 };
+
+// Initial-state element idx of zi, and the zi row of channel (row) c.
+template <typename TACC>
+__device__ __forceinline__ double zi_in(const FwdArgs& a, int64_t idx) {
+  const uint32_t f = a.zi_f64 < 0 ? a.flags : (a.zi_f64 ? TSIIR_STATE_F64 : 0u);
+  return state_in<TACC>(a.zi, f, idx);
+}
+__device__ __forceinline__ int64_t zi_row_of(const FwdArgs& a, int64_t c) {
+  if (a.flags & TSIIR_ZI_BROADCAST) return 0;
+  return a.warm_fused ? c / a.seg_n : c;
+}
 
 __device__ __forceinline__ int64_t row_offset(const FwdArgs& a, int64_t r, int64_t stride,
                                               int64_t seg_len) {

@@ -127,7 +127,7 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
   __shared__ double wsin[2];      // incoming state of the window
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = threadIdx.x >> 6;
-  const uint32_t flag = launch_flag(a.hdr, 0);
+  const uint32_t flag = launch_flag(a.hdr, a.epoch_back);
   const int64_t k = TAIL ? k_tail : static_cast<int64_t>(blockIdx.x) / a.B;
   const int64_t c = TAIL ? static_cast<int64_t>(blockIdx.x)
                          : static_cast<int64_t>(blockIdx.x) % a.B;
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
   const bool has_zi = (a.flags & TSIIR_HAS_ZI) != 0;
   const bool write_zf = (a.flags & TSIIR_WRITE_ZF) != 0 &&
                         (TAIL ? (t0 < a.T && t0 + W >= a.T) : (t0 + W == a.T));
-  const int64_t zi_row = (a.flags & TSIIR_ZI_BROADCAST) ? 0 : c;
+  const int64_t zi_row = zi_row_of(a, c);
 
   // CHAINED: lane l < S fetches window k-1's inclusive state of section l
   double pv0 = 0.0, pv1 = 0.0;
@@ -221,8 +221,8 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
       if (k == 0) {
         if (has_zi) {
           const int64_t zi_i = (zi_row * a.S + s) * 2;
-          sw0 = state_in<TACC>(a.zi, a.flags, zi_i);
-          sw1 = state_in<TACC>(a.zi, a.flags, zi_i + 1);
+          sw0 = zi_in<TACC>(a, zi_i);
+          sw1 = zi_in<TACC>(a, zi_i + 1);
         }
       } else {
         sw0 = __shfl(pv0, s);
@@ -281,8 +281,8 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
       double zi0 = 0.0, zi1 = 0.0;
       if (has_zi) {
         const int64_t zi_i = (zi_row * a.S + s) * 2;
-        zi0 = state_in<TACC>(a.zi, a.flags, zi_i);
-        zi1 = state_in<TACC>(a.zi, a.flags, zi_i + 1);
+        zi0 = zi_in<TACC>(a, zi_i);
+        zi1 = zi_in<TACC>(a, zi_i + 1);
       }
       double sw0 = zi0, sw1 = zi1;
       if (!TAIL) {
@@ -503,10 +503,9 @@ __global__ __launch_bounds__(64) void sos_chan_kernel(FwdArgs a,
 
   double ws0 = 0.0, ws1 = 0.0;  // lane s < S: section s's state
   if ((a.flags & TSIIR_HAS_ZI) && lane < S) {
-    const int64_t zrow = (a.flags & TSIIR_ZI_BROADCAST) ? 0 : c;
-    const int64_t zi_i = (zrow * S + lane) * 2;
-    ws0 = state_in<TACC>(a.zi, a.flags, zi_i);
-    ws1 = state_in<TACC>(a.zi, a.flags, zi_i + 1);
+    const int64_t zi_i = (zi_row_of(a, c) * S + lane) * 2;
+    ws0 = zi_in<TACC>(a, zi_i);
+    ws1 = zi_in<TACC>(a, zi_i + 1);
   }
 
   u32x4 pf[L::kNV];
@@ -801,10 +800,11 @@ bool fwd_geometry(const tsiir_sosfilt_desc* d, FwdGeometry* g) {
   g->table_bytes = align_up(static_cast<size_t>(g->n_filters) * d->n_sections * kTableStride * 8, 256);
   g->status_bytes = static_cast<size_t>(g->n_windows) * d->channels * d->n_sections *
                     kStatusWords * 8;
-  g->fast_off = align_up(sizeof(WsHeader) + g->table_bytes + g->status_bytes, 256);
+  g->fast_off = align_up(sizeof(WsHeader) + g->table_bytes, 256);
   const size_t fast_bytes =
       use_fwd32(d) ? static_cast<size_t>(g->n_filters) * d->n_sections * sizeof(Fast32) : 0;
-  g->total = g->fast_off + fast_bytes;
+  g->status_off = align_up(g->fast_off + fast_bytes, 256);
+  g->total = g->status_off + g->status_bytes;
   return true;
 }
 
@@ -824,7 +824,7 @@ static tsiir_status launch_fwd(const tsiir_sosfilt_desc* d, const FwdGeometry& g
                                hipStream_t stream, void* saved, const FwdGate* gate) {
   WsHeader* hdr = reinterpret_cast<WsHeader*>(ws);
   double* tables = reinterpret_cast<double*>(ws + sizeof(WsHeader));
-  uint64_t* status = reinterpret_cast<uint64_t*>(ws + sizeof(WsHeader) + g.table_bytes);
+  uint64_t* status = reinterpret_cast<uint64_t*>(ws + g.status_off);
   const int64_t n_rows = g.n_filters * d->n_sections;
   const int log2w = (W == 64) ? 6 : 5;
   FwdArgs a;
@@ -851,13 +851,16 @@ static tsiir_status launch_fwd(const tsiir_sosfilt_desc* d, const FwdGeometry& g
   a.seg_n = gate ? gate->seg_n : 0;
   a.x_seg_len = gate ? gate->x_seg_len : 0;
   a.y_seg_len = gate ? gate->y_seg_len : 0;
+  a.epoch_back = gate ? gate->epoch_back : 0u;
+  a.warm_fused = (gate && gate->warm_fused) ? 1 : 0;
+  a.zi_f64 = (gate && gate->warm_fused) ? (gate->zi_f64 ? 1 : 0) : -1;
   const FwdPolicy pol = fwd_policy(d);
   if constexpr (std::is_same<TR, float>::value && W == 64) {
     if (use_fwd32(d)) {
       // one fused launch builds both table sets (and advances the epoch)
       Fast32* fast = reinterpret_cast<Fast32*>(ws + g.fast_off);
       const tsiir_status rc = launch_fwd32_full(a, sos, d->coef_dtype, g.n_filters, fast,
-                                                stream);
+                                                gate != nullptr && gate->tables_ready, stream);
       if (rc != TSIIR_OK) return rc;
       const int64_t full32 = d->n_samples / fwd32_window();
       if (full32 * fwd32_window() < d->n_samples)
@@ -1079,9 +1082,15 @@ size_t fwd_workspace_bytes(const tsiir_sosfilt_desc* d, bool saved) {
 // as two-level rows; then the remainder of each row from the state its
 // segments end in, and zf.  Every launch after the check runs only while
 // the gate is set.
+// FP32-recurrence forms (the direct-form-I kernels) instead build the
+// tables once for every launch of the call, the look-back form's included
+// (`fused` returns true), and fuse the warm-up into the main pass: the
+// first group of each segment runs it itself (FwdArgs::warm_fused), so the
+// warm-up pass, the state assembly and two table preparations drop out.
 static tsiir_status forward_segmented(const tsiir_sosfilt_desc* d, const SegPlan& p,
                                       const void* x, void* y, const void* sos, const void* zi,
-                                      void* zf, char* ws, hipStream_t st, int64_t n_filters) {
+                                      void* zf, char* ws, hipStream_t st, int64_t n_filters,
+                                      bool* fused) {
   const int S = d->n_sections;
   WsHeader* hdr = reinterpret_cast<WsHeader*>(ws);
   const uint32_t* gate = &hdr->seg_gate;
@@ -1106,33 +1115,51 @@ static tsiir_status forward_segmented(const tsiir_sosfilt_desc* d, const SegPlan
   char* yc = static_cast<char*>(y);
   // the segments start at sample 0 (rev: at the remainder's end)
   const int64_t seg0 = p.rev ? p.rem : 0;
-  // warm-up rows start TSIIR_SEGMENT_WARMUP samples before each segment's
-  // end (rev: at the start of the segment after it)
-  const int64_t wofs = seg0 + (p.rev ? p.lseg : p.lseg - TSIIR_SEGMENT_WARMUP);
-  const FwdGate on_warm{gate, 1u, p.nseg, p.lseg, 0};
-  tsiir_status rc = forward_impl(&p.warm, xc + wofs * esz, ws + p.ywarm_off, sos, nullptr, warm,
-                                 ws, p.base, st, nullptr, &on_warm);
-  if (rc != TSIIR_OK) return rc;
-  const bool has_zi = (d->flags & TSIIR_HAS_ZI) != 0;
-  const int bc = (d->flags & TSIIR_ZI_BROADCAST) ? 1 : 0;
-  const int rev = p.rev ? 1 : 0;
-  const bool sd = d->io_dtype == TSIIR_F64 || (d->flags & TSIIR_STATE_F64);
-  const int64_t n_in = p.rows * S * 2;
-  const unsigned bi = static_cast<unsigned>((n_in + 255) / 256);
-  if (sd)
-    hipLaunchKernelGGL(seg_state_in_kernel<double>, dim3(bi), dim3(256), 0, st, gate, warm,
-                       has_zi ? static_cast<const double*>(zi) : nullptr, bc, zin, p.rows,
-                       p.nseg, S, rev);
-  else
-    hipLaunchKernelGGL(seg_state_in_kernel<float>, dim3(bi), dim3(256), 0, st, gate, warm,
-                       has_zi ? static_cast<const float*>(zi) : nullptr, bc, zin, p.rows,
-                       p.nseg, S, rev);
-  if (hipGetLastError() != hipSuccess) return TSIIR_ELAUNCH;
   const bool want_zf = (d->flags & TSIIR_WRITE_ZF) != 0;
-  const FwdGate on_main{gate, 1u, p.nseg, p.lseg, p.lseg};
-  rc = forward_impl(&p.main, xc + seg0 * esz, yc + seg0 * esz, sos, zin,
-                    (want_zf || p.rem > 0) ? zall : nullptr, ws, p.base, st, nullptr, &on_main);
-  if (rc != TSIIR_OK) return rc;
+  const bool sd = d->io_dtype == TSIIR_F64 || (d->flags & TSIIR_STATE_F64);
+  const int rev = p.rev ? 1 : 0;
+  tsiir_status rc = TSIIR_OK;
+  *fused = use_fwd32(d);
+  if (*fused) {
+    FwdGeometry gm;
+    if (!fwd_geometry(&p.main, &gm) || gm.n_filters != n_filters) return TSIIR_EINVAL;
+    launch_prep32(sos, d->coef_dtype, n_filters, S, reinterpret_cast<double*>(ws + sizeof(WsHeader)),
+                  reinterpret_cast<Fast32*>(ws + gm.fast_off), hdr, 1u, st);
+    tsiir_sosfilt_desc m = p.main;
+    m.flags = (m.flags & ~TSIIR_HAS_ZI) | (d->flags & (TSIIR_HAS_ZI | TSIIR_ZI_BROADCAST));
+    FwdGate on_main{gate, 1u, p.nseg, p.lseg, p.lseg};
+    on_main.tables_ready = true;
+    on_main.warm_fused = true;
+    on_main.zi_f64 = sd;
+    rc = forward_impl(&m, xc + seg0 * esz, yc + seg0 * esz, sos, zi,
+                      (want_zf || p.rem > 0) ? zall : nullptr, ws, p.base, st, nullptr, &on_main);
+    if (rc != TSIIR_OK) return rc;
+  } else {
+    // warm-up rows start TSIIR_SEGMENT_WARMUP samples before each segment's
+    // end (rev: at the start of the segment after it)
+    const int64_t wofs = seg0 + (p.rev ? p.lseg : p.lseg - TSIIR_SEGMENT_WARMUP);
+    const FwdGate on_warm{gate, 1u, p.nseg, p.lseg, 0};
+    rc = forward_impl(&p.warm, xc + wofs * esz, ws + p.ywarm_off, sos, nullptr, warm, ws, p.base,
+                      st, nullptr, &on_warm);
+    if (rc != TSIIR_OK) return rc;
+    const bool has_zi = (d->flags & TSIIR_HAS_ZI) != 0;
+    const int bc = (d->flags & TSIIR_ZI_BROADCAST) ? 1 : 0;
+    const int64_t n_in = p.rows * S * 2;
+    const unsigned bi = static_cast<unsigned>((n_in + 255) / 256);
+    if (sd)
+      hipLaunchKernelGGL(seg_state_in_kernel<double>, dim3(bi), dim3(256), 0, st, gate, warm,
+                         has_zi ? static_cast<const double*>(zi) : nullptr, bc, zin, p.rows,
+                         p.nseg, S, rev);
+    else
+      hipLaunchKernelGGL(seg_state_in_kernel<float>, dim3(bi), dim3(256), 0, st, gate, warm,
+                         has_zi ? static_cast<const float*>(zi) : nullptr, bc, zin, p.rows,
+                         p.nseg, S, rev);
+    if (hipGetLastError() != hipSuccess) return TSIIR_ELAUNCH;
+    const FwdGate on_main{gate, 1u, p.nseg, p.lseg, p.lseg};
+    rc = forward_impl(&p.main, xc + seg0 * esz, yc + seg0 * esz, sos, zin,
+                      (want_zf || p.rem > 0) ? zall : nullptr, ws, p.base, st, nullptr, &on_main);
+    if (rc != TSIIR_OK) return rc;
+  }
   const int64_t n_out = d->channels * S * 2;
   const unsigned bo = static_cast<unsigned>((n_out + 255) / 256);
   const double* zlast = zall;  // the state the segments end in, per channel
@@ -1206,11 +1233,14 @@ tsiir_status forward_impl(const tsiir_sosfilt_desc* d, const void* x, void* y, c
   SegPlan sp;
   if (gate == nullptr && seg_plan(d, saved != nullptr, &sp)) {
     if (workspace_bytes < sp.total) return TSIIR_EWORKSPACE;
+    bool fused = false;
     const tsiir_status rc =
-        forward_segmented(d, sp, x, y, sos, zi, zf, ws, st, g.n_filters);
+        forward_segmented(d, sp, x, y, sos, zi, zf, ws, st, g.n_filters, &fused);
     if (rc != TSIIR_OK) return rc;
-    // the look-back form runs only where the check cleared the gate
+    // the look-back form runs only where the check cleared the gate (on the
+    // tables the segmented form built, for the FP32-recurrence forms)
     off = FwdGate{&reinterpret_cast<WsHeader*>(ws)->seg_gate, 0u};
+    off.tables_ready = fused;
     gate = &off;
   }
   const bool comp = d->mode == TSIIR_MODE_COMPENSATED;

@@ -510,8 +510,8 @@ __device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32<NWV>
       double zi0 = 0.0, zi1 = 0.0;
       if (w.has_zi) {
         const int64_t zi_i = (w.zi_row * S + s) * 2;
-        zi0 = state_in<float>(a.zi, a.flags, zi_i);
-        zi1 = state_in<float>(a.zi, a.flags, zi_i + 1);
+        zi0 = zi_in<float>(a, zi_i);
+        zi1 = zi_in<float>(a, zi_i + 1);
       }
       sw0 = zi0;
       sw1 = zi1;
@@ -696,12 +696,80 @@ __device__ __forceinline__ void read_group_in(const FwdArgs& a, const Win32<NWV>
     ok = (read_pair(st + 4, w.flag, p0, p1) && !debug_give_up(a.flags)) ? 1 : 0;
   } else if (w.has_zi) {
     const int64_t zi_i = (w.zi_row * S + lane) * 2;
-    p0 = state_in<float>(a.zi, a.flags, zi_i);
-    p1 = state_in<float>(a.zi, a.flags, zi_i + 1);
+    p0 = zi_in<float>(a, zi_i);
+    p1 = zi_in<float>(a, zi_i + 1);
   }
   prev_ok[lane] = ok;
   prev[2 * lane] = p0;
   prev[2 * lane + 1] = p1;
+}
+
+// Segmented form with the fused warm-up: does the group start a segment
+// whose incoming states come from a warm-up (every segment but the first one
+// processed: k = 0 forward, k = seg_n - 1 along decreasing time)?
+template <bool REV, int NWV>
+__device__ __forceinline__ bool warm_group(const FwdArgs& a, const Win32<NWV>& w) {
+  if (!a.warm_fused || w.kg != 0) return false;
+  const int64_t k = w.c % a.seg_n;
+  return k != (REV ? a.seg_n - 1 : 0);
+}
+
+// The fused warm-up, one wave (wave 0 of such a group): the cascade from
+// zero state over the kWin32 samples before the segment (REV: the samples
+// after it, along decreasing time), whose end states stand for the
+// segment's incoming states because the filter forgets its state within
+// that span (checked on the device before the launch).  The same zero-state
+// window steps as run_cascade; the end state of section s is the window's
+// aggregate in true units, written to prev[] (prev_ok set) where the chained
+// form reads the group's incoming states.
+// Section s's end state of the warm-up window (its aggregate, zero incoming
+// state) in true units, as the group's incoming state of section s.
+__device__ __forceinline__ void warm_record(const Fast32* fs, int s, float ag0, float ag1,
+                                            double* prev, int* prev_ok) {
+  if (lane_now() == 0) {
+    prev[2 * s] = fs->p * static_cast<double>(ag0);
+    prev[2 * s + 1] = fs->p * static_cast<double>(ag1);
+    prev_ok[s] = 1;
+  }
+}
+
+template <typename TIO, bool REV, bool UNIT, int NWV>
+__device__ __forceinline__ void warm_states(const FwdArgs& a, const Win32<NWV>& w, double* prev,
+                                            int* prev_ok) {
+  const TIO* xr =
+      static_cast<const TIO*>(a.x) + row_offset(a, w.c % a.x_rows, a.x_stride, a.x_seg_len);
+  const bool tvec = !REV || (a.T % IOTraits<TIO>::kVec) == 0;
+  const bool xvec = tvec && (reinterpret_cast<uintptr_t>(xr) & 15u) == 0;
+  float v[kW32];
+  if (REV) {
+    // processing indices -kWin32 .. -1 are times T + kWin32 - 1 .. T
+    if (xvec)
+      load_window32<TIO, true>(xr + a.T, w.stage, v);
+    else
+      load_run_rev<TIO, float, kW32>(
+          xr, a.T + kWin32 - 1 - static_cast<int64_t>(lane_now()) * kW32, false, kW32, v);
+  } else if (xvec) {
+    load_window32<TIO, false>(xr - kWin32, w.stage, v);
+  } else {
+    load_run<TIO, float, kW32>(xr - kWin32 + static_cast<int64_t>(lane_now()) * kW32, false, kW32,
+                               v);
+  }
+  // section 0 peeled, as in run_cascade
+  float z0, z1, x0, x1, ag0, ag1;
+  LanePow lp = lane_pow(w.ff->mq);
+  df1_sweep<UNIT, false>(v, w.ff, table_g(w.tf), 0.0f, 0.0f, z0, z1);
+  lane_scan32(w.ff, lp.pm[0], lp.pm[1], z0, z1, x0, x1, ag0, ag1);
+  warm_record(w.ff, 0, ag0, ag1, prev, prev_ok);
+  for (int s = 1; s < a.S; ++s) {
+    const Fast32* fs = w.ff + s;
+    lp = lane_pow(fs->mq);
+    // the lane's incoming state of section s-1 with a zero window state is
+    // its exclusive scan value: the pending correction
+    df1_sweep<UNIT, true>(v, fs, table_g(w.tf + static_cast<int64_t>(s - 1) * kTableStride), x0,
+                          x1, z0, z1);
+    lane_scan32(fs, lp.pm[0], lp.pm[1], z0, z1, x0, x1, ag0, ag1);
+    warm_record(fs, s, ag0, ag1, prev, prev_ok);
+  }
 }
 
 // Full windows of every channel, NWV consecutive windows (a group) per
@@ -742,7 +810,7 @@ __global__ __launch_bounds__(64 * NWV) void sos_fwd32_kernel(FwdArgs a,
   w.tf = tables + fi * static_cast<int64_t>(S) * kTableStride;
   w.t0w = (w.kg + w.wv) * kWin32;
   w.has_zi = (a.flags & TSIIR_HAS_ZI) != 0;
-  w.zi_row = (a.flags & TSIIR_ZI_BROADCAST) ? 0 : w.c;
+  w.zi_row = zi_row_of(a, w.c);
   w.write_zf = (a.flags & TSIIR_WRITE_ZF) != 0 && (w.kg + NWV) * kWin32 == a.T;
   double* prev = reinterpret_cast<double*>(smem + NWV * kStageV);
   int* prev_ok = reinterpret_cast<int*>(prev + 2 * kWave);
@@ -750,11 +818,22 @@ __global__ __launch_bounds__(64 * NWV) void sos_fwd32_kernel(FwdArgs a,
   w.prev_ok = prev_ok;
   w.wagg = reinterpret_cast<double*>(prev_ok + kWave);
   w.sgrp = w.wagg + 2 * NWV * 2;
-  w.flag = launch_flag(a.hdr, 0);
-  if (!LB) read_group_in<NWV>(a, w, prev, prev_ok);
-  __syncthreads();
+  w.flag = launch_flag(a.hdr, a.epoch_back);
   u32x4* stage = smem + w.wv * kStageV;
   w.stage = stage;
+  if (!LB) {
+    if (warm_group<REV, NWV>(a, w)) {
+      if (w.wv == 0) {
+        if (w.ff->unit != 0)
+          warm_states<TIO, REV, true, NWV>(a, w, prev, prev_ok);
+        else
+          warm_states<TIO, REV, false, NWV>(a, w, prev, prev_ok);
+      }
+    } else {
+      read_group_in<NWV>(a, w, prev, prev_ok);
+    }
+  }
+  __syncthreads();
   if (w.ff->unit != 0)
     window32<TIO, REV, SAVED, true, NWV, LB>(a, w, stage);
   else
@@ -772,7 +851,8 @@ __global__ __launch_bounds__(64 * NWV) void sos_fwd32_kernel(FwdArgs a,
 // stepped by C^16; M^l by binary powering of M = A^64, stepped by M^16.
 // Coefficients are
 // normalised by a0 in the coefficient type, then promoted.  The first thread
-// advances the workspace epoch (one launch publishes status words after it).
+// advances the workspace epoch by n_epochs (one per following launch that
+// publishes status words).
 constexpr int kPrepLanes = 16;
 constexpr int kPrepRowsPerBlock = 256 / kPrepLanes;
 
@@ -781,13 +861,13 @@ __global__ __launch_bounds__(256) void sos_prep32_kernel(const TC* __restrict__ 
                                                          int64_t n_rows, int S,
                                                          double* __restrict__ tables,
                                                          Fast32* __restrict__ fast,
-                                                         WsHeader* hdr, const uint32_t* gate,
-                                                         uint32_t gate_on) {
+                                                         WsHeader* hdr, uint32_t n_epochs,
+                                                         const uint32_t* gate, uint32_t gate_on) {
   if (gate != nullptr && ((*gate != 0u) != (gate_on != 0u))) return;
   const int64_t i =
       static_cast<int64_t>(blockIdx.x) * kPrepRowsPerBlock + (threadIdx.x / kPrepLanes);
   const int l = threadIdx.x % kPrepLanes;
-  if (blockIdx.x == 0 && threadIdx.x == 0 && hdr != nullptr) hdr->epoch += 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && hdr != nullptr) hdr->epoch += n_epochs;
   if (i >= n_rows) return;
   const int64_t f = i / S;
   const int s = static_cast<int>(i % S);
@@ -941,20 +1021,33 @@ static void launch_fwd32_windows(const FwdArgs& a, const Fast32* fast, hipStream
   }
 }
 
-tsiir_status launch_fwd32_full(const FwdArgs& a, const void* sos, int coef_dtype,
-                               int64_t n_filters, Fast32* fast, hipStream_t stream) {
-  const int64_t n_rows = n_filters * a.S;
+static void prep32(const void* sos, int coef_dtype, int64_t n_filters, int S, double* tables,
+                   Fast32* fast, WsHeader* hdr, uint32_t n_epochs, const uint32_t* gate,
+                   uint32_t gate_on, hipStream_t stream) {
+  const int64_t n_rows = n_filters * S;
   const unsigned pblocks =
       static_cast<unsigned>((n_rows + kPrepRowsPerBlock - 1) / kPrepRowsPerBlock);
-  double* tables = const_cast<double*>(a.tables);
   if (coef_dtype == TSIIR_F64)
     hipLaunchKernelGGL(sos_prep32_kernel<double>, dim3(pblocks), dim3(256), 0, stream,
-                       static_cast<const double*>(sos), n_rows, a.S, tables, fast, a.hdr,
-                       a.gate, a.gate_on);
+                       static_cast<const double*>(sos), n_rows, S, tables, fast, hdr, n_epochs,
+                       gate, gate_on);
   else
     hipLaunchKernelGGL(sos_prep32_kernel<float>, dim3(pblocks), dim3(256), 0, stream,
-                       static_cast<const float*>(sos), n_rows, a.S, tables, fast, a.hdr,
-                       a.gate, a.gate_on);
+                       static_cast<const float*>(sos), n_rows, S, tables, fast, hdr, n_epochs,
+                       gate, gate_on);
+}
+
+void launch_prep32(const void* sos, int coef_dtype, int64_t n_filters, int S, double* tables,
+                   Fast32* fast, WsHeader* hdr, uint32_t n_epochs, hipStream_t stream) {
+  prep32(sos, coef_dtype, n_filters, S, tables, fast, hdr, n_epochs, nullptr, 0u, stream);
+}
+
+tsiir_status launch_fwd32_full(const FwdArgs& a, const void* sos, int coef_dtype,
+                               int64_t n_filters, Fast32* fast, bool tables_ready,
+                               hipStream_t stream) {
+  if (!tables_ready)
+    prep32(sos, coef_dtype, n_filters, a.S, const_cast<double*>(a.tables), fast, a.hdr, 1u,
+           a.gate, a.gate_on, stream);
   const bool rev = (a.flags & TSIIR_REVERSE) != 0;
   if (a.io_dtype == TSIIR_BF16) {
     if (rev)
