@@ -43,6 +43,8 @@
 
 #include <math.h>
 
+#include <type_traits>
+
 #include "sos_common.h"
 #include "sos_internal.h"
 #include "tsiir.h"
@@ -258,7 +260,12 @@ __device__ __forceinline__ void load_window32(const TIO* win, u32x4* lds, float 
 }
 
 // Registers -> window, each value multiplied by `scale` (non-temporal stores).
-template <typename TIO, bool REV>
+// LEAN: one 16-byte vector in flight at a time (scheduling barriers between
+// them), for stores in the middle of the cascade, where the sample registers
+// stay live and a batch of staged vectors would need registers of its own.
+// ELEM: the staged vectors go out element by element (a window that is not
+// 16-byte aligned), still one coalesced store instruction per element slot.
+template <typename TIO, bool REV, bool LEAN = false, bool ELEM = false>
 __device__ __forceinline__ void store_window32(TIO* win, u32x4* lds, const float (&v)[kW32],
                                                float scale) {
   using L = Stage32<TIO>;
@@ -278,12 +285,25 @@ __device__ __forceinline__ void store_window32(TIO* win, u32x4* lds, const float
         e[j] = IOTraits<TIO>::from_f(v[REV ? kW32 - 1 - idx : idx] * scale);
       }
       lds[L::slot(q, t)] = *reinterpret_cast<const u32x4*>(&rv);
+      if constexpr (LEAN) __builtin_amdgcn_sched_barrier(0);
     }
     wave_lds_sync();
 #pragma unroll
-    for (int i = 0; i < L::kH; ++i)
-      __builtin_nontemporal_store(lds[L::slot(L::grow(i, lane), lane % L::kH)],
-                                  gp + L::gvec(r, i, lane));
+    for (int i = 0; i < L::kH; ++i) {
+      const u32x4 o = lds[L::slot(L::grow(i, lane), lane % L::kH)];
+      if constexpr (ELEM) {
+        // element bits (FP32: 32, BF16: 16)
+        using raw_t = typename std::conditional<sizeof(TIO) == 2, uint16_t, uint32_t>::type;
+        const raw_t* oe = reinterpret_cast<const raw_t*>(&o);
+        raw_t* dst = reinterpret_cast<raw_t*>(win + static_cast<int64_t>(L::gvec(r, i, lane)) *
+                                                        L::kVec);
+#pragma unroll
+        for (int k = 0; k < L::kVec; ++k) __builtin_nontemporal_store(oe[k], dst + k);
+      } else {
+        __builtin_nontemporal_store(o, gp + L::gvec(r, i, lane));
+      }
+      if constexpr (LEAN) __builtin_amdgcn_sched_barrier(0);
+    }
     wave_lds_sync();
   }
 }
@@ -577,7 +597,7 @@ __device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32<NWV>
 
 // Section s's exact output in true units, for the backward pass: applies the
 // pending correction now (which clears it) and stores p_s * v.
-template <typename TIO, int NWV>
+template <typename TIO, int NWV, bool ALIGNED>
 __device__ __forceinline__ void keep_section(const FwdArgs& a, const Win32<NWV>& w, int s,
                                              float (&v)[kW32], float& pw0, float& pw1) {
   const double* tb = w.tf + static_cast<int64_t>(s) * kTableStride;
@@ -588,18 +608,17 @@ __device__ __forceinline__ void keep_section(const FwdArgs& a, const Win32<NWV>&
   // measured 7% slower on the training step)
   TIO* sw = static_cast<TIO*>(a.saved) + (static_cast<int64_t>(s) * a.B + w.c) * a.T + w.t0w;
   const float scale = static_cast<float>(w.ff[s].p);
-  if ((reinterpret_cast<uintptr_t>(sw) & 15u) == 0)
-    store_window32<TIO, false>(sw, w.stage, v, scale);
-  else
-    store_lane32<TIO>(sw + static_cast<int64_t>(lane_now()) * kW32, false, v, scale);
+  // (the alignment of every saved row is known at launch: one store form per
+  // kernel keeps the register allocation of the cascade intact)
+  store_window32<TIO, false, true, !ALIGNED>(sw, w.stage, v, scale);
 }
 
 // All sections of the window (section 0 has no pending correction, the
 // others take the previous section's); on return v holds the cascade output
 // in scaled units with the last section's correction (pw0, pw1) pending
-// (finish_cascade applies it).  SAVED: the outputs of
-// sections 0..S-2 are also stored.
-template <typename TIO, bool SAVED, bool UNIT, int NWV, bool LB>
+// (finish_cascade applies it).  SAVED (nonzero): the outputs of
+// sections 0..S-2 are also stored; 1: their rows are 16-byte aligned.
+template <typename TIO, int SAVED, bool UNIT, int NWV, bool LB>
 __device__ __forceinline__ void run_cascade(const FwdArgs& a, const Win32<NWV>& w,
                                             float (&v)[kW32], float& pw0, float& pw1) {
   const int S = a.S;
@@ -610,14 +629,14 @@ __device__ __forceinline__ void run_cascade(const FwdArgs& a, const Win32<NWV>& 
   GroupPrefetch pf = prefetch_group_in<NWV, LB>(a, w, 0);
   df1_sweep<UNIT, false>(v, w.ff, table_g(w.tf), 0.0f, 0.0f, z0, z1);
   section_carry<NWV, LB>(a, w, 0, lp, z0, z1, pw0, pw1, pf);
-  if (SAVED && S > 1) keep_section<TIO, NWV>(a, w, 0, v, pw0, pw1);
+  if (SAVED && S > 1) keep_section<TIO, NWV, SAVED == 1>(a, w, 0, v, pw0, pw1);
   for (int s = 1; s < S; ++s) {
     lp = lane_pow(w.ff[s].mq);
     pf = prefetch_group_in<NWV, LB>(a, w, s);
     const float* pg = table_g(w.tf + static_cast<int64_t>(s - 1) * kTableStride);
     df1_sweep<UNIT, true>(v, w.ff + s, pg, pw0, pw1, z0, z1);
     section_carry<NWV, LB>(a, w, s, lp, z0, z1, pw0, pw1, pf);
-    if (SAVED && s + 1 < S) keep_section<TIO, NWV>(a, w, s, v, pw0, pw1);
+    if (SAVED && s + 1 < S) keep_section<TIO, NWV, SAVED == 1>(a, w, s, v, pw0, pw1);
   }
 }
 
@@ -640,7 +659,7 @@ __device__ __forceinline__ void finish_cascade(const FwdArgs& a, const Win32<NWV
 }
 
 // One window per wave: load, cascade, store.
-template <typename TIO, bool REV, bool SAVED, bool UNIT, int NWV, bool LB>
+template <typename TIO, bool REV, int SAVED, bool UNIT, int NWV, bool LB>
 __device__ __forceinline__ void window32(const FwdArgs& a, Win32<NWV>& w, u32x4* stage) {
   const int lane = lane_now();
   const TIO* xr =
@@ -714,7 +733,7 @@ __device__ __forceinline__ bool warm_group(const FwdArgs& a, const Win32<NWV>& w
   return k != (REV ? a.seg_n - 1 : 0);
 }
 
-// The fused warm-up, one wave (wave 0 of such a group): the cascade from
+// The fused warm-up, one wave of such a group: the cascade from
 // zero state over the kWin32 samples before the segment (REV: the samples
 // after it, along decreasing time), whose end states stand for the
 // segment's incoming states because the filter forgets its state within
@@ -788,8 +807,8 @@ __device__ __forceinline__ void warm_states(const FwdArgs& a, const Win32<NWV>& 
 // tap 1 with the running b0 product, or b0 kept as a tap) branch before
 // anything is loaded, so no window data is live across the branch.  REV:
 // along decreasing time (TSIIR_REVERSE); SAVED: keep the section outputs
-// (forward_saved).
-template <typename TIO, bool REV, bool SAVED, int NWV, bool LB>
+// (forward_saved), 1 where every saved row is 16-byte aligned, 2 where not.
+template <typename TIO, bool REV, int SAVED, int NWV, bool LB>
 __global__ __launch_bounds__(64 * NWV) void sos_fwd32_kernel(FwdArgs a,
                                                              const Fast32* __restrict__ fast,
                                                              const double* __restrict__ tables,
@@ -823,7 +842,9 @@ __global__ __launch_bounds__(64 * NWV) void sos_fwd32_kernel(FwdArgs a,
   w.stage = stage;
   if (!LB) {
     if (warm_group<REV, NWV>(a, w)) {
-      if (w.wv == 0) {
+      // one wave per group runs the warm-up; which one rotates with the
+      // workgroup so that the groups resident on a CU spread it over its SIMDs
+      if (w.wv == static_cast<int>(blockIdx.x % NWV)) {
         if (w.ff->unit != 0)
           warm_states<TIO, REV, true, NWV>(a, w, prev, prev_ok);
         else
@@ -990,11 +1011,17 @@ template <typename TIO, bool REV, int NWV, bool LB>
 static void launch_fwd32_io(const FwdArgs& a, const Fast32* fast, int64_t groups, int64_t k0,
                             hipStream_t stream) {
   const dim3 grid(static_cast<unsigned>(groups)), block(kWave * NWV);
-  if (a.saved != nullptr)
-    hipLaunchKernelGGL((sos_fwd32_kernel<TIO, REV, true, NWV, LB>), grid, block, 0, stream, a,
+  // saved rows (s * B + c) * T: all 16-byte aligned, or not
+  const bool saved_aligned = (reinterpret_cast<uintptr_t>(a.saved) & 15u) == 0 &&
+                             (a.T * static_cast<int64_t>(sizeof(TIO))) % 16 == 0;
+  if (a.saved == nullptr)
+    hipLaunchKernelGGL((sos_fwd32_kernel<TIO, REV, 0, NWV, LB>), grid, block, 0, stream, a,
+                       fast, a.tables, k0);
+  else if (saved_aligned)
+    hipLaunchKernelGGL((sos_fwd32_kernel<TIO, REV, 1, NWV, LB>), grid, block, 0, stream, a,
                        fast, a.tables, k0);
   else
-    hipLaunchKernelGGL((sos_fwd32_kernel<TIO, REV, false, NWV, LB>), grid, block, 0, stream, a,
+    hipLaunchKernelGGL((sos_fwd32_kernel<TIO, REV, 2, NWV, LB>), grid, block, 0, stream, a,
                        fast, a.tables, k0);
 }
 
