@@ -160,12 +160,15 @@ def test_reverse(oracle_mod, T):
     np.testing.assert_allclose(zf.cpu().numpy(), zf64, rtol=1e-4, atol=1e-4)
 
 
-def test_saved_section_outputs(oracle_mod):
+@pytest.mark.parametrize("T", [2 * 4096 + 11, 2 * 4096 + 12])
+def test_saved_section_outputs(oracle_mod, T):
     """The training forward keeps the output of every section but the last,
-    in true units (the b0 normalisation undone)."""
+    in true units (the b0 normalisation undone).  T = 8203: the saved rows
+    are not 16-byte aligned (element stores); T = 8204: they are."""
     from torchscience_amd.signal_processing.filter._design import butterworth_sos
     dev = _dev()
-    B, T = 1024, 2 * 4096 + 11
+    B = 1024
+    print(f"saved sections T={T}", flush=True)
     rng = np.random.default_rng(5)
     sos = butterworth_sos(8, 0.2).astype(np.float32)
     x = rng.standard_normal((B, T)).astype(np.float32)
