@@ -60,11 +60,11 @@ def test_workspace_size_query_without_gpu():
                          mode=0, flags=0, reserved=0, x_rows=4096, x_row_stride=65536,
                          y_row_stride=65536, chans_per_filter=4096)
     n = L.tsiir_sosfilt_workspace_bytes(ctypes.byref(d))
-    # 1024-byte header + one 896-byte table entry per section (rounded to 256 B) +
-    # 64 B of status per (window, channel, section); windows of 4096 samples;
-    # then (256-B aligned) one 1088-byte FP32-kernel entry per section
-    fp64_part = 1024 + 5376 + 16 * 4096 * 6 * 64
-    assert n == (fp64_part + 255) // 256 * 256 + 6 * 1088
+    # 1024-byte header + one 896-byte table entry per section (rounded to 256 B),
+    # then one 1088-byte FP32-kernel entry per section (rounded to 256 B), then
+    # 64 B of status per (window, channel, section); windows of 4096 samples
+    tables = 1024 + 5376 + 6 * 1088
+    assert n == (tables + 255) // 256 * 256 + 16 * 4096 * 6 * 64
     d.n_sections = 0
     assert L.tsiir_sosfilt_workspace_bytes(ctypes.byref(d)) == 0
     assert L.tsiir_status_string(2) == b"unsupported request"
@@ -90,9 +90,9 @@ def test_segmented_workspace_plan_without_gpu():
         return (v + 255) // 256 * 256
 
     def plain(B, T):
-        # header, one 896-byte table entry per section, 64 B of status per
-        # (4096-sample window, row, section), one 1088-byte FP32 entry per section
-        return align(1024 + 5376 + (-(-T // 4096)) * B * 6 * 64) + 6 * 1088
+        # header, one 896-byte table entry per section, one 1088-byte FP32
+        # entry per section, 64 B of status per (4096-sample window, row, section)
+        return align(1024 + 5376 + 6 * 1088) + (-(-T // 4096)) * B * 6 * 64
 
     def seg(B, T, rows, lseg, wrows, rem):
         base = max(plain(wrows, 4096), plain(rows, lseg), plain(B, rem) if rem else 0)
