@@ -29,9 +29,12 @@
 //      DPP row broadcasts that carry the row totals across the wave.
 //   3. the group's incoming state, in FP64: zi, or the inclusive state of
 //      the group before (many channels: read once for all sections up
-//      front; few channels: decoupled look-back), composed through LDS with
-//      the wave aggregates of the group; the group's inclusive state is
-//      published for the groups after it.
+//      front; few channels: decoupled look-back), or, for the first group
+//      of a segment of the segmented form (sosfilt_fwd.hip), the end states
+//      of a zero-state pass over the warm-up window before the segment that
+//      one wave of the group runs first; composed through LDS with the wave
+//      aggregates of the group; the group's inclusive state is published
+//      for the groups after it.
 //   4. each lane's incoming state M^q s_wave + exclusive_q (M^q per lane
 //      from the table) becomes the pending zero-input response.
 // x is read from HBM once and y written once, through LDS with non-temporal
