@@ -47,6 +47,7 @@
 #include <math.h>
 
 #include <type_traits>
+#include <utility>
 
 #include "sos_common.h"
 #include "sos_internal.h"
@@ -59,6 +60,7 @@ namespace {
 constexpr int kW32 = 64;                              // samples per lane
 constexpr int64_t kChainedChannels = 1024;  // chained groups from this many channels on
 constexpr int64_t kWin32 = static_cast<int64_t>(kWave) * kW32;  // samples per window
+constexpr int kLine = 128;  // bytes per L2 cache line
 
 // The lane index, re-read at each use: the empty asm hides that it is
 // loop-invariant, so values derived from it (LDS slots, lane offsets) are
@@ -72,6 +74,7 @@ __device__ __forceinline__ int lane_now() {
 
 // DPP controls (GFX9 encoding)
 constexpr int kDppRowShr = 0x110;      // + shift 1..15, within each row of 16 lanes
+constexpr int kDppWaveShl1 = 0x130;    // whole wave, one lane towards lower lanes
 constexpr int kDppWaveShr1 = 0x138;    // whole wave, one lane towards higher lanes
 constexpr int kDppRowBcast15 = 0x142;  // lane 15 of each row to the next row
 constexpr int kDppRowBcast31 = 0x143;  // lane 31 to rows 2 and 3
@@ -262,15 +265,31 @@ __device__ __forceinline__ void load_window32(const TIO* win, u32x4* lds, float 
   stage_window32<TIO, REV>(raw, lds, v);
 }
 
-// Registers -> window, each value multiplied by `scale` (non-temporal stores).
-// LEAN: one 16-byte vector in flight at a time (scheduling barriers between
+// *p = x; NT: a streaming (non-temporal) store.  Streaming stores suit
+// whole 128-byte lines: two of them that write the two parts of one line reach
+// memory as partial lines (a window 16 bytes off a line boundary measured 2x
+// slower), where regular stores meet in the L2 first.
+template <bool NT, typename T>
+__device__ __forceinline__ void put(T* p, const T& x) {
+  if constexpr (NT)
+    __builtin_nontemporal_store(x, p);
+  else
+    *p = x;
+}
+
+// Registers -> window, each value multiplied by `scale`.  NT: streaming
+// stores (every window of the launch starts on a 128-byte line).  LEAN: one 16-byte vector in flight at a time (scheduling barriers between
 // them), for stores in the middle of the cascade, where the sample registers
 // stay live and a batch of staged vectors would need registers of its own.
 // ELEM: the staged vectors go out element by element (a window that is not
 // 16-byte aligned), still one coalesced store instruction per element slot.
-template <typename TIO, bool REV, bool LEAN = false, bool ELEM = false>
+// HEAD: the window's first 16-byte vector is written from element head_skip
+// on (the elements before it belong to the window before; see
+// store_window32_any).
+template <typename TIO, bool REV, bool NT, bool LEAN = false, bool ELEM = false,
+          bool HEAD = false>
 __device__ __forceinline__ void store_window32(TIO* win, u32x4* lds, const float (&v)[kW32],
-                                               float scale) {
+                                               float scale, int head_skip = 0) {
   using L = Stage32<TIO>;
   using vec_t = typename IOTraits<TIO>::vec_t;
   const int lane = lane_now();
@@ -294,16 +313,23 @@ __device__ __forceinline__ void store_window32(TIO* win, u32x4* lds, const float
 #pragma unroll
     for (int i = 0; i < L::kH; ++i) {
       const u32x4 o = lds[L::slot(L::grow(i, lane), lane % L::kH)];
-      if constexpr (ELEM) {
-        // element bits (FP32: 32, BF16: 16)
-        using raw_t = typename std::conditional<sizeof(TIO) == 2, uint16_t, uint32_t>::type;
+      // element bits (FP32: 32, BF16: 16)
+      using raw_t = typename std::conditional<sizeof(TIO) == 2, uint16_t, uint32_t>::type;
+      if (HEAD && r == 0 && i == 0 && lane == 0) {
+        // vector 0 of the window (round 0, instruction 0, lane 0), in part
+        const raw_t* oe = reinterpret_cast<const raw_t*>(&o);
+        raw_t* dst = reinterpret_cast<raw_t*>(win);
+#pragma unroll
+        for (int k = 1; k < L::kVec; ++k)
+          if (k >= head_skip) put<NT>(dst + k, oe[k]);
+      } else if constexpr (ELEM) {
         const raw_t* oe = reinterpret_cast<const raw_t*>(&o);
         raw_t* dst = reinterpret_cast<raw_t*>(win + static_cast<int64_t>(L::gvec(r, i, lane)) *
                                                         L::kVec);
 #pragma unroll
-        for (int k = 0; k < L::kVec; ++k) __builtin_nontemporal_store(oe[k], dst + k);
+        for (int k = 0; k < L::kVec; ++k) put<NT>(dst + k, oe[k]);
       } else {
-        __builtin_nontemporal_store(o, gp + L::gvec(r, i, lane));
+        put<NT>(gp + L::gvec(r, i, lane), o);
       }
       if constexpr (LEAN) __builtin_amdgcn_sched_barrier(0);
     }
@@ -311,13 +337,218 @@ __device__ __forceinline__ void store_window32(TIO* win, u32x4* lds, const float
   }
 }
 
-// Direct per-lane store of the lane's samples (rows not 16-byte aligned):
-// forward, p[j] = v[j]; REVERSE, p[-j] = v[j] (p = the lane's last time).
+// Direct per-lane store of the lane's samples: forward, p[j] = v[j]; REVERSE,
+// p[-j] = v[j] (p = the lane's last time).
 template <typename TIO>
 __device__ __forceinline__ void store_lane32(TIO* p, bool rev, const float (&v)[kW32],
                                              float scale) {
 #pragma unroll
   for (int j = 0; j < kW32; ++j) p[rev ? -j : j] = IOTraits<TIO>::from_f(v[j] * scale);
+}
+
+// ---- windows that do not start on a 16-byte boundary -------------------------
+//
+// A window (time-ordered start `win`) r elements past a 16-byte boundary (0 <
+// r < kVec) is staged as the aligned run that starts r elements earlier, base
+// = win - r, so every 16-byte access stays aligned and coalesced; each lane
+// then moves its samples by r in registers and exchanges the r samples at
+// its edge with the neighbouring lane (one DPP wave shift per sample).  The
+// edge lane takes them from the aligned vector at base + kWin32 instead,
+// which holds the window's last r elements.  Every vector read holds at least
+// one element of the window (the staged run's first vector holds win[0], the
+// extra one win[kWin32 - 1]), and an aligned 16-byte vector never crosses a
+// page, so no read can fault beyond the caller's row.  Stores write only the
+// window's own elements: the staged run's first vector in part, the last r
+// elements one by one.
+//
+// Forward (lane q holds times q*64 .. q*64+63 of the window): the staged lane
+// holds A[j] = win[q*64 + j - r], so v[j] = A[j + r] and the top r come from
+// lane q+1's A[0 .. r-1] (lane 63: the extra vector).  REV (lane q holds the
+// window's times 4095 - q*64 - j): A[j] = time 4095 - q*64 - j - r, so v[j] =
+// A[j - r] and the bottom r come from lane q-1's A[64-r .. 63] (lane 0: the
+// extra vector, reversed).
+
+// The moves by the runtime r (wave-uniform, 0 <= r < KV) are branch-free:
+// one select per element for each bit of r, over the lane's samples extended
+// by KV neighbouring ones.  Every index of the selects is a constant of a
+// fold expression: written as a loop, a select between two elements of the
+// array is turned into a load through a selected address before the loop
+// unrolls, and the array then stays in scratch memory.
+
+// f[j] = c ? f[j + S] : f[j], j ascending
+template <int S, int N, size_t... J>
+__device__ __forceinline__ void select_from_above(float (&f)[N], bool c,
+                                                  std::index_sequence<J...>) {
+  ((f[J] = c ? f[J + S] : f[J]), ...);
+}
+// f[j] = c ? f[j - S] : f[j], j descending
+template <int S, int N, size_t... J>
+__device__ __forceinline__ void select_from_below(float (&f)[N], bool c,
+                                                  std::index_sequence<J...>) {
+  ((f[N - 1 - J] = c ? f[N - 1 - J - S] : f[N - 1 - J]), ...);
+}
+// f[j] = f[j + m] for j < N - m (0 <= m < KV <= 8, KV a power of two)
+template <int KV, int N>
+__device__ __forceinline__ void move_down(float (&f)[N], int m) {
+  if constexpr (KV > 1) select_from_above<1>(f, (m & 1) != 0, std::make_index_sequence<N - 1>{});
+  if constexpr (KV > 2) select_from_above<2>(f, (m & 2) != 0, std::make_index_sequence<N - 2>{});
+  if constexpr (KV > 4) select_from_above<4>(f, (m & 4) != 0, std::make_index_sequence<N - 4>{});
+}
+// f[j] = f[j - m] for j >= m (0 <= m < KV <= 8, KV a power of two)
+template <int KV, int N>
+__device__ __forceinline__ void move_up(float (&f)[N], int m) {
+  if constexpr (KV > 1) select_from_below<1>(f, (m & 1) != 0, std::make_index_sequence<N - 1>{});
+  if constexpr (KV > 2) select_from_below<2>(f, (m & 2) != 0, std::make_index_sequence<N - 2>{});
+  if constexpr (KV > 4) select_from_below<4>(f, (m & 4) != 0, std::make_index_sequence<N - 4>{});
+}
+
+// v[j] = (v ++ ext)[j + r]; ext[k] = the KV samples after v[kW32 - 1].
+template <int KV>
+__device__ __forceinline__ void shift_down_any(float (&v)[kW32], const float (&ext)[8], int r) {
+  float f[kW32 + KV];
+#pragma unroll
+  for (int j = 0; j < kW32; ++j) f[j] = v[j];
+#pragma unroll
+  for (int k = 0; k < KV; ++k) f[kW32 + k] = ext[k];
+  move_down<KV>(f, r);
+#pragma unroll
+  for (int j = 0; j < kW32; ++j) v[j] = f[j];
+}
+// v[j] = (ext ++ v)[KV + j - r]; ext[k] = the KV samples before v[0], in
+// order (ext[KV - 1] is next to v[0]).
+template <int KV>
+__device__ __forceinline__ void shift_up_any(float (&v)[kW32], const float (&ext)[8], int r) {
+  float f[KV + kW32];
+#pragma unroll
+  for (int k = 0; k < KV; ++k) f[k] = ext[k];
+#pragma unroll
+  for (int j = 0; j < kW32; ++j) f[KV + j] = v[j];
+  move_up<KV>(f, r);
+#pragma unroll
+  for (int j = 0; j < kW32; ++j) v[j] = f[KV + j];
+}
+
+// v of the lane a wave shift (CTRL) names, evaluated in every lane before the
+// caller selects: left to itself the compiler may run the DPP move only in
+// the lanes that keep its result, and a source lane outside EXEC reads as 0.
+template <int CTRL>
+__device__ __forceinline__ float dpp_all_lanes(float v) {
+  float d = dpp_f32<CTRL, 0xf>(v);
+  asm volatile("" : "+v"(d));
+  return d;
+}
+
+// The KV samples after this lane's v (lane q+1's v[0 .. KV-1]), or tail[] in
+// lane 63.
+template <int KV>
+__device__ __forceinline__ void next_lane_head(const float (&v)[kW32], const float (&tail)[8],
+                                               float (&ext)[8]) {
+  const bool top = lane_now() == kWave - 1;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if (k < KV) {
+      const float d = dpp_all_lanes<kDppWaveShl1>(v[k]);
+      ext[k] = top ? tail[k] : d;
+    } else {
+      ext[k] = 0.0f;
+    }
+}
+// The KV samples before this lane's v (lane q-1's v[kW32-KV .. kW32-1]), or
+// head[] in lane 0.
+template <int KV>
+__device__ __forceinline__ void prev_lane_tail(const float (&v)[kW32], const float (&head)[8],
+                                               float (&ext)[8]) {
+  const bool bottom = lane_now() == 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if (k < KV) {
+      const float d = dpp_all_lanes<kDppWaveShr1>(v[kW32 - KV + k]);
+      ext[k] = bottom ? head[k] : d;
+    } else {
+      ext[k] = 0.0f;
+    }
+}
+
+// Elements (converted) of the aligned vector at p.
+template <typename TIO>
+__device__ __forceinline__ void load_vec_f(const TIO* p, float (&out)[8]) {
+  using vec_t = typename IOTraits<TIO>::vec_t;
+  const vec_t rv = *reinterpret_cast<const vec_t*>(p);
+  const TIO* e = reinterpret_cast<const TIO*>(&rv);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) out[k] = k < IOTraits<TIO>::kVec ? IOTraits<TIO>::to_f(e[k]) : 0.0f;
+}
+
+// Window -> registers at any element alignment of `win` (its time-ordered
+// start).
+template <typename TIO, bool REV>
+__device__ __forceinline__ void load_window32_any(const TIO* win, u32x4* lds, float (&v)[kW32]) {
+  constexpr int kV = IOTraits<TIO>::kVec;
+  const int r = static_cast<int>((reinterpret_cast<uintptr_t>(win) & 15u) / sizeof(TIO));
+  if (r == 0) {
+    load_window32<TIO, REV>(win, lds, v);
+    return;
+  }
+  const TIO* base = win - r;
+  load_window32<TIO, REV>(base, lds, v);
+  float x[8];
+  load_vec_f<TIO>(base + kWin32, x);  // win[kWin32 - r .. kWin32 - 1] in x[0 .. r-1]
+  float ext[8];
+  if constexpr (REV) {
+    // lane 0: the samples before v[0] are the window's last times, x
+    // reversed (only head[kV - r ..] are used)
+    float head[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) head[k] = k < kV ? x[kV - 1 - k] : 0.0f;
+    prev_lane_tail<kV>(v, head, ext);
+    shift_up_any<kV>(v, ext, r);
+  } else {
+    next_lane_head<kV>(v, x, ext);
+    shift_down_any<kV>(v, ext, r);
+  }
+}
+
+// Registers -> window at any element alignment of `win` (time-ordered start);
+// v is consumed.
+template <typename TIO, bool REV>
+__device__ __forceinline__ void store_window32_any(TIO* win, u32x4* lds, float (&v)[kW32]) {
+  constexpr int kV = IOTraits<TIO>::kVec;
+  const int r = static_cast<int>((reinterpret_cast<uintptr_t>(win) & 15u) / sizeof(TIO));
+  if (r == 0) {
+    store_window32<TIO, REV, false>(win, lds, v, 1.0f);
+    return;
+  }
+  TIO* base = win - r;
+  // the window's last r elements, win[kWin32 - r + i], from the edge lane
+  // (forward: lane 63's top samples v[64 - r + i]; REV: lane 0's bottom
+  // samples, v[r - 1 - i]), before the move below overwrites them
+  if (lane_now() == (REV ? 0 : kWave - 1)) {
+    // g[k] = forward v[kW32 - kV + k], REV v[kV - 1 - k]: win[kWin32 - r + i]
+    // is g[kV - r + i]
+    float g[2 * kV];
+#pragma unroll
+    for (int k = 0; k < kV; ++k) {
+      g[k] = REV ? v[kV - 1 - k] : v[kW32 - kV + k];
+      g[kV + k] = 0.0f;
+    }
+    move_down<kV>(g, kV - r);
+#pragma unroll
+    for (int i = 0; i < kV; ++i)
+      if (i < r) win[kWin32 - r + i] = IOTraits<TIO>::from_f(g[i]);
+  }
+  // aligned layout: forward A[j] = v[j - r] (lane 0's first r belong to the
+  // window before and are not stored), REV A[j] = v[j + r] (lane 63's last
+  // r likewise)
+  const float zero[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  float ext[8];
+  if constexpr (REV) {
+    next_lane_head<kV>(v, zero, ext);
+    shift_down_any<kV>(v, ext, r);
+  } else {
+    prev_lane_tail<kV>(v, zero, ext);
+    shift_up_any<kV>(v, ext, r);
+  }
+  store_window32<TIO, REV, false, false, false, true>(base, lds, v, 1.0f, r);
 }
 
 // Per-wave state shared by the section steps.  A workgroup of NWV waves
@@ -600,7 +831,7 @@ __device__ __forceinline__ void section_carry(const FwdArgs& a, const Win32<NWV>
 
 // Section s's exact output in true units, for the backward pass: applies the
 // pending correction now (which clears it) and stores p_s * v.
-template <typename TIO, int NWV, bool ALIGNED>
+template <typename TIO, int NWV, int SAVED>
 __device__ __forceinline__ void keep_section(const FwdArgs& a, const Win32<NWV>& w, int s,
                                              float (&v)[kW32], float& pw0, float& pw1) {
   const double* tb = w.tf + static_cast<int64_t>(s) * kTableStride;
@@ -613,14 +844,16 @@ __device__ __forceinline__ void keep_section(const FwdArgs& a, const Win32<NWV>&
   const float scale = static_cast<float>(w.ff[s].p);
   // (the alignment of every saved row is known at launch: one store form per
   // kernel keeps the register allocation of the cascade intact)
-  store_window32<TIO, false, true, !ALIGNED>(sw, w.stage, v, scale);
+  store_window32<TIO, false, SAVED == 1, true, SAVED == 2>(sw, w.stage, v, scale);
 }
 
 // All sections of the window (section 0 has no pending correction, the
 // others take the previous section's); on return v holds the cascade output
 // in scaled units with the last section's correction (pw0, pw1) pending
 // (finish_cascade applies it).  SAVED (nonzero): the outputs of
-// sections 0..S-2 are also stored; 1: their rows are 16-byte aligned.
+// sections 0..S-2 are also stored, with 1: streaming stores of 16-byte
+// vectors (every saved window starts on a 128-byte line), 3: regular stores
+// of 16-byte vectors (on 16-byte boundaries), 2: element stores.
 template <typename TIO, int SAVED, bool UNIT, int NWV, bool LB>
 __device__ __forceinline__ void run_cascade(const FwdArgs& a, const Win32<NWV>& w,
                                             float (&v)[kW32], float& pw0, float& pw1) {
@@ -632,14 +865,14 @@ __device__ __forceinline__ void run_cascade(const FwdArgs& a, const Win32<NWV>& 
   GroupPrefetch pf = prefetch_group_in<NWV, LB>(a, w, 0);
   df1_sweep<UNIT, false>(v, w.ff, table_g(w.tf), 0.0f, 0.0f, z0, z1);
   section_carry<NWV, LB>(a, w, 0, lp, z0, z1, pw0, pw1, pf);
-  if (SAVED && S > 1) keep_section<TIO, NWV, SAVED == 1>(a, w, 0, v, pw0, pw1);
+  if (SAVED && S > 1) keep_section<TIO, NWV, SAVED>(a, w, 0, v, pw0, pw1);
   for (int s = 1; s < S; ++s) {
     lp = lane_pow(w.ff[s].mq);
     pf = prefetch_group_in<NWV, LB>(a, w, s);
     const float* pg = table_g(w.tf + static_cast<int64_t>(s - 1) * kTableStride);
     df1_sweep<UNIT, true>(v, w.ff + s, pg, pw0, pw1, z0, z1);
     section_carry<NWV, LB>(a, w, s, lp, z0, z1, pw0, pw1, pf);
-    if (SAVED && s + 1 < S) keep_section<TIO, NWV, SAVED == 1>(a, w, s, v, pw0, pw1);
+    if (SAVED && s + 1 < S) keep_section<TIO, NWV, SAVED>(a, w, s, v, pw0, pw1);
   }
 }
 
@@ -661,43 +894,49 @@ __device__ __forceinline__ void finish_cascade(const FwdArgs& a, const Win32<NWV
   apply_corr<UNIT>(v, last_g(a, w), pw0, pw1, out_scale<UNIT>(a, w));
 }
 
-// One window per wave: load, cascade, store.
-template <typename TIO, bool REV, int SAVED, bool UNIT, int NWV, bool LB>
+// One window per wave: load, cascade, store.  ANY = false: every window of
+// x starts on a 16-byte boundary and every window of y on a 128-byte line
+// (streaming stores).  ANY: windows at any element alignment, regular stores
+// (the moves of load_window32_any / store_window32_any take registers, so
+// aligned launches use the plain form).
+template <typename TIO, bool REV, int SAVED, bool UNIT, int NWV, bool LB, bool ANY>
 __device__ __forceinline__ void window32(const FwdArgs& a, Win32<NWV>& w, u32x4* stage) {
-  const int lane = lane_now();
   const TIO* xr =
       static_cast<const TIO*>(a.x) + row_offset(a, w.c % a.x_rows, a.x_stride, a.x_seg_len);
   TIO* yr = static_cast<TIO*>(a.y) + row_offset(a, w.c, a.y_stride, a.y_seg_len);
   // REV: processing index t is time T-1-t, so the window is the time-ordered
-  // run [T - t0w - 4096, T - t0w)
-  const bool tvec = !REV || (a.T % IOTraits<TIO>::kVec) == 0;
-  const bool xvec = tvec && (reinterpret_cast<uintptr_t>(xr) & 15u) == 0;
-  const bool yvec = tvec && (reinterpret_cast<uintptr_t>(yr) & 15u) == 0;
-  const int64_t t0 = w.t0w + static_cast<int64_t>(lane) * kW32;
-
+  // run [T - t0w - 4096, T - t0w); either way at any element alignment
+  const int64_t tw = REV ? a.T - w.t0w - kWin32 : w.t0w;
   float v[kW32];
-  if (REV) {
-    if (xvec)
-      load_window32<TIO, true>(xr + (a.T - w.t0w - kWin32), stage, v);
-    else
-      load_run_rev<TIO, float, kW32>(xr, a.T - 1 - t0, false, kW32, v);
-  } else if (xvec) {
-    load_window32<TIO, false>(xr + w.t0w, stage, v);
+  if constexpr (ANY) {
+    load_window32_any<TIO, REV>(xr + tw, stage, v);
   } else {
-    load_run<TIO, float, kW32>(xr + t0, false, kW32, v);
+    // the launch checked that every window is aligned; the per-lane form
+    // below stays as the register allocation of these kernels was measured
+    // with it
+    const bool xvec = (reinterpret_cast<uintptr_t>(xr + tw) & 15u) == 0;
+    const int64_t t0 = w.t0w + static_cast<int64_t>(lane_now()) * kW32;
+    if (xvec)
+      load_window32<TIO, REV>(xr + tw, stage, v);
+    else if (REV)
+      load_run_rev<TIO, float, kW32>(xr, a.T - 1 - t0, false, kW32, v);
+    else
+      load_run<TIO, float, kW32>(xr + t0, false, kW32, v);
   }
   float pw0, pw1;
   run_cascade<TIO, SAVED, UNIT, NWV, LB>(a, w, v, pw0, pw1);
   finish_cascade<UNIT>(a, w, v, pw0, pw1);
-  if (REV) {
-    if (yvec)
-      store_window32<TIO, true>(yr + (a.T - w.t0w - kWin32), stage, v, 1.0f);
-    else
-      store_lane32<TIO>(yr + (a.T - 1 - t0), true, v, 1.0f);
-  } else if (yvec) {
-    store_window32<TIO, false>(yr + w.t0w, stage, v, 1.0f);
+  if constexpr (ANY) {
+    store_window32_any<TIO, REV>(yr + tw, stage, v);
   } else {
-    store_lane32<TIO>(yr + t0, false, v, 1.0f);
+    const bool yvec = (reinterpret_cast<uintptr_t>(yr + tw) & 15u) == 0;
+    const int64_t t0 = w.t0w + static_cast<int64_t>(lane_now()) * kW32;
+    if (yvec)
+      store_window32<TIO, REV, true>(yr + tw, stage, v, 1.0f);
+    else if (REV)
+      store_lane32<TIO>(yr + (a.T - 1 - t0), true, v, 1.0f);
+    else
+      store_lane32<TIO>(yr + t0, false, v, 1.0f);
   }
 }
 
@@ -755,27 +994,17 @@ __device__ __forceinline__ void warm_record(const Fast32* fs, int s, float ag0, 
   }
 }
 
-template <typename TIO, bool REV, bool UNIT, int NWV>
+template <typename TIO, bool REV, bool UNIT, int NWV, bool ANY>
 __device__ __forceinline__ void warm_states(const FwdArgs& a, const Win32<NWV>& w, double* prev,
                                             int* prev_ok) {
   const TIO* xr =
       static_cast<const TIO*>(a.x) + row_offset(a, w.c % a.x_rows, a.x_stride, a.x_seg_len);
-  const bool tvec = !REV || (a.T % IOTraits<TIO>::kVec) == 0;
-  const bool xvec = tvec && (reinterpret_cast<uintptr_t>(xr) & 15u) == 0;
+  // REV: processing indices -kWin32 .. -1 are times T + kWin32 - 1 .. T
   float v[kW32];
-  if (REV) {
-    // processing indices -kWin32 .. -1 are times T + kWin32 - 1 .. T
-    if (xvec)
-      load_window32<TIO, true>(xr + a.T, w.stage, v);
-    else
-      load_run_rev<TIO, float, kW32>(
-          xr, a.T + kWin32 - 1 - static_cast<int64_t>(lane_now()) * kW32, false, kW32, v);
-  } else if (xvec) {
-    load_window32<TIO, false>(xr - kWin32, w.stage, v);
-  } else {
-    load_run<TIO, float, kW32>(xr - kWin32 + static_cast<int64_t>(lane_now()) * kW32, false, kW32,
-                               v);
-  }
+  if constexpr (ANY)
+    load_window32_any<TIO, REV>(xr + (REV ? a.T : -kWin32), w.stage, v);
+  else
+    load_window32<TIO, REV>(xr + (REV ? a.T : -kWin32), w.stage, v);
   // section 0 peeled, as in run_cascade
   float z0, z1, x0, x1, ag0, ag1;
   LanePow lp = lane_pow(w.ff->mq);
@@ -809,9 +1038,10 @@ __device__ __forceinline__ void warm_states(const FwdArgs& a, const Win32<NWV>& 
 // that 4 waves share a SIMD (5 measured slower).  The two forms (first input
 // tap 1 with the running b0 product, or b0 kept as a tap) branch before
 // anything is loaded, so no window data is live across the branch.  REV:
-// along decreasing time (TSIIR_REVERSE); SAVED: keep the section outputs
+// along decreasing time (TSIIR_REVERSE); ANY: windows at any element
+// alignment (fwd32_windows_aligned false); SAVED: keep the section outputs
 // (forward_saved), 1 where every saved row is 16-byte aligned, 2 where not.
-template <typename TIO, bool REV, int SAVED, int NWV, bool LB>
+template <typename TIO, bool REV, int SAVED, int NWV, bool LB, bool ANY>
 __global__ __launch_bounds__(64 * NWV) void sos_fwd32_kernel(FwdArgs a,
                                                              const Fast32* __restrict__ fast,
                                                              const double* __restrict__ tables,
@@ -849,9 +1079,9 @@ __global__ __launch_bounds__(64 * NWV) void sos_fwd32_kernel(FwdArgs a,
       // workgroup so that the groups resident on a CU spread it over its SIMDs
       if (w.wv == static_cast<int>(blockIdx.x % NWV)) {
         if (w.ff->unit != 0)
-          warm_states<TIO, REV, true, NWV>(a, w, prev, prev_ok);
+          warm_states<TIO, REV, true, NWV, ANY>(a, w, prev, prev_ok);
         else
-          warm_states<TIO, REV, false, NWV>(a, w, prev, prev_ok);
+          warm_states<TIO, REV, false, NWV, ANY>(a, w, prev, prev_ok);
       }
     } else {
       read_group_in<NWV>(a, w, prev, prev_ok);
@@ -859,9 +1089,9 @@ __global__ __launch_bounds__(64 * NWV) void sos_fwd32_kernel(FwdArgs a,
   }
   __syncthreads();
   if (w.ff->unit != 0)
-    window32<TIO, REV, SAVED, true, NWV, LB>(a, w, stage);
+    window32<TIO, REV, SAVED, true, NWV, LB, ANY>(a, w, stage);
   else
-    window32<TIO, REV, SAVED, false, NWV, LB>(a, w, stage);
+    window32<TIO, REV, SAVED, false, NWV, LB, ANY>(a, w, stage);
 }
 
 // Sixteen lanes per (filter, section) row, four rows per wave: both the FP64
@@ -1010,22 +1240,54 @@ __global__ __launch_bounds__(256) void sos_prep32_kernel(const TC* __restrict__ 
 
 int64_t fwd32_window() { return kWin32; }
 
+// Can the launch take the plain (aligned, streaming-store) kernels: does every
+// window (and every warm-up window of the segmented form) start on a 16-byte
+// boundary in x and on a 128-byte line in y and in the saved rows?  Windows
+// start at a row's offset plus a multiple of kWin32 samples, REV at the row's
+// end minus such a multiple, so this is a question about the row bases and T.
+static bool fwd32_windows_aligned(const FwdArgs& a, size_t esz, bool rev) {
+  auto al = [&](int64_t elems, int64_t bytes) {
+    return (elems * static_cast<int64_t>(esz)) % bytes == 0;
+  };
+  auto ptr_al = [](const void* p, uintptr_t bytes) {
+    return (reinterpret_cast<uintptr_t>(p) % bytes) == 0;
+  };
+  if (!ptr_al(a.x, 16) || !ptr_al(a.y, kLine)) return false;
+  if ((a.x_rows > 1 || a.seg_n > 0) && !al(a.x_stride, 16)) return false;
+  if ((a.B > 1 || a.seg_n > 0) && !al(a.y_stride, kLine)) return false;
+  if (a.seg_n > 0 && (!al(a.x_seg_len, 16) || !al(a.y_seg_len, kLine))) return false;
+  if (rev && !al(a.T, kLine)) return false;  // x and y windows end-anchored
+  // saved rows (s * B + c) * T, forward only
+  if (a.saved != nullptr && (!ptr_al(a.saved, kLine) || !al(a.T, kLine))) return false;
+  return true;
+}
+
 template <typename TIO, bool REV, int NWV, bool LB>
 static void launch_fwd32_io(const FwdArgs& a, const Fast32* fast, int64_t groups, int64_t k0,
                             hipStream_t stream) {
   const dim3 grid(static_cast<unsigned>(groups)), block(kWave * NWV);
-  // saved rows (s * B + c) * T: all 16-byte aligned, or not
-  const bool saved_aligned = (reinterpret_cast<uintptr_t>(a.saved) & 15u) == 0 &&
-                             (a.T * static_cast<int64_t>(sizeof(TIO))) % 16 == 0;
-  if (a.saved == nullptr)
-    hipLaunchKernelGGL((sos_fwd32_kernel<TIO, REV, 0, NWV, LB>), grid, block, 0, stream, a,
-                       fast, a.tables, k0);
-  else if (saved_aligned)
-    hipLaunchKernelGGL((sos_fwd32_kernel<TIO, REV, 1, NWV, LB>), grid, block, 0, stream, a,
-                       fast, a.tables, k0);
-  else
-    hipLaunchKernelGGL((sos_fwd32_kernel<TIO, REV, 2, NWV, LB>), grid, block, 0, stream, a,
-                       fast, a.tables, k0);
+  const bool any = !fwd32_windows_aligned(a, sizeof(TIO), REV);
+#define TSIIR_LAUNCH_FWD32(SAVED, ANY)                                                       \
+  hipLaunchKernelGGL((sos_fwd32_kernel<TIO, REV, SAVED, NWV, LB, ANY>), grid, block, 0, stream, \
+                     a, fast, a.tables, k0)
+  if (a.saved == nullptr) {
+    if (any)
+      TSIIR_LAUNCH_FWD32(0, true);
+    else
+      TSIIR_LAUNCH_FWD32(0, false);
+  } else if constexpr (!REV) {  // saved section outputs: forward only
+    // saved rows (s * B + c) * T: on 128-byte lines (the plain kernels
+    // only), on 16-byte boundaries, or neither
+    const bool saved_vec = (reinterpret_cast<uintptr_t>(a.saved) & 15u) == 0 &&
+                           (a.T * static_cast<int64_t>(sizeof(TIO))) % 16 == 0;
+    if (!any)
+      TSIIR_LAUNCH_FWD32(1, false);
+    else if (saved_vec)
+      TSIIR_LAUNCH_FWD32(3, true);
+    else
+      TSIIR_LAUNCH_FWD32(2, true);
+  }
+#undef TSIIR_LAUNCH_FWD32
 }
 
 // Many channels: groups of four windows, chained, then the leftover full
