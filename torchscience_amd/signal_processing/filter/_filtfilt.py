@@ -3,13 +3,20 @@
 References: src/torchscience/signal_processing/filter/_sosfiltfilt.py:14-173
 (edge padding _pad_signal :176-217) and _filtfilt.py:14-165 (:168-214).
 
-sosfiltfilt makes two passes of the SOS kernel over the padded signal: the
-forward pass from zi = sosfilt_zi(sos) * xp[0], then the backward pass in
-the kernel's reverse-time mode (input read and output written from the end)
-from zi = sosfilt_zi(sos) * y1[-1].  No flipped copies are made; the result
-is the central T samples of the second pass.  When gradients are needed it
-composes the differentiable sosfilt with flips, as the reference does.
-filtfilt follows the reference's composition of lfilter passes.
+sosfiltfilt makes two passes of the SOS kernel over the padded signal
+xp = [left, x, right]: the forward pass from zi = sosfilt_zi(sos) * xp[0],
+then the backward pass in the kernel's reverse-time mode (input read and
+output written from the end) from zi = sosfilt_zi(sos) * y1[-1]; the result
+is the central T samples of the second pass.  The padded signal is never
+built: filtering a concatenation is filtering its pieces with the state
+carried from one to the next, so the forward pass runs over the left
+extension (for its final state only), x and the right extension, and the
+backward pass over the right extension (for its state) and the central part,
+the only output kept.  The two long passes read and write rows of T samples
+at the caller's alignment; no padded or flipped copies are made.  When
+gradients are needed it composes the differentiable sosfilt with flips, as
+the reference does.  filtfilt follows the reference's composition of lfilter
+passes.
 """
 
 from __future__ import annotations
@@ -28,21 +35,25 @@ from ._zi import lfilter_zi, sosfilt_zi
 PadType = Optional[Literal["odd", "even", "constant"]]
 
 
-def _pad_signal(x: Tensor, padtype: str, padlen: int) -> Tensor:
-    """Edge extension along the last axis (_sosfiltfilt.py:176-217)."""
+def _extensions(x: Tensor, padtype: str, padlen: int) -> tuple[Tensor, Tensor]:
+    """The left and right edge extensions along the last axis, padlen
+    samples each (_sosfiltfilt.py:176-217)."""
     if padtype == "odd":
         left = 2 * x[..., 0:1] - x[..., 1:padlen + 1].flip(-1)
         right = 2 * x[..., -1:] - x[..., -padlen - 1:-1].flip(-1)
-        return torch.cat([left, x, right], dim=-1)
+        return left, right
     if padtype == "even":
-        left = x[..., 1:padlen + 1].flip(-1)
-        right = x[..., -padlen - 1:-1].flip(-1)
-        return torch.cat([left, x, right], dim=-1)
+        return x[..., 1:padlen + 1].flip(-1), x[..., -padlen - 1:-1].flip(-1)
     if padtype == "constant":
-        left = x[..., 0:1].expand(*x.shape[:-1], padlen)
-        right = x[..., -1:].expand(*x.shape[:-1], padlen)
-        return torch.cat([left, x, right], dim=-1)
+        return (x[..., 0:1].expand(*x.shape[:-1], padlen),
+                x[..., -1:].expand(*x.shape[:-1], padlen))
     raise ValueError(f"Unknown padtype: {padtype}")
+
+
+def _pad_signal(x: Tensor, padtype: str, padlen: int) -> Tensor:
+    """[left extension, x, right extension] along the last axis."""
+    left, right = _extensions(x, padtype, padlen)
+    return torch.cat([left, x, right], dim=-1)
 
 
 def _needs_grad(*ts) -> bool:
@@ -56,18 +67,28 @@ def sosfiltfilt(sos: Tensor, x: Tensor, axis: int = -1, padtype: PadType = "odd"
     if sos.ndim != 2 or sos.shape[1] != 6:
         raise ValueError("sos must be shape (n_sections, 6)")
     S = sos.shape[0]
+    fast = S > 0 and not (_needs_grad(sos, x) or x.is_complex() or sos.is_complex())
+    # The kernel form takes the coefficients to the host once: the default
+    # padlen and sosfilt_zi are small host computations there, where on the
+    # device each of their scalar tests and solves would wait for the device.
+    sos_h = sos.detach().cpu() if fast else sos
     if padlen is None:
         ntaps = 2 * S + 1
-        ntaps -= min(int((sos[:, 2] == 0).sum().item()), int((sos[:, 5] == 0).sum().item()))
+        ntaps -= min(int((sos_h[:, 2] == 0).sum().item()), int((sos_h[:, 5] == 0).sum().item()))
         padlen = 3 * ntaps
     xm = torch.moveaxis(x, axis, -1)
     T = xm.shape[-1]
     if padtype is not None and padlen >= T:
         raise ValueError(f"padlen ({padlen}) must be less than x.shape[axis] ({T})")
-    xp = _pad_signal(xm, padtype, padlen) if padtype is not None and padlen > 0 else xm
+    pad = padlen if padtype is not None else 0
+    if pad > 0 and padtype not in ("odd", "even", "constant"):
+        raise ValueError(f"Unknown padtype: {padtype}")
+    if fast:
+        return torch.moveaxis(_sosfiltfilt_rows(sos, sos_h, xm, padtype, pad), -1, axis)
+    xp = _pad_signal(xm, padtype, padlen) if pad > 0 else xm
     if S == 0:
         y = xp
-    elif _needs_grad(sos, x) or xp.is_complex() or sos.is_complex():
+    else:
         zi = sosfilt_zi(sos)
         zi_f = zi.unsqueeze(0) * xp[..., 0:1].unsqueeze(-1)
         if xp.ndim == 1:
@@ -79,28 +100,41 @@ def sosfiltfilt(sos: Tensor, x: Tensor, axis: int = -1, padtype: PadType = "odd"
             zi_b = zi_b.squeeze(0)
         y2, _ = sosfilt(sos, yr, axis=-1, zi=zi_b)
         y = y2.flip(-1)
-    else:
-        y = _sosfiltfilt_rows(sos, xp)
-    if padtype is not None and padlen > 0:
+    if pad > 0:
         y = y[..., padlen:-padlen]
     return torch.moveaxis(y, -1, axis)
 
 
-def _sosfiltfilt_rows(sos: Tensor, xp: Tensor) -> Tensor:
-    """Both passes on the HIP kernel; xp is the padded signal, time last."""
-    plan = plan_sosfilt(sos.shape, sos.dtype, xp.shape, xp.dtype, -1, None, None, None)
-    S, Tp, B = plan.n_sections, plan.n_samples, plan.channels
+def _sosfiltfilt_rows(sos: Tensor, sos_h: Tensor, x: Tensor, padtype: PadType,
+                      padlen: int) -> Tensor:
+    """Both passes on the HIP kernel over [left, x, right] (padlen samples
+    of each extension, none when padlen is 0), time last; returns the
+    central T samples of the second pass.  sos_h: the coefficients on the
+    host."""
+    plan = plan_sosfilt(sos.shape, sos.dtype, x.shape, x.dtype, -1, None, None, None)
+    S, T, B = plan.n_sections, plan.n_samples, plan.channels
     comp = plan.compute_dtype
-    sos_c = sos.to(device=xp.device, dtype=comp).reshape(1, S, 6).contiguous()
-    rows = xp.reshape(B, Tp).to(comp).contiguous()
-    zi = sosfilt_zi(sos).to(device=xp.device, dtype=comp)
-    zi_f = (zi.unsqueeze(0) * rows[:, 0:1].unsqueeze(-1)).contiguous()
-    y1, _ = sosfilt_rows_forward(sos_c, rows, zi_f, channels=B, chans_per_filter=B,
-                                 want_zf=False)
-    zi_b = (zi.unsqueeze(0) * y1[:, -1:].unsqueeze(-1)).contiguous()
-    y2, _ = sosfilt_rows_forward(sos_c, y1, zi_b, channels=B, chans_per_filter=B,
-                                 want_zf=False, reverse=True)
-    y = y2.reshape(plan.batch_shape + (Tp,))
+    sos_c = sos.to(device=x.device, dtype=comp).reshape(1, S, 6).contiguous()
+    rows = x.reshape(B, T).to(comp)
+    if rows.stride(-1) != 1:
+        rows = rows.contiguous()
+    zi = sosfilt_zi(sos_h).to(device=x.device, dtype=comp)
+
+    def run(inp, z, want_zf, reverse=False):
+        return sosfilt_rows_forward(sos_c, inp, z.contiguous(), channels=B, chans_per_filter=B,
+                                    want_zf=want_zf, reverse=reverse)
+
+    if padlen > 0:
+        left, right = (e.contiguous() for e in _extensions(rows, padtype, padlen))
+        _, z = run(left, zi.unsqueeze(0) * left[:, :1].unsqueeze(-1), True)
+        y1, z = run(rows, z, True)
+        y1r, _ = run(right, z, False)
+        _, z = run(y1r, zi.unsqueeze(0) * y1r[:, -1:].unsqueeze(-1), True, reverse=True)
+    else:
+        y1, _ = run(rows, zi.unsqueeze(0) * rows[:, :1].unsqueeze(-1), False)
+        z = zi.unsqueeze(0) * y1[:, -1:].unsqueeze(-1)
+    y2, _ = run(y1, z, False, reverse=True)
+    y = y2.reshape(plan.batch_shape + (T,))
     return y if y.dtype == plan.out_dtype else y.to(plan.out_dtype)
 
 
