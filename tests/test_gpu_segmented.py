@@ -17,8 +17,9 @@ word is the device check's verdict (1: the filter forgets its state, the
 segmented form ran) and its third the launch epoch, which every table
 preparation that runs advances by one.  FP32 recurrences prepare the tables
 once for the segmented main pass (which runs each segment's warm-up itself)
-and the look-back form alike: 1, and 2 with a remainder pass, which
-prepares its own.  A lowpass at cutoff 0.002 (||Phi^4096|| ~ 1e-7) must
+and the look-back form alike: 1, and 2 with a remainder pass long enough to
+prepare its own (a short remainder runs the short-row kernel, which
+prepares none).  A lowpass at cutoff 0.002 (||Phi^4096|| ~ 1e-7) must
 fail the check and keep the look-back.
 """
 
@@ -87,8 +88,11 @@ def test_device_check_picks_the_form(oracle_mod):
         y_lb, hdr_lb = _abi_header(st, x, _lib.NO_SEGMENT)
         assert hdr_lb[2] == 1, hdr_lb
         # with a remainder after the segments (128 segments of 16384 and 78
-        # samples), its pass prepares the tables a second time
+        # samples): its pass runs the short-row kernel, which prepares no
+        # tables; one of 12345 samples prepares them a second time
         _, hdr_r = _abi_header(st, torch.cat([x, x[:, :78]], dim=1))
+        assert hdr_r[0] == (1 if forgets else 0) and hdr_r[2] == 1, hdr_r
+        _, hdr_r = _abi_header(st, torch.cat([x, x[:, :12345]], dim=1))
         assert hdr_r[0] == (1 if forgets else 0) and hdr_r[2] == (2 if forgets else 1), hdr_r
         if not forgets:
             # the look-back form's output (its accuracy on this filter, whose

@@ -612,6 +612,106 @@ void launch_prep(const void* sos, int coef_dtype, int64_t n_rows, int log2w, boo
                        hdr, n_epochs, gate, gate_on);
 }
 
+// Short rows: one thread per channel filters its whole row, every section
+// of a sample before the next sample, with every section's state in
+// registers (at most SMAX sections: the loops unroll and test s < S).  The
+// update is the window kernels' DF2T (the order of _lfilter.py:158-166 with
+// fused multiply-adds) in TR, on the coefficients normalised as the table
+// preparation normalises them (by a0, in the coefficient type).  Samples
+// are loaded kShortBlk at a time, one block ahead of the block being
+// filtered.  No workspace is used.
+constexpr int kShortBlk = 8;
+constexpr int64_t kShortMaxT = 512;        // rows up to this length
+constexpr int64_t kShortMaxWork = 4096;    // and samples x sections up to this
+constexpr int kShortMaxS = 8;              // sections held in registers
+template <typename TIO, typename TACC, typename TR, typename TC, int SMAX>
+__global__ __launch_bounds__(kWave) void sos_short_kernel(FwdArgs a, const TC* __restrict__ sos) {
+  if (gated_out(a)) return;
+  const int64_t c = static_cast<int64_t>(blockIdx.x) * kWave + threadIdx.x;
+  if (c >= a.B) return;
+  using io_f = typename std::conditional<std::is_same<TIO, double>::value, double, float>::type;
+  const int S = a.S;
+  const TC* f = sos + (c / a.cpf) * static_cast<int64_t>(S) * 6;
+  const bool has_zi = (a.flags & TSIIR_HAS_ZI) != 0;
+  const int64_t zi_row = zi_row_of(a, c);
+  TR b0[SMAX], b1[SMAX], b2[SMAX], a1[SMAX], a2[SMAX], z0[SMAX], z1[SMAX];
+#pragma unroll
+  for (int s = 0; s < SMAX; ++s) {
+    b0[s] = b1[s] = b2[s] = a1[s] = a2[s] = z0[s] = z1[s] = TR(0);
+    if (s < S) {
+      const TC* r = f + 6 * s;
+      TC cb0 = r[0], cb1 = r[1], cb2 = r[2], ca0 = r[3], ca1 = r[4], ca2 = r[5];
+      if (ca0 != TC(1)) {
+        cb0 = cb0 / ca0;
+        cb1 = cb1 / ca0;
+        cb2 = cb2 / ca0;
+        ca1 = ca1 / ca0;
+        ca2 = ca2 / ca0;
+      }
+      b0[s] = static_cast<TR>(static_cast<double>(cb0));
+      b1[s] = static_cast<TR>(static_cast<double>(cb1));
+      b2[s] = static_cast<TR>(static_cast<double>(cb2));
+      a1[s] = static_cast<TR>(static_cast<double>(ca1));
+      a2[s] = static_cast<TR>(static_cast<double>(ca2));
+      if (has_zi) {
+        z0[s] = static_cast<TR>(zi_in<TACC>(a, (zi_row * S + s) * 2));
+        z1[s] = static_cast<TR>(zi_in<TACC>(a, (zi_row * S + s) * 2 + 1));
+      }
+    }
+  }
+  const TIO* xr =
+      static_cast<const TIO*>(a.x) + row_offset(a, c % a.x_rows, a.x_stride, a.x_seg_len);
+  TIO* yr = static_cast<TIO*>(a.y) + row_offset(a, c, a.y_stride, a.y_seg_len);
+  const bool rev = (a.flags & TSIIR_REVERSE) != 0;  // processing index t is time T-1-t
+  const int64_t T = a.T;
+  // block t0 of the row (samples at or after T read as 0)
+  auto load_block = [&](int64_t t0, TR (&u)[kShortBlk]) {
+#pragma unroll
+    for (int j = 0; j < kShortBlk; ++j) {
+      const int64_t t = t0 + j;
+      u[j] = t < T ? static_cast<TR>(IOTraits<TIO>::to_f(xr[rev ? T - 1 - t : t])) : TR(0);
+    }
+  };
+  TR nxt[kShortBlk];
+  load_block(0, nxt);
+  for (int64_t t0 = 0; t0 < T; t0 += kShortBlk) {
+    TR u[kShortBlk];
+#pragma unroll
+    for (int j = 0; j < kShortBlk; ++j) u[j] = nxt[j];
+    // the next block's loads are in flight while this block is filtered
+    if (t0 + kShortBlk < T) load_block(t0 + kShortBlk, nxt);
+#pragma unroll
+    for (int j = 0; j < kShortBlk; ++j) {
+      TR v = u[j];
+#pragma unroll
+      for (int s = 0; s < SMAX; ++s) {
+        if (s < S) {
+          const TR yv = fma3(b0[s], v, z0[s]);
+          z0[s] = b1[s] * v + (z1[s] - a1[s] * yv);
+          z1[s] = b2[s] * v + (-a2[s] * yv);
+          v = yv;
+        }
+      }
+      u[j] = v;
+      if (t0 + j + 1 == T) break;  // the states stay at the last sample
+    }
+#pragma unroll
+    for (int j = 0; j < kShortBlk; ++j) {
+      const int64_t t = t0 + j;
+      if (t < T) yr[rev ? T - 1 - t : t] = IOTraits<TIO>::from_f(static_cast<io_f>(u[j]));
+    }
+  }
+  if (a.flags & TSIIR_WRITE_ZF) {
+#pragma unroll
+    for (int s = 0; s < SMAX; ++s) {
+      if (s < S) {
+        state_out<TACC>(a.zf, a.flags, (c * S + s) * 2, static_cast<double>(z0[s]));
+        state_out<TACC>(a.zf, a.flags, (c * S + s) * 2 + 1, static_cast<double>(z1[s]));
+      }
+    }
+  }
+}
+
 // zf = zi (or zeros) for an empty time axis.
 template <typename TACC>
 __global__ void copy_state_kernel(const TACC* zi, TACC* zf, int64_t B, int S, int bcast) {
@@ -1189,6 +1289,51 @@ static tsiir_status forward_segmented(const tsiir_sosfilt_desc* d, const SegPlan
   return hipGetLastError() == hipSuccess ? TSIIR_OK : TSIIR_ELAUNCH;
 }
 
+// Short rows (and no section outputs kept): the thread-per-channel kernel.
+// A row of 39 samples took 75-85 µs in the one-wave-per-channel window form
+// over 4096 channels (one lane of each wave busy); the forms chosen by
+// TSIIR_PATH_WINDOW / TSIIR_PATH_CHANNEL and the look-back tests keep theirs.
+static bool use_short(const tsiir_sosfilt_desc* d, const void* saved) {
+  if (saved != nullptr || d->n_sections > kShortMaxS) return false;
+  if (d->flags & (TSIIR_PATH_WINDOW | TSIIR_PATH_CHANNEL | TSIIR_DEBUG_GIVE_UP)) return false;
+  return d->n_samples <= kShortMaxT && d->n_samples * d->n_sections <= kShortMaxWork;
+}
+
+template <typename TIO, typename TACC, typename TR>
+static tsiir_status launch_short(const tsiir_sosfilt_desc* d, const void* x, void* y,
+                                 const void* sos, const void* zi, void* zf, hipStream_t stream,
+                                 const FwdGate* gate) {
+  FwdArgs a;
+  a.x = x;
+  a.y = y;
+  a.zi = zi;
+  a.zf = zf;
+  a.B = d->channels;
+  a.T = d->n_samples;
+  a.x_rows = d->x_rows;
+  a.x_stride = d->x_row_stride;
+  a.y_stride = d->y_row_stride;
+  a.cpf = d->chans_per_filter;
+  a.S = d->n_sections;
+  a.flags = d->flags;
+  a.io_dtype = d->io_dtype;
+  a.gate = gate ? gate->word : nullptr;
+  a.gate_on = gate ? gate->on : 0u;
+  a.seg_n = gate ? gate->seg_n : 0;
+  a.x_seg_len = gate ? gate->x_seg_len : 0;
+  a.y_seg_len = gate ? gate->y_seg_len : 0;
+  a.warm_fused = (gate && gate->warm_fused) ? 1 : 0;
+  a.zi_f64 = (gate && gate->warm_fused) ? (gate->zi_f64 ? 1 : 0) : -1;
+  const dim3 grid(static_cast<unsigned>((d->channels + kWave - 1) / kWave)), block(kWave);
+  if (d->coef_dtype == TSIIR_F64)
+    hipLaunchKernelGGL((sos_short_kernel<TIO, TACC, TR, double, kShortMaxS>), grid, block, 0,
+                       stream, a, static_cast<const double*>(sos));
+  else
+    hipLaunchKernelGGL((sos_short_kernel<TIO, TACC, TR, float, kShortMaxS>), grid, block, 0,
+                       stream, a, static_cast<const float*>(sos));
+  return hipGetLastError() == hipSuccess ? TSIIR_OK : TSIIR_ELAUNCH;
+}
+
 tsiir_status forward_impl(const tsiir_sosfilt_desc* d, const void* x, void* y, const void* sos,
                           const void* zi, void* zf, void* workspace, size_t workspace_bytes,
                           void* stream, void* saved, const FwdGate* gate) {
@@ -1244,6 +1389,18 @@ tsiir_status forward_impl(const tsiir_sosfilt_desc* d, const void* x, void* y, c
     gate = &off;
   }
   const bool comp = d->mode == TSIIR_MODE_COMPENSATED;
+  if (use_short(d, saved)) {
+    switch (d->io_dtype) {
+      case TSIIR_F32:
+        return comp ? launch_short<float, float, double>(d, x, y, sos, zi, zf, st, gate)
+                    : launch_short<float, float, float>(d, x, y, sos, zi, zf, st, gate);
+      case TSIIR_BF16:
+        return comp ? launch_short<bf16_t, float, double>(d, x, y, sos, zi, zf, st, gate)
+                    : launch_short<bf16_t, float, float>(d, x, y, sos, zi, zf, st, gate);
+      default:
+        return launch_short<double, double, double>(d, x, y, sos, zi, zf, st, gate);
+    }
+  }
   if (use_chan(d, saved)) {
     const bool w64 = chan_w_f32() == 64;
     switch (d->io_dtype) {
