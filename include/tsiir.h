@@ -165,6 +165,31 @@ tsiir_status tsiir_sosfilt_forward(const tsiir_sosfilt_desc* desc,
                                    void* workspace, size_t workspace_bytes,
                                    void* stream);
 
+/* Zero-phase filtering: y = sosfiltfilt(sos, x) of the rows desc describes
+ * (no zi / zf: desc->flags may hold only TSIIR_FP32_DF2T).  The rows are
+ * extended at both ends by padlen samples (padtype 1 odd, 2 even, 3
+ * constant; 0 none, with padlen 0), filtered forward from zi_unit * (first
+ * extended sample), then backward from zi_unit * (last sample of the first
+ * pass); y is the central n_samples of the second pass.  zi_unit: FP64
+ * (n_filters, n_sections, 2), the steady state sosfilt_zi of each filter.
+ * The extensions are never stored: two thread-per-channel kernels filter
+ * them and hand FP64 states to the two sosfilt passes over the rows.
+ * `scratch` (tsiir_sosfiltfilt_scratch_bytes, any device memory, no
+ * initialisation) holds the first pass's rows; `workspace` is the sosfilt
+ * workspace of desc (tsiir_sosfilt_workspace_bytes), used by both passes.
+ * At most 8 sections and padlen * n_sections <= 16384, else
+ * TSIIR_EUNSUPPORTED.  Replaces _sosfiltfilt.py:96-173 (the padded
+ * concatenation, the two sosfilt calls and the flips). */
+#define TSIIR_PAD_NONE 0
+#define TSIIR_PAD_ODD 1
+#define TSIIR_PAD_EVEN 2
+#define TSIIR_PAD_CONSTANT 3
+size_t tsiir_sosfiltfilt_scratch_bytes(const tsiir_sosfilt_desc* desc, int64_t padlen);
+tsiir_status tsiir_sosfiltfilt(const tsiir_sosfilt_desc* desc, int padtype, int64_t padlen,
+                               const void* x, void* y, const void* sos, const double* zi_unit,
+                               void* scratch, size_t scratch_bytes,
+                               void* workspace, size_t workspace_bytes, void* stream);
+
 /* tsiir_sosfilt_forward without a workspace argument, for callers that
  * cannot allocate device memory (code under HIP graph capture, serving loops
  * and harnesses that forbid hipMalloc).  No counterpart in the reference.

@@ -399,3 +399,31 @@ def test_pole_radius_routing_criterion():
         ref = max(np.max(np.abs(np.roots(row[3:]))) for row in sos)
         assert abs(r - ref) < 1e-12
     assert _lib.ops().max_pole_radius(torch.tensor([[1.0, 0, 0, 0.0, 1.0, 0.5]])) == float("inf")
+
+
+def test_sosfilt_zi_host_equals_sosfilt_zi():
+    """The host steady state used by the kernel form of sosfiltfilt equals
+    sosfilt_zi (torch ops, the reference's statement order) to rounding, and
+    scipy's sosfilt_zi; a0 != 1 is normalised, a section with a0 == 0 and a
+    singular system take sosfilt_zi's path and errors."""
+    import numpy as np
+    import scipy.signal as ss
+    from torchscience_amd.signal_processing.filter._design import (butterworth_sos,
+                                                                    chebyshev1_sos)
+    from torchscience_amd.signal_processing.filter._zi import sosfilt_zi, sosfilt_zi_host
+    cases = [butterworth_sos(6, [0.05, 0.3], "bandpass"), chebyshev1_sos(16, 0.01, 1.0),
+             ss.butter(4, 0.2, output="sos") * 2.0, ss.cheby2(6, 40, 0.3, output="sos")]
+    for sos in cases:
+        s32 = sos.astype(np.float32).astype(np.float64)  # the values the host copy holds
+        got = sosfilt_zi_host(torch.tensor(sos, dtype=torch.float32)).numpy()
+        ref = sosfilt_zi(torch.tensor(s32, dtype=torch.float64)).numpy()
+        sp = ss.sosfilt_zi(s32)
+        assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+        assert np.max(np.abs(got - sp)) <= 1e-9 * np.max(np.abs(sp))
+    lead0 = torch.tensor([[1.0, 0.5, 0.2, 0.0, 1.0, -0.3]], dtype=torch.float64)
+    assert torch.allclose(sosfilt_zi_host(lead0), sosfilt_zi(lead0))
+    singular = torch.tensor([[1.0, 0.0, 0.0, 1.0, -2.0, 1.0]], dtype=torch.float64)
+    with pytest.raises(Exception) as e_ref:
+        sosfilt_zi(singular)
+    with pytest.raises(type(e_ref.value)):
+        sosfilt_zi_host(singular)

@@ -28,6 +28,10 @@ DEBUG_GIVE_UP = 0x100
 STATE_F64 = 0x200
 NO_SEGMENT = 0x400
 SEGMENT_WARMUP = 4096
+PAD_NONE, PAD_ODD, PAD_EVEN, PAD_CONSTANT = 0, 1, 2, 3
+# tsiir_sosfiltfilt's bounds (beyond them it returns EUNSUPPORTED)
+SOSFILTFILT_MAX_SECTIONS = 8
+SOSFILTFILT_MAX_WORK = 16384  # padlen x sections
 
 # Every symbol include/tsiir.h declares.
 EXPORTED = (
@@ -37,6 +41,8 @@ EXPORTED = (
     "tsiir_workspace_init",
     "tsiir_workspace_set_error_mailbox",
     "tsiir_sosfilt_forward",
+    "tsiir_sosfiltfilt_scratch_bytes",
+    "tsiir_sosfiltfilt",
     "tsiir_sosfilt_forward_noalloc",
     "tsiir_lfilter_forward",
     "tsiir_lfilter_workspace_bytes",
@@ -172,6 +178,11 @@ def load(path: str | None = None) -> ctypes.CDLL:
     L.tsiir_workspace_set_error_mailbox.argtypes = [vp, sz, vp, vp]
     L.tsiir_sosfilt_forward.restype = ctypes.c_int
     L.tsiir_sosfilt_forward.argtypes = [dp, vp, vp, vp, vp, vp, vp, sz, vp]
+    L.tsiir_sosfiltfilt_scratch_bytes.restype = sz
+    L.tsiir_sosfiltfilt_scratch_bytes.argtypes = [dp, ctypes.c_int64]
+    L.tsiir_sosfiltfilt.restype = ctypes.c_int
+    L.tsiir_sosfiltfilt.argtypes = [dp, ctypes.c_int, ctypes.c_int64, vp, vp, vp, vp, vp, sz, vp,
+                                    sz, vp]
     L.tsiir_sosfilt_forward_noalloc.restype = ctypes.c_int
     L.tsiir_sosfilt_forward_noalloc.argtypes = [dp, vp, vp, vp, vp, vp, vp]
     L.tsiir_lfilter_forward.restype = ctypes.c_int

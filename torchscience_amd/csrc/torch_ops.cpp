@@ -5,10 +5,11 @@
 // dispatch key, among them a Meta key for shape inference
 // (src/torchscience/csrc/meta/signal_processing/filter.h:14-34,77-92).  Its
 // sosfilt itself is Python (_sosfilt.py:13-133, _lfilter.py:153-166); here
-// that path becomes two operators:
+// that path becomes these operators:
 //
 //   tsiir::sosfilt_forward   y, zf, saved section outputs
 //   tsiir::sosfilt_backward  dx, dsos, dzi from the saved outputs
+//   tsiir::sosfiltfilt       zero-phase filtering (_sosfiltfilt.py:14-173)
 //
 // with a CUDA-key implementation (HIP devices on ROCm carry PyTorch's CUDA
 // dispatch key) that calls the C ABI of libtsiir (include/tsiir.h) on the
@@ -142,26 +143,39 @@ double max_pole_radius(const at::Tensor& sos) {
 // error on all of them.
 constexpr double kDf2tPoleRadius = 0.995;
 
-// Device coefficients are read once per (storage, version): the answer only
-// picks between two correct kernels, so a stale entry after an address is
-// reused could cost accuracy on such poles, never correctness.
+// Device coefficients are read once per (storage, version): an entry holds a
+// weak reference to the storage it was computed for and is used only while
+// that same storage is alive (the weak reference also keeps its address from
+// being reused by another storage), so a tensor allocated where a freed one
+// lived is read again.  Writes that bypass the version counter (through
+// .data or foreign pointers) are not seen; the answer only picks between two
+// correct kernels, so that could cost accuracy on such poles, never
+// correctness.
+struct RouteEntry {
+  c10::weak_intrusive_ptr<c10::StorageImpl> storage;
+  bool df2t;
+};
 std::mutex g_route_mu;
-std::map<std::tuple<const void*, int64_t, int64_t, int, int>, bool> g_route;
+std::map<std::tuple<const void*, int64_t, int64_t, int, int>, RouteEntry> g_route;
 
 bool needs_fp32_df2t(const at::Tensor& sos) {
   if (sos.device().is_cpu()) return max_pole_radius(sos) > kDf2tPoleRadius;
   const auto key = std::make_tuple(sos.data_ptr(), sos._version(), sos.numel(),
                                    static_cast<int>(sos.scalar_type()),
                                    static_cast<int>(sos.device().index()));
+  const c10::StorageImpl* impl = sos.storage().unsafeGetStorageImpl();
   {
     std::lock_guard<std::mutex> lk(g_route_mu);
     auto it = g_route.find(key);
-    if (it != g_route.end()) return it->second;
+    if (it != g_route.end()) {
+      if (it->second.storage.lock().get() == impl) return it->second.df2t;
+      g_route.erase(it);
+    }
   }
   const bool r = max_pole_radius(sos) > kDf2tPoleRadius;
   std::lock_guard<std::mutex> lk(g_route_mu);
   if (g_route.size() > 256) g_route.clear();
-  g_route[key] = r;
+  g_route.insert_or_assign(key, RouteEntry{sos.storage().getWeakStorageImpl(), r});
   return r;
 }
 
@@ -334,6 +348,49 @@ Bwd sosfilt_backward_hip(const at::Tensor& x, const at::Tensor& y, const at::Ten
   return out;
 }
 
+// Zero-phase filtering of rows x (channels, samples) with one filter sos
+// (1, S, 6) and its steady state zi_unit (1, S, 2) in double: the C ABI's
+// tsiir_sosfiltfilt, the extensions never stored.  The sosfilt workspace of
+// the stream is shared with sosfilt_forward; the first pass's rows go to a
+// scratch tensor of the caching allocator.
+at::Tensor sosfiltfilt_meta(const at::Tensor& x, const at::Tensor& sos, const at::Tensor& zi_unit,
+                            int64_t padtype, int64_t padlen) {
+  check_rows(x, sos, x.size(0), x.size(0));
+  return at::empty({x.size(0), x.size(1)}, x.options());
+}
+
+at::Tensor sosfiltfilt_hip(const at::Tensor& x, const at::Tensor& sos, const at::Tensor& zi_unit,
+                           int64_t padtype, int64_t padlen) {
+  check_rows(x, sos, x.size(0), x.size(0));
+  c10::DeviceGuard guard(x.device());
+  TORCH_CHECK(sos.device() == x.device() && zi_unit.device() == x.device(),
+              "tsiir: sos and zi_unit must be on x's device");
+  TORCH_CHECK(sos.size(0) == 1, "tsiir: sosfiltfilt takes one filter");
+  const int64_t S = sos.size(1), B = x.size(0), T = x.size(1);
+  TORCH_CHECK(zi_unit.scalar_type() == at::kDouble && zi_unit.is_contiguous() &&
+                  zi_unit.numel() == S * 2,
+              "tsiir: zi_unit must be contiguous double (1, n_sections, 2)");
+  const at::ScalarType io = x.scalar_type();
+  at::Tensor y = at::empty({B, T}, x.options());
+  if (B == 0) return y;
+  uint32_t flags = 0;
+  if (io != at::kDouble && needs_fp32_df2t(sos)) flags |= TSIIR_FP32_DF2T;
+  const tsiir_sosfilt_desc d = make_desc(B, T, S, io, sos.scalar_type(), TSIIR_MODE_PLAIN, flags,
+                                         B, B > 1 ? x.stride(0) : T, B);
+  const size_t nws = tsiir_sosfilt_workspace_bytes(&d);
+  const size_t nsc = tsiir_sosfiltfilt_scratch_bytes(&d, padlen);
+  TORCH_CHECK(nws > 0 && nsc > 0, "tsiir: invalid sosfiltfilt descriptor");
+  hipStream_t stream = at::hip::getCurrentHIPStreamMasqueradingAsCUDA(x.device().index()).stream();
+  at::Tensor ws = workspace(x.device(), stream, 0, nws);
+  at::Tensor sc = at::empty({static_cast<int64_t>(nsc)}, x.options().dtype(at::kByte));
+  check_status(tsiir_sosfiltfilt(&d, static_cast<int>(padtype), padlen, x.data_ptr(),
+                                 y.data_ptr(), sos.data_ptr(), zi_unit.data_ptr<double>(),
+                                 sc.data_ptr(), static_cast<size_t>(sc.numel()), ws.data_ptr(),
+                                 static_cast<size_t>(ws.numel()), stream),
+               "tsiir_sosfiltfilt");
+  return y;
+}
+
 }  // namespace
 
 TORCH_LIBRARY(tsiir, m) {
@@ -345,6 +402,7 @@ TORCH_LIBRARY(tsiir, m) {
       "sosfilt_backward(Tensor x, Tensor y, Tensor saved, Tensor sos, Tensor? zi, Tensor dy, "
       "Tensor? dzf, int chans_per_filter, bool zi_broadcast) -> (Tensor dx, Tensor dsos, "
       "Tensor dzi)");
+  m.def("sosfiltfilt(Tensor x, Tensor sos, Tensor zi_unit, int padtype, int padlen) -> Tensor");
   m.def("take_error() -> int", &take_error);
   m.def("max_pole_radius(Tensor sos) -> float", &max_pole_radius);
 }
@@ -352,9 +410,11 @@ TORCH_LIBRARY(tsiir, m) {
 TORCH_LIBRARY_IMPL(tsiir, CUDA, m) {
   m.impl("sosfilt_forward", &sosfilt_forward_hip);
   m.impl("sosfilt_backward", &sosfilt_backward_hip);
+  m.impl("sosfiltfilt", &sosfiltfilt_hip);
 }
 
 TORCH_LIBRARY_IMPL(tsiir, Meta, m) {
   m.impl("sosfilt_forward", &sosfilt_forward_meta);
   m.impl("sosfilt_backward", &sosfilt_backward_meta);
+  m.impl("sosfiltfilt", &sosfiltfilt_meta);
 }

@@ -12,8 +12,11 @@ built: filtering a concatenation is filtering its pieces with the state
 carried from one to the next, so the forward pass runs over the left
 extension (for its final state only), x and the right extension, and the
 backward pass over the right extension (for its state) and the central part,
-the only output kept.  The two long passes read and write rows of T samples
-at the caller's alignment; no padded or flipped copies are made.  When
+the only output kept.  The operator tsiir::sosfiltfilt (C ABI
+tsiir_sosfiltfilt) does this in one call: a thread-per-channel kernel per
+extension and the two sosfilt passes over the rows; beyond its bounds (more
+than 8 sections, padlen x sections above 16384) the same pieces run as
+separate sosfilt calls.  No padded or flipped copies are made.  When
 gradients are needed it composes the differentiable sosfilt with flips, as
 the reference does.  filtfilt follows the reference's composition of lfilter
 passes.
@@ -26,11 +29,12 @@ from typing import Literal, Optional
 import torch
 from torch import Tensor
 
+from ... import _lib
 from ._lfilter import lfilter
-from ._ops import sosfilt_rows_forward
+from ._ops import _require_device, sosfilt_rows_forward
 from ._plan import plan_sosfilt
 from ._sosfilt import sosfilt
-from ._zi import lfilter_zi, sosfilt_zi
+from ._zi import lfilter_zi, sosfilt_zi, sosfilt_zi_host
 
 PadType = Optional[Literal["odd", "even", "constant"]]
 
@@ -118,7 +122,19 @@ def _sosfiltfilt_rows(sos: Tensor, sos_h: Tensor, x: Tensor, padtype: PadType,
     rows = x.reshape(B, T).to(comp)
     if rows.stride(-1) != 1:
         rows = rows.contiguous()
-    zi = sosfilt_zi(sos_h).to(device=x.device, dtype=comp)
+    if S <= _lib.SOSFILTFILT_MAX_SECTIONS and padlen * S <= _lib.SOSFILTFILT_MAX_WORK:
+        # one operator: edge kernels plus the two passes over the rows
+        zu = sosfilt_zi_host(sos_h).reshape(1, S, 2)
+        zu = zu.pin_memory().to(x.device, non_blocking=True) if x.is_cuda else zu.to(x.device)
+        pad = {None: _lib.PAD_NONE, "odd": _lib.PAD_ODD, "even": _lib.PAD_EVEN,
+               "constant": _lib.PAD_CONSTANT}[padtype if padlen > 0 else None]
+        _require_device(sos_c, rows)
+        _lib.ops()
+        y = torch.ops.tsiir.sosfiltfilt(rows, sos_c, zu, pad, padlen)
+        y = y.reshape(plan.batch_shape + (T,))
+        return y if y.dtype == plan.out_dtype else y.to(plan.out_dtype)
+    zi = sosfilt_zi_host(sos_h).to(comp)
+    zi = zi.pin_memory().to(x.device, non_blocking=True) if x.is_cuda else zi.to(x.device)
 
     def run(inp, z, want_zf, reverse=False):
         return sosfilt_rows_forward(sos_c, inp, z.contiguous(), channels=B, chans_per_filter=B,

@@ -7,13 +7,15 @@ sosfilt_zi: src/torchscience/signal_processing/filter/_sosfilt_zi.py:54-86
 lfiltic: src/torchscience/signal_processing/filter/_lfilter.py:181-305
   the DF2T state that continues a filter from its past outputs y and inputs
   x (most recent first): zi[k] = sum_i b[k+i+1] x[i] - a[k+i+1] y[i].
-All are small dense computations done with torch ops on the tensors' device.
+All are small dense computations done with torch ops on the tensors' device;
+sosfilt_zi_host is sosfilt_zi for coefficients already on the host.
 """
 
 from __future__ import annotations
 
 from typing import Optional
 
+import numpy as np
 import torch
 import torch.nn.functional as F
 from torch import Tensor
@@ -63,6 +65,32 @@ def sosfilt_zi(sos: Tensor) -> Tensor:
         zi[s, :] = scale * lfilter_zi(b, a)
         scale = scale * torch.sum(b) / torch.sum(a)
     return zi
+
+
+def sosfilt_zi_host(sos: Tensor) -> Tensor:
+    """sosfilt_zi of host coefficients (S, 6), in FP64 with numpy, as a CPU
+    tensor.  Each section's system (I - A^T) zi = b[1:] - a[1:] b[0]
+    (_lfilter_zi.py:100-128; for a section [[1 + a1, -1], [a2, 1]]) is
+    solved in closed form, then scaled by the DC gains sum(b)/sum(a) of the
+    sections before it (_sosfilt_zi.py:55-86).  Sections with a0 == 0 or a
+    singular system go through sosfilt_zi, with its errors."""
+    if sos.ndim != 2 or sos.shape[1] != 6:
+        raise ValueError("sos must be shape (n_sections, 6)")
+    c = sos.detach().to(device="cpu", dtype=torch.float64).numpy()
+    b, a = c[:, :3], c[:, 3:]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if not bool((a[:, 0] != 0).all()):
+            return sosfilt_zi(torch.from_numpy(c))
+        bn, an = b / a[:, :1], a / a[:, :1]
+        r0 = bn[:, 1] - an[:, 1] * bn[:, 0]
+        r1 = bn[:, 2] - an[:, 2] * bn[:, 0]
+        det = (1.0 + an[:, 1]) + an[:, 2]
+        if not bool((det != 0).all()):
+            return sosfilt_zi(torch.from_numpy(c))
+        zi = np.stack([(r0 + r1) / det, ((1.0 + an[:, 1]) * r1 - an[:, 2] * r0) / det], axis=1)
+        gain = b.sum(axis=1) / a.sum(axis=1)
+        scale = np.concatenate([[1.0], np.cumprod(gain)[:-1]])
+    return torch.from_numpy(zi * scale[:, None])
 
 
 def lfiltic(b: Tensor, a: Tensor, y: Tensor, x: Optional[Tensor] = None) -> Tensor:
