@@ -164,6 +164,9 @@ def filtfilt(b: Tensor, a: Tensor, x: Tensor, axis: int = -1, padtype: PadType =
         raise ValueError("b must be a 1-D tensor")
     if a.ndim != 1:
         raise ValueError("a must be a 1-D tensor")
+    if (max(b.shape[0], a.shape[0]) in (2, 3) and b.is_floating_point() and
+            a.is_floating_point() and not (_needs_grad(b, a, x) or x.is_complex())):
+        return _filtfilt_one_section(b, a, x, axis, padtype, padlen)
     if bool(a[0] != 1.0):
         b = b / a[0]
         a = a / a[0]
@@ -193,3 +196,31 @@ def filtfilt(b: Tensor, a: Tensor, x: Tensor, axis: int = -1, padtype: PadType =
     if padtype is not None and padlen > 0:
         y = y[..., padlen:-padlen]
     return torch.moveaxis(y, -1, axis)
+
+
+def _filtfilt_one_section(b: Tensor, a: Tensor, x: Tensor, axis: int, padtype: PadType,
+                          padlen: Optional[int]) -> Tensor:
+    """filtfilt of a filter of order 1 or 2: lfilter runs it as one second-order
+    section, so its zero-phase form is sosfiltfilt's kernel form with the
+    section [b, a] (zero-padded to three taps each, normalised by a[0] as
+    _filtfilt.py:93-95 does), filtfilt's default padlen 3 * max(len(a),
+    len(b)) and zi = lfilter_zi(b, a), which is sosfilt_zi of the section."""
+    b_h, a_h = b.detach().cpu(), a.detach().cpu()
+    if bool(a_h[0] != 1.0):
+        b_h = b_h / a_h[0]
+        a_h = a_h / a_h[0]
+    n_b, n_a = b_h.shape[0], a_h.shape[0]
+    if padlen is None:
+        padlen = 3 * max(n_b, n_a)
+    xm = torch.moveaxis(x, axis, -1)
+    T = xm.shape[-1]
+    if padtype is not None and padlen >= T:
+        raise ValueError(f"padlen ({padlen}) must be less than x.shape[axis] ({T})")
+    pad = padlen if padtype is not None else 0
+    if pad > 0 and padtype not in ("odd", "even", "constant"):
+        raise ValueError(f"Unknown padtype: {padtype}")
+    sos_h = torch.zeros(1, 6, dtype=torch.promote_types(b_h.dtype, a_h.dtype))
+    sos_h[0, :n_b] = b_h
+    sos_h[0, 3:3 + n_a] = a_h
+    sos = sos_h.pin_memory().to(x.device, non_blocking=True) if x.is_cuda else sos_h.to(x.device)
+    return torch.moveaxis(_sosfiltfilt_rows(sos, sos_h, xm, padtype, pad), -1, axis)
