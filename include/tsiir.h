@@ -108,13 +108,6 @@ typedef enum tsiir_mode {
  * look-back form only (for comparisons). */
 #define TSIIR_NO_SEGMENT 0x400u
 #define TSIIR_SEGMENT_WARMUP 4096
-/* The caller has judged that every filter forgets its state within
- * TSIIR_SEGMENT_WARMUP samples (the criterion above, e.g. computed once on
- * the host for coefficients it reuses): the segmented form then runs without
- * the device check and the look-back form is not launched.  Wrong for a
- * filter that does not forget (segments start from truncated states).
- * Ignored where the segmented form does not apply. */
-#define TSIIR_FORGETS 0x800u
 
 typedef struct tsiir_sosfilt_desc {
   int64_t channels;          /* output channels B                          */
@@ -173,8 +166,7 @@ tsiir_status tsiir_sosfilt_forward(const tsiir_sosfilt_desc* desc,
                                    void* stream);
 
 /* Zero-phase filtering: y = sosfiltfilt(sos, x) of the rows desc describes
- * (no zi / zf: desc->flags may hold only TSIIR_FP32_DF2T and TSIIR_FORGETS,
- * which both passes over the rows take).  The rows are
+ * (no zi / zf: desc->flags may hold only TSIIR_FP32_DF2T).  The rows are
  * extended at both ends by padlen samples (padtype 1 odd, 2 even, 3
  * constant; 0 none, with padlen 0), filtered forward from zi_unit * (first
  * extended sample), then backward from zi_unit * (last sample of the first

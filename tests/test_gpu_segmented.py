@@ -279,23 +279,3 @@ def test_public_sosfilt_takes_the_segmented_form(oracle_mod):
     err = relerr(y.cpu().numpy(), ref)
     print(f"  max rel err {err:.3e}", flush=True)
     assert err < 1e-5
-
-
-def test_host_verdict_equals_device_check():
-    """The operators pass TSIIR_FORGETS from the host verdict
-    (tsiir::forgets_within_warmup) instead of running the device check: the
-    two agree on filters on both sides of the criterion."""
-    from torchscience_amd import _lib
-    from torchscience_amd.signal_processing.filter._design import (butterworth_sos,
-                                                                    chebyshev1_sos)
-    dev = _dev()
-    x = torch.randn(8, 1 << 21, device=dev)
-    for name, sos in (("bandpass", _bandpass()), ("lowpass 0.002", _lowpass(0.002)),
-                      ("lowpass 0.02", _lowpass(0.02)), ("lowpass 0.2", _lowpass(0.2)),
-                      ("cheby1 0.05", chebyshev1_sos(8, 0.05, 1.0)),
-                      ("butter 16 0.1", butterworth_sos(16, 0.1))):
-        s32 = sos.astype(np.float32)
-        _, hdr = _abi_header(torch.tensor(s32[None], device=dev), x)
-        host = _lib.ops().forgets_within_warmup(torch.tensor(s32))
-        print(f"{name}: device {hdr[0]} host {host}", flush=True)
-        assert bool(hdr[0]) == host, name

@@ -427,25 +427,3 @@ def test_sosfilt_zi_host_equals_sosfilt_zi():
         sosfilt_zi(singular)
     with pytest.raises(type(e_ref.value)):
         sosfilt_zi_host(singular)
-
-
-def test_host_forgetting_verdict():
-    """tsiir::forgets_within_warmup (the host form of the segmented form's
-    device check, which the operators pass down as TSIIR_FORGETS): the
-    bandpass of the long-sequence workload forgets its state within 4096
-    samples, a lowpass at 0.002 (pole radius ~0.9995) does not, nor does a
-    cascade of more than 16 sections or a section with a0 = 0; per-filter
-    batches forget only when every filter does."""
-    import numpy as np
-    from torchscience_amd.signal_processing.filter._design import butterworth_sos
-    f = _lib.ops().forgets_within_warmup
-    bp = butterworth_sos(6, [0.05, 0.3], "bandpass")
-    lp = butterworth_sos(8, 0.002)
-    assert f(torch.tensor(bp)) and f(torch.tensor(bp, dtype=torch.float32))
-    assert not f(torch.tensor(lp))
-    assert not f(torch.tensor(np.concatenate([bp] * 3)))  # 18 sections
-    bad = bp.copy()
-    bad[0, 3] = 0.0
-    assert not f(torch.tensor(bad))
-    assert f(torch.tensor(np.stack([bp, bp])))
-    assert not f(torch.tensor(np.stack([bp, np.concatenate([lp, lp[:2]])])))

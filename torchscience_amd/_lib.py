@@ -27,7 +27,6 @@ FP32_DF2T = 0x80
 DEBUG_GIVE_UP = 0x100
 STATE_F64 = 0x200
 NO_SEGMENT = 0x400
-FORGETS = 0x800
 SEGMENT_WARMUP = 4096
 PAD_NONE, PAD_ODD, PAD_EVEN, PAD_CONSTANT = 0, 1, 2, 3
 # tsiir_sosfiltfilt's bounds (beyond them it returns EUNSUPPORTED)

@@ -733,7 +733,7 @@ __global__ void seg_state_in_kernel(const uint32_t* gate, const double* __restri
                                     const TS* __restrict__ zi, int bcast,
                                     double* __restrict__ zin, int64_t rows, int64_t nseg, int S,
                                     int rev) {
-  if (gate != nullptr && *gate == 0u) return;
+  if (*gate == 0u) return;
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t per = static_cast<int64_t>(S) * 2;
   if (i >= rows * per) return;
@@ -753,7 +753,7 @@ template <typename TS>
 __global__ void seg_state_out_kernel(const uint32_t* gate, const double* __restrict__ zall,
                                      TS* __restrict__ zf, int64_t channels, int64_t nseg, int S,
                                      int rev) {
-  if (gate != nullptr && *gate == 0u) return;
+  if (*gate == 0u) return;
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t per = static_cast<int64_t>(S) * 2;
   if (i >= channels * per) return;
@@ -1193,23 +1193,18 @@ static tsiir_status forward_segmented(const tsiir_sosfilt_desc* d, const SegPlan
                                       bool* fused) {
   const int S = d->n_sections;
   WsHeader* hdr = reinterpret_cast<WsHeader*>(ws);
-  // TSIIR_FORGETS: the caller's verdict stands in for the device check and
-  // every launch below runs ungated
-  const bool judged = (d->flags & TSIIR_FORGETS) != 0;
-  const uint32_t* gate = judged ? nullptr : &hdr->seg_gate;
+  const uint32_t* gate = &hdr->seg_gate;
   const int single = n_filters == 1 ? 1 : 0;
-  if (!judged) {
-    if (!single) hipLaunchKernelGGL(seg_gate_init_kernel, dim3(1), dim3(1), 0, st, hdr);
-    if (d->coef_dtype == TSIIR_F64)
-      hipLaunchKernelGGL(seg_check_kernel<double>, dim3(static_cast<unsigned>(n_filters)),
-                         dim3(kSegCheckThreads), 0, st, static_cast<const double*>(sos), S,
-                         kSegSquarings, d->coef_dtype, single, hdr);
-    else
-      hipLaunchKernelGGL(seg_check_kernel<float>, dim3(static_cast<unsigned>(n_filters)),
-                         dim3(kSegCheckThreads), 0, st, static_cast<const float*>(sos), S,
-                         kSegSquarings, d->coef_dtype, single, hdr);
-    if (hipGetLastError() != hipSuccess) return TSIIR_ELAUNCH;
-  }
+  if (!single) hipLaunchKernelGGL(seg_gate_init_kernel, dim3(1), dim3(1), 0, st, hdr);
+  if (d->coef_dtype == TSIIR_F64)
+    hipLaunchKernelGGL(seg_check_kernel<double>, dim3(static_cast<unsigned>(n_filters)),
+                       dim3(kSegCheckThreads), 0, st, static_cast<const double*>(sos), S,
+                       kSegSquarings, d->coef_dtype, single, hdr);
+  else
+    hipLaunchKernelGGL(seg_check_kernel<float>, dim3(static_cast<unsigned>(n_filters)),
+                       dim3(kSegCheckThreads), 0, st, static_cast<const float*>(sos), S,
+                       kSegSquarings, d->coef_dtype, single, hdr);
+  if (hipGetLastError() != hipSuccess) return TSIIR_ELAUNCH;
   const int64_t esz = static_cast<int64_t>(io_bytes(d->io_dtype));
   double* warm = reinterpret_cast<double*>(ws + p.warm_off);
   double* zin = reinterpret_cast<double*>(ws + p.zin_off);
@@ -1386,7 +1381,7 @@ tsiir_status forward_impl(const tsiir_sosfilt_desc* d, const void* x, void* y, c
     bool fused = false;
     const tsiir_status rc =
         forward_segmented(d, sp, x, y, sos, zi, zf, ws, st, g.n_filters, &fused);
-    if (rc != TSIIR_OK || (d->flags & TSIIR_FORGETS)) return rc;
+    if (rc != TSIIR_OK) return rc;
     // the look-back form runs only where the check cleared the gate (on the
     // tables the segmented form built, for the FP32-recurrence forms)
     off = FwdGate{&reinterpret_cast<WsHeader*>(ws)->seg_gate, 0u};

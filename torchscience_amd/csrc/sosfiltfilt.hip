@@ -261,8 +261,7 @@ tsiir_status tsiir_sosfiltfilt(const tsiir_sosfilt_desc* desc, int padtype, int6
                                size_t workspace_bytes, void* stream) {
   if (!desc) return TSIIR_EINVAL;
   const tsiir_sosfilt_desc d = *desc;
-  if ((d.flags & ~(TSIIR_FP32_DF2T | TSIIR_FORGETS)) != 0 || d.mode != TSIIR_MODE_PLAIN)
-    return TSIIR_EINVAL;
+  if ((d.flags & ~TSIIR_FP32_DF2T) != 0 || d.mode != TSIIR_MODE_PLAIN) return TSIIR_EINVAL;
   if (padtype < TSIIR_PAD_NONE || padtype > TSIIR_PAD_CONSTANT || padlen < 0) return TSIIR_EINVAL;
   if ((padtype == TSIIR_PAD_NONE) != (padlen == 0)) return TSIIR_EINVAL;
   if (d.n_samples < 1 || padlen >= d.n_samples || d.n_sections < 1) return TSIIR_EINVAL;
@@ -299,7 +298,7 @@ tsiir_status tsiir_sosfiltfilt(const tsiir_sosfilt_desc* desc, int padtype, int6
   launch_edge_io<false>(e, d.io_dtype, d.coef_dtype, st);
   if (hipGetLastError() != hipSuccess) return TSIIR_ELAUNCH;
 
-  const uint32_t keep = d.flags & (TSIIR_FP32_DF2T | TSIIR_FORGETS);
+  const uint32_t keep = d.flags & TSIIR_FP32_DF2T;
   tsiir_sosfilt_desc m = d;
   m.flags = TSIIR_HAS_ZI | TSIIR_WRITE_ZF | TSIIR_STATE_F64 | keep;
   m.y_row_stride = L.y1_stride;

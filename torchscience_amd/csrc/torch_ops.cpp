@@ -11,9 +11,6 @@
 //   tsiir::sosfilt_backward  dx, dsos, dzi from the saved outputs
 //   tsiir::sosfiltfilt       zero-phase filtering (_sosfiltfilt.py:14-173)
 //
-// and two host helpers of the routing below (max_pole_radius,
-// forgets_within_warmup).
-//
 // with a CUDA-key implementation (HIP devices on ROCm carry PyTorch's CUDA
 // dispatch key) that calls the C ABI of libtsiir (include/tsiir.h) on the
 // current HIP stream, and a Meta implementation (shapes only) so FakeTensor
@@ -30,12 +27,10 @@
 #include <c10/core/DeviceGuard.h>
 #include <torch/library.h>
 
-#include <algorithm>
 #include <cmath>
 #include <map>
 #include <mutex>
 #include <tuple>
-#include <vector>
 
 #include "tsiir.h"
 
@@ -148,95 +143,23 @@ double max_pole_radius(const at::Tensor& sos) {
 // error on all of them.
 constexpr double kDf2tPoleRadius = 0.995;
 
-// Does every filter forget its state within TSIIR_SEGMENT_WARMUP samples?
-// The criterion of the segmented form's device check (seg_check_kernel in
-// sosfilt_fwd.hip), evaluated on the host: P, the cascade's one-sample DF2T
-// state transition (2S x 2S, column j the state one zero-input step after
-// unit state e_j), squared up to 12 times until its infinity norm falls
-// below 2^-64 (12 squarings: P^4096).  a: CPU double (n_filters, S, 6).
-bool forgets_within_warmup(const at::Tensor& a) {
-  const int64_t F = a.size(0), S = a.size(1);
-  if (S > 16) return false;
-  const int n = static_cast<int>(2 * S);
-  const double* c = a.data_ptr<double>();
-  std::vector<double> P(n * n), Q(n * n), col(n);
-  for (int64_t f = 0; f < F; ++f) {
-    const double* sf = c + f * S * 6;
-    for (int j = 0; j < n; ++j) {
-      for (int i = 0; i < n; ++i) col[i] = (i == j) ? 1.0 : 0.0;
-      double u = 0.0;
-      for (int64_t s = 0; s < S; ++s) {
-        const double a0 = sf[6 * s + 3];
-        if (!(a0 != 0.0) || !std::isfinite(a0)) return false;
-        const double b0 = sf[6 * s] / a0, b1 = sf[6 * s + 1] / a0, b2 = sf[6 * s + 2] / a0;
-        const double a1 = sf[6 * s + 4] / a0, a2 = sf[6 * s + 5] / a0;
-        const double yv = b0 * u + col[2 * s];
-        const double n0 = b1 * u - a1 * yv + col[2 * s + 1];
-        const double n1 = b2 * u - a2 * yv;
-        col[2 * s] = n0;
-        col[2 * s + 1] = n1;
-        u = yv;
-      }
-      for (int i = 0; i < n; ++i) P[i * n + j] = col[i];
-    }
-    bool small = false;
-    for (int q = 0; q <= 12; ++q) {
-      double m = 0.0;
-      for (int i = 0; i < n; ++i) {
-        double row = 0.0;
-        for (int j = 0; j < n; ++j) row += std::fabs(P[i * n + j]);
-        if (!std::isfinite(row)) return false;
-        m = std::max(m, row);
-      }
-      if (m < 0x1p-64) {
-        small = true;
-        break;
-      }
-      if (q == 12) break;
-      for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) {
-          double acc = 0.0;
-          for (int k = 0; k < n; ++k) acc = std::fma(P[i * n + k], P[k * n + j], acc);
-          Q[i * n + j] = acc;
-        }
-      P.swap(Q);
-    }
-    if (!small) return false;
-  }
-  return true;
-}
-
-// What the host needs to know of a coefficient tensor: the FP32 routing
-// (a pole beyond kDf2tPoleRadius) and whether the segmented form applies
-// without a device check (TSIIR_FORGETS).
-struct CoefFacts {
-  bool df2t;
-  bool forgets;
-};
-
-CoefFacts coef_facts_now(const at::Tensor& sos) {
-  const at::Tensor a =
-      sos.detach().to(at::kCPU, at::kDouble).contiguous().reshape({-1, sos.size(-2), 6});
-  return CoefFacts{max_pole_radius(a) > kDf2tPoleRadius, forgets_within_warmup(a)};
-}
-
 // Device coefficients are read once per (storage, version): an entry holds a
 // weak reference to the storage it was computed for and is used only while
 // that same storage is alive (the weak reference also keeps its address from
 // being reused by another storage), so a tensor allocated where a freed one
 // lived is read again.  Writes that bypass the version counter (through
-// .data or foreign pointers) are not seen: the routing then picks the less
-// accurate of two correct kernels for such poles, and a stale verdict that
-// a filter forgets would truncate segment states.
+// .data or foreign pointers) are not seen; the answer only picks between two
+// correct kernels, so that could cost accuracy on such poles, never
+// correctness.
 struct RouteEntry {
   c10::weak_intrusive_ptr<c10::StorageImpl> storage;
-  CoefFacts facts;
+  bool df2t;
 };
 std::mutex g_route_mu;
 std::map<std::tuple<const void*, int64_t, int64_t, int, int>, RouteEntry> g_route;
 
-CoefFacts coef_facts(const at::Tensor& sos) {
-  if (sos.device().is_cpu()) return coef_facts_now(sos);
+bool needs_fp32_df2t(const at::Tensor& sos) {
+  if (sos.device().is_cpu()) return max_pole_radius(sos) > kDf2tPoleRadius;
   const auto key = std::make_tuple(sos.data_ptr(), sos._version(), sos.numel(),
                                    static_cast<int>(sos.scalar_type()),
                                    static_cast<int>(sos.device().index()));
@@ -245,15 +168,15 @@ CoefFacts coef_facts(const at::Tensor& sos) {
     std::lock_guard<std::mutex> lk(g_route_mu);
     auto it = g_route.find(key);
     if (it != g_route.end()) {
-      if (it->second.storage.lock().get() == impl) return it->second.facts;
+      if (it->second.storage.lock().get() == impl) return it->second.df2t;
       g_route.erase(it);
     }
   }
-  const CoefFacts f = coef_facts_now(sos);
+  const bool r = max_pole_radius(sos) > kDf2tPoleRadius;
   std::lock_guard<std::mutex> lk(g_route_mu);
   if (g_route.size() > 256) g_route.clear();
-  g_route.insert_or_assign(key, RouteEntry{sos.storage().getWeakStorageImpl(), f});
-  return f;
+  g_route.insert_or_assign(key, RouteEntry{sos.storage().getWeakStorageImpl(), r});
+  return r;
 }
 
 // ---- operators -----------------------------------------------------------
@@ -345,9 +268,8 @@ Fwd sosfilt_forward_hip(const at::Tensor& x, const at::Tensor& sos,
   }
   if (want_zf) flags |= TSIIR_WRITE_ZF;
   if (f64_state) flags |= TSIIR_STATE_F64;
-  const CoefFacts facts = coef_facts(sos);
-  if (io != at::kDouble && mode == TSIIR_MODE_PLAIN && facts.df2t) flags |= TSIIR_FP32_DF2T;
-  if (facts.forgets && !(flags & TSIIR_NO_SEGMENT)) flags |= TSIIR_FORGETS;
+  if (io != at::kDouble && mode == TSIIR_MODE_PLAIN && needs_fp32_df2t(sos))
+    flags |= TSIIR_FP32_DF2T;
   const tsiir_sosfilt_desc d = make_desc(channels, T, S, io, sos.scalar_type(), mode, flags,
                                          x.size(0), x.size(0) > 1 ? x.stride(0) : T,
                                          chans_per_filter);
@@ -452,9 +374,7 @@ at::Tensor sosfiltfilt_hip(const at::Tensor& x, const at::Tensor& sos, const at:
   at::Tensor y = at::empty({B, T}, x.options());
   if (B == 0) return y;
   uint32_t flags = 0;
-  const CoefFacts facts = coef_facts(sos);
-  if (io != at::kDouble && facts.df2t) flags |= TSIIR_FP32_DF2T;
-  if (facts.forgets) flags |= TSIIR_FORGETS;
+  if (io != at::kDouble && needs_fp32_df2t(sos)) flags |= TSIIR_FP32_DF2T;
   const tsiir_sosfilt_desc d = make_desc(B, T, S, io, sos.scalar_type(), TSIIR_MODE_PLAIN, flags,
                                          B, B > 1 ? x.stride(0) : T, B);
   const size_t nws = tsiir_sosfilt_workspace_bytes(&d);
@@ -485,9 +405,6 @@ TORCH_LIBRARY(tsiir, m) {
   m.def("sosfiltfilt(Tensor x, Tensor sos, Tensor zi_unit, int padtype, int padlen) -> Tensor");
   m.def("take_error() -> int", &take_error);
   m.def("max_pole_radius(Tensor sos) -> float", &max_pole_radius);
-  m.def("forgets_within_warmup(Tensor sos) -> bool", [](const at::Tensor& sos) {
-    return coef_facts_now(sos).forgets;
-  });
 }
 
 TORCH_LIBRARY_IMPL(tsiir, CUDA, m) {
