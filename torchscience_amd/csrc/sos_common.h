@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 #include "tsiir.h"
 
 namespace tsiir {
@@ -576,6 +578,95 @@ __device__ __forceinline__ void store_run_rev(TIO* p, int64_t n_hi, bool vec, in
 #pragma unroll
     for (int j = 0; j < W; ++j)
       if (j < m) p[n_hi - j] = Tr::from_f(v[j]);
+  }
+}
+
+// ---- per-lane runs at any element alignment ----------------------------------
+//
+// A lane's run of W elements that starts r elements past a 16-byte boundary
+// (0 < r < kVec) moves as the W / kVec + 1 aligned vectors that cover it:
+// each holds at least one element of the run, so none reaches a page the
+// run does not touch.  The elements are moved by r in registers, one select
+// per element for each bit of r (fold expressions over constant indices, so
+// the arrays stay in registers).  Stores write the run's own elements only:
+// the first and last vectors element by element.
+
+template <typename T, int S, int N, size_t... J>
+__device__ __forceinline__ void run_select_above(T (&f)[N], bool c, std::index_sequence<J...>) {
+  ((f[J] = c ? f[J + S] : f[J]), ...);
+}
+template <typename T, int S, int N, size_t... J>
+__device__ __forceinline__ void run_select_below(T (&f)[N], bool c, std::index_sequence<J...>) {
+  ((f[N - 1 - J] = c ? f[N - 1 - J - S] : f[N - 1 - J]), ...);
+}
+// f[j] = f[j + m] (j < N - m), m < KV, KV a power of two <= 8
+template <int KV, typename T, int N>
+__device__ __forceinline__ void run_move_down(T (&f)[N], int m) {
+  if constexpr (KV > 1) run_select_above<T, 1>(f, (m & 1) != 0, std::make_index_sequence<N - 1>{});
+  if constexpr (KV > 2) run_select_above<T, 2>(f, (m & 2) != 0, std::make_index_sequence<N - 2>{});
+  if constexpr (KV > 4) run_select_above<T, 4>(f, (m & 4) != 0, std::make_index_sequence<N - 4>{});
+}
+// f[j] = f[j - m] (j >= m), m < KV, KV a power of two <= 8
+template <int KV, typename T, int N>
+__device__ __forceinline__ void run_move_up(T (&f)[N], int m) {
+  if constexpr (KV > 1) run_select_below<T, 1>(f, (m & 1) != 0, std::make_index_sequence<N - 1>{});
+  if constexpr (KV > 2) run_select_below<T, 2>(f, (m & 2) != 0, std::make_index_sequence<N - 2>{});
+  if constexpr (KV > 4) run_select_below<T, 4>(f, (m & 4) != 0, std::make_index_sequence<N - 4>{});
+}
+
+// v[j] = p[j] (REV: v[j] = p[W - 1 - j]), p at any element alignment.
+template <typename TIO, typename TACC, int W, bool REV = false>
+__device__ __forceinline__ void load_run_any(const TIO* p, TACC (&v)[W]) {
+  using Tr = IOTraits<TIO>;
+  constexpr int kV = Tr::kVec;
+  const int r = static_cast<int>((reinterpret_cast<uintptr_t>(p) & 15u) / sizeof(TIO));
+  const typename Tr::vec_t* q = reinterpret_cast<const typename Tr::vec_t*>(p - r);
+  TACC f[W + kV];
+#pragma unroll
+  for (int i = 0; i <= W / kV; ++i) {
+    // (with r = 0 the last vector holds no element of the run: not read)
+    typename Tr::vec_t rv{};
+    if (i < W / kV || r != 0) rv = q[i];
+    const TIO* e = reinterpret_cast<const TIO*>(&rv);
+#pragma unroll
+    for (int j = 0; j < kV; ++j) f[i * kV + j] = static_cast<TACC>(Tr::to_f(e[j]));
+  }
+  run_move_down<kV>(f, r);
+#pragma unroll
+  for (int j = 0; j < W; ++j) v[j] = f[REV ? W - 1 - j : j];
+}
+
+// p[j] = v[j] (REV: p[W - 1 - j] = v[j]), p at any element alignment.
+template <typename TIO, typename TACC, int W, bool REV = false>
+__device__ __forceinline__ void store_run_any(TIO* p, const TACC (&v)[W]) {
+  using Tr = IOTraits<TIO>;
+  constexpr int kV = Tr::kVec;
+  const int r = static_cast<int>((reinterpret_cast<uintptr_t>(p) & 15u) / sizeof(TIO));
+  typename Tr::vec_t* q = reinterpret_cast<typename Tr::vec_t*>(p - r);
+  // h = [kV zeros, the run in time order, kV zeros], moved up by r: h[kV + j]
+  // is then element j of the aligned vectors that cover the run
+  TACC h[W + 2 * kV];
+#pragma unroll
+  for (int j = 0; j < kV; ++j) h[j] = h[W + kV + j] = TACC(0);
+#pragma unroll
+  for (int j = 0; j < W; ++j) h[kV + j] = v[REV ? W - 1 - j : j];
+  run_move_up<kV>(h, r);
+  TIO* base = p - r;
+#pragma unroll
+  for (int i = 0; i <= W / kV; ++i) {
+    if (i == 0 || i == W / kV) {
+#pragma unroll
+      for (int j = 0; j < kV; ++j) {
+        const int k = i * kV + j;
+        if (k >= r && k < W + r) base[k] = Tr::from_f(h[kV + k]);
+      }
+    } else {
+      typename Tr::vec_t rv;
+      TIO* e = reinterpret_cast<TIO*>(&rv);
+#pragma unroll
+      for (int j = 0; j < kV; ++j) e[j] = Tr::from_f(h[kV + i * kV + j]);
+      q[i] = rv;
+    }
   }
 }
 

@@ -147,19 +147,34 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
   // the wave's samples are the time-ordered run ending at T-1-t0w.
   const bool rev = (a.flags & TSIIR_REVERSE) != 0;
   const bool tvec = !rev || (a.T % IOTraits<TIO>::kVec) == 0;
-  const bool xvec = !TAIL && tvec && (reinterpret_cast<uintptr_t>(xr) & 15u) == 0;
-  const bool yvec = !TAIL && tvec && (reinterpret_cast<uintptr_t>(yr) & 15u) == 0;
+  // a lane whose W samples all lie before T reads and writes them as 16-byte
+  // vectors (its run starts at a multiple of W, so the row's alignment is its
+  // own); in the TAIL variant the lane that holds sample T-1 goes element by
+  // element and the lanes after it do nothing
+  const bool whole = !TAIL || m == W;
+  const bool xvec = whole && tvec && (reinterpret_cast<uintptr_t>(xr) & 15u) == 0;
+  const bool yvec = whole && tvec && (reinterpret_cast<uintptr_t>(yr) & 15u) == 0;
+  // the partial window's whole runs at any alignment (load_run_any); the
+  // full-window variants keep element runs there, as the register moves
+  // would cost them 20-40 VGPRs and an occupancy step
+  const bool any_run = TAIL && whole;
   const int64_t t_rev = a.T - t0w - wwin;  // first time of the wave's run (REVERSE)
   TIO* my_lds = lds + ((LDS && !TAIL) ? wv * LdsStage<TIO, W>::kElems : 0);
 
   TACC v[W];
+  // (partial window: whole runs of rows that are not 16-byte aligned move
+  // as aligned vectors, the lane that holds sample T-1 element by element)
   if (rev) {
     if (LDS && !TAIL && xvec)
       load_window_lds<TIO, TACC, W, true>(xr + t_rev, my_lds, v);
+    else if (any_run && !xvec)
+      load_run_any<TIO, TACC, W, true>(xr + (a.T - t0 - W), v);
     else
       load_run_rev<TIO, TACC, W>(xr, a.T - 1 - t0, xvec, m, v);
   } else if (LDS && !TAIL && xvec) {
     load_window_lds<TIO, TACC, W>(xr + t0w, my_lds, v);
+  } else if (any_run && !xvec) {
+    load_run_any<TIO, TACC, W>(xr + t0, v);
   } else {
     load_run<TIO, TACC, W>(xr + t0, xvec, m, v);
   }
@@ -372,9 +387,11 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
         pw1 = 0;
       }
       TIO* sr = static_cast<TIO*>(a.saved) + (static_cast<int64_t>(s) * a.B + c) * a.T;
-      const bool svec = !TAIL && (reinterpret_cast<uintptr_t>(sr) & 15u) == 0;
+      const bool svec = whole && (reinterpret_cast<uintptr_t>(sr) & 15u) == 0;
       if (LDS && !TAIL && svec)
         store_window_lds<TIO, TACC, W>(sr + t0w, my_lds, v);
+      else if (any_run && !svec)
+        store_run_any<TIO, TACC, W>(sr + t0, v);
       else
         store_run<TIO, TACC, W>(sr + t0, svec, m, v);
     }
@@ -385,10 +402,14 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
   if (rev) {
     if (LDS && !TAIL && yvec)
       store_window_lds<TIO, TACC, W, true>(yr + t_rev, my_lds, v);
+    else if (any_run && !yvec)
+      store_run_any<TIO, TACC, W, true>(yr + (a.T - t0 - W), v);
     else
       store_run_rev<TIO, TACC, W>(yr, a.T - 1 - t0, yvec, m, v);
   } else if (LDS && !TAIL && yvec) {
     store_window_lds<TIO, TACC, W>(yr + t0w, my_lds, v);
+  } else if (any_run && !yvec) {
+    store_run_any<TIO, TACC, W>(yr + t0, v);
   } else {
     store_run<TIO, TACC, W>(yr + t0, yvec, m, v);
   }
