@@ -185,10 +185,11 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
                         (TAIL ? (t0 < a.T && t0 + W >= a.T) : (t0 + W == a.T));
   const int64_t zi_row = zi_row_of(a, c);
 
-  // CHAINED: lane l < S fetches window k-1's inclusive state of section l
+  // CHAINED and TAIL: lane l < S fetches window k-1's inclusive state of
+  // section l (one round trip for all sections)
   double pv0 = 0.0, pv1 = 0.0;
   bool pok = true;
-  if (CHAINED && k > 0 && lane < a.S) {
+  if ((CHAINED || TAIL) && k > 0 && lane < a.S) {
     const uint64_t* st = a.status + (((k - 1) * a.B + c) * a.S + lane) * kStatusWords;
     pok = read_pair(st + 4, flag, pv0, pv1);
   }
@@ -310,8 +311,9 @@ __global__ __launch_bounds__(64 * NW) void sos_fwd_kernel(FwdArgs a, int64_t k_t
         publish_pair(st + 4, Mwin.a * sw0 + Mwin.b * sw1 + g0, Mwin.c * sw0 + Mwin.d * sw1 + g1,
                      flag, lane);
       } else if (k > 0) {
-        const uint64_t* st = a.status + (((k - 1) * a.B + c) * a.S + s) * kStatusWords;
-        if (!read_pair(st + 4, flag, sw0, sw1)) give_up(a.hdr, 2u, sw0, sw1);
+        sw0 = __shfl(pv0, s);
+        sw1 = __shfl(pv1, s);
+        if (!__shfl(static_cast<int>(pok), s)) give_up(a.hdr, 2u, sw0, sw1);
       }
       if (NW > 1 && lane == 0) {
         wsin[0] = sw0;
