@@ -8,7 +8,9 @@
 // per-lane scratch above that.  Each lane streams its row in blocks of 16
 // samples (8 for orders 9..16), the next block's loads issued before the
 // current block's recurrence, so the serial chain does not wait on memory
-// every sample.
+// every sample.  TSIIR_REVERSE runs along decreasing time (processing index
+// n is time T-1-n, for input and output alike): the second pass of filtfilt
+// without flipped copies.
 
 #include "sos_common.h"
 #include "tsiir.h"
@@ -19,6 +21,18 @@ namespace tsiir {
 // recurrence would not fit the registers
 template <int NMAX>
 constexpr int kGenBlk = NMAX <= 8 ? 16 : (NMAX <= 16 ? 8 : 1);
+
+// A row along processing index n: p[n], or with REV p[-n], p then being the
+// row's last element (time T-1).
+template <typename P, bool REV>
+struct RowRef {
+  P* p;
+  __device__ __forceinline__ P& operator[](int64_t n) const { return p[REV ? -n : n]; }
+};
+template <bool REV, typename P>
+__device__ __forceinline__ RowRef<P, REV> row_ref(P* row, int64_t T) {
+  return RowRef<P, REV>{REV ? row + (T - 1) : row};
+}
 
 template <typename TACC, int NMAX>
 __device__ inline void df2t_step(const TACC (&bn)[NMAX + 1], const TACC (&an)[NMAX + 1],
@@ -52,10 +66,9 @@ __device__ inline void load_coef(const TACC* __restrict__ coef, int N, TACC (&bn
 // DF2T over xr[n0, n1) from state z (updated in place); outputs to yr when
 // OUT.  Whole blocks of BLK samples have the next block's loads in flight
 // while the current block runs through the recurrence.
-template <typename TACC, int NMAX, bool OUT>
+template <typename TACC, int NMAX, bool OUT, typename XR, typename YR>
 __device__ inline void run_span(const TACC (&bn)[NMAX + 1], const TACC (&an)[NMAX + 1], int N,
-                                TACC (&z)[NMAX], const TACC* __restrict__ xr,
-                                TACC* __restrict__ yr, int64_t n0, int64_t n1) {
+                                TACC (&z)[NMAX], XR xr, YR yr, int64_t n0, int64_t n1) {
   constexpr int BLK = kGenBlk<NMAX>;
   const int64_t nb = BLK > 1 ? n0 + (n1 - n0) / BLK * BLK : n0;
   if constexpr (BLK > 1) {
@@ -87,7 +100,7 @@ __device__ inline void run_span(const TACC (&bn)[NMAX + 1], const TACC (&an)[NMA
 }
 
 // Serial form: one lane per channel walks the whole row.
-template <typename TACC, int NMAX>
+template <typename TACC, int NMAX, bool REV>
 __global__ __launch_bounds__(64) void lfilter_general_kernel(
     const TACC* __restrict__ coef, int N, const TACC* __restrict__ x, TACC* __restrict__ y,
     int64_t B, int64_t T, int64_t x_rows, int64_t x_stride, int64_t y_stride,
@@ -100,8 +113,8 @@ __global__ __launch_bounds__(64) void lfilter_general_kernel(
 #pragma unroll
   for (int k = 0; k < NMAX; ++k)
     z[k] = (k < N && (flags & TSIIR_HAS_ZI)) ? zi[zrow * N + k] : TACC(0);
-  const TACC* xr = x + (c % x_rows) * x_stride;
-  TACC* yr = y + c * y_stride;
+  const auto xr = row_ref<REV>(x + (c % x_rows) * x_stride, T);
+  const auto yr = row_ref<REV>(y + c * y_stride, T);
   if constexpr (kGenBlk<NMAX> > 1) {
     run_span<TACC, NMAX, true>(bn, an, N, z, xr, yr, 0, T);
   } else {
@@ -138,7 +151,7 @@ __global__ __launch_bounds__(64) void lfilter_general_kernel(
 // sample by sample.
 constexpr int kGenChunk = 256;
 
-template <typename TACC, int NMAX, bool SECOND>
+template <typename TACC, int NMAX, bool SECOND, bool REV>
 __global__ __launch_bounds__(64) void lfilter_chunk_kernel(
     const TACC* __restrict__ coef, int N, const TACC* __restrict__ x, TACC* __restrict__ y,
     int64_t T, int64_t M, int64_t G, int64_t x_rows, int64_t x_stride, int64_t y_stride,
@@ -163,8 +176,8 @@ __global__ __launch_bounds__(64) void lfilter_chunk_kernel(
 #pragma unroll
   for (int k = 0; k < NMAX; ++k)
     z[k] = (SECOND && valid && k < N) ? static_cast<TACC>(st[k]) : TACC(0);
-  const TACC* xr = x + (c % x_rows) * x_stride;
-  TACC* yr = y + c * y_stride;
+  const auto xr = row_ref<REV>(x + (c % x_rows) * x_stride, T);
+  const auto yr = row_ref<REV>(y + c * y_stride, T);
   TACC cur[BLK], nxt[BLK];
   if (full) {
 #pragma unroll
@@ -190,20 +203,22 @@ __global__ __launch_bounds__(64) void lfilter_chunk_kernel(
       // vector e = l + 64 q: source lane e / (BLK / V), its samples from
       // V (e % (BLK / V)); stored whole when the row is 16-byte aligned
       constexpr int VPR = BLK / V;                  // vectors per tile row
-      const bool vec_ok = ((reinterpret_cast<uintptr_t>(yr) | (y_stride * sizeof(TACC))) & 15) == 0;
+      const bool vec_ok =
+          !REV && ((reinterpret_cast<uintptr_t>(y + c * y_stride) | (y_stride * sizeof(TACC))) &
+                   15) == 0;
 #pragma unroll
       for (int q = 0; q < VPR; ++q) {
         const int e = l + 64 * q;
         const int src = e / VPR, part = e % VPR;
         const int64_t js = g * 64 + src;
         if ((js + 1) * kGenChunk > T) continue;
-        TACC* dst = yr + js * kGenChunk + sb * BLK + part * V;
+        const int64_t k0 = js * kGenChunk + sb * BLK + part * V;
         const TACC* from = tile + src * TS + part * V;
         if (vec_ok) {
-          *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(from);
+          *reinterpret_cast<uint4*>(&yr[k0]) = *reinterpret_cast<const uint4*>(from);
         } else {
 #pragma unroll
-          for (int v = 0; v < V; ++v) dst[v] = from[v];
+          for (int v = 0; v < V; ++v) yr[k0 + v] = from[v];
         }
       }
       __syncthreads();
@@ -340,7 +355,7 @@ int64_t general_chunks(const tsiir_sosfilt_desc* d) {
   return M >= 2 ? M : 0;
 }
 
-template <typename TACC>
+template <typename TACC, bool REV>
 static tsiir_status launch_general(const tsiir_sosfilt_desc* d, const void* coef, const void* x,
                                    void* y, const void* zi, void* zf, void* ws, hipStream_t st) {
   const int N = d->n_sections;
@@ -365,14 +380,14 @@ static tsiir_status launch_general(const tsiir_sosfilt_desc* d, const void* coef
                          static_cast<TACC*>(zf), d->flags);
     };
     if (N <= 4)
-      go(lfilter_chunk_kernel<TACC, 4, false>, lfilter_scan_kernel<TACC, 4>,
-         lfilter_chunk_kernel<TACC, 4, true>);
+      go(lfilter_chunk_kernel<TACC, 4, false, REV>, lfilter_scan_kernel<TACC, 4>,
+         lfilter_chunk_kernel<TACC, 4, true, REV>);
     else if (N <= 8)
-      go(lfilter_chunk_kernel<TACC, 8, false>, lfilter_scan_kernel<TACC, 8>,
-         lfilter_chunk_kernel<TACC, 8, true>);
+      go(lfilter_chunk_kernel<TACC, 8, false, REV>, lfilter_scan_kernel<TACC, 8>,
+         lfilter_chunk_kernel<TACC, 8, true, REV>);
     else
-      go(lfilter_chunk_kernel<TACC, 16, false>, lfilter_scan_kernel<TACC, 16>,
-         lfilter_chunk_kernel<TACC, 16, true>);
+      go(lfilter_chunk_kernel<TACC, 16, false, REV>, lfilter_scan_kernel<TACC, 16>,
+         lfilter_chunk_kernel<TACC, 16, true, REV>);
     return hipGetLastError() == hipSuccess ? TSIIR_OK : TSIIR_ELAUNCH;
   }
   const unsigned blocks = static_cast<unsigned>((d->channels + 63) / 64);
@@ -383,19 +398,30 @@ static tsiir_status launch_general(const tsiir_sosfilt_desc* d, const void* coef
                        static_cast<const TACC*>(zi), static_cast<TACC*>(zf), d->flags);
   };
   if (N <= 4)
-    go(lfilter_general_kernel<TACC, 4>);
+    go(lfilter_general_kernel<TACC, 4, REV>);
   else if (N <= 8)
-    go(lfilter_general_kernel<TACC, 8>);
+    go(lfilter_general_kernel<TACC, 8, REV>);
   else if (N <= 16)
-    go(lfilter_general_kernel<TACC, 16>);
+    go(lfilter_general_kernel<TACC, 16, REV>);
   else
-    go(lfilter_general_kernel<TACC, 64>);
+    go(lfilter_general_kernel<TACC, 64, REV>);
   return hipGetLastError() == hipSuccess ? TSIIR_OK : TSIIR_ELAUNCH;
 }
 
 }  // namespace tsiir
 
 using namespace tsiir;
+
+static tsiir_status launch_general_io(const tsiir_sosfilt_desc* d, const void* coef,
+                                      const void* x, void* y, const void* zi, void* zf, void* ws,
+                                      hipStream_t st) {
+  const bool rev = (d->flags & TSIIR_REVERSE) != 0;
+  if (d->io_dtype == TSIIR_F64)
+    return rev ? launch_general<double, true>(d, coef, x, y, zi, zf, ws, st)
+               : launch_general<double, false>(d, coef, x, y, zi, zf, ws, st);
+  return rev ? launch_general<float, true>(d, coef, x, y, zi, zf, ws, st)
+             : launch_general<float, false>(d, coef, x, y, zi, zf, ws, st);
+}
 
 static tsiir_status check_general(const tsiir_sosfilt_desc* d, const void* coef, const void* x,
                                   void* y, const void* zi, void* zf) {
@@ -416,9 +442,7 @@ extern "C" tsiir_status tsiir_lfilter_forward(const tsiir_sosfilt_desc* d, const
   const tsiir_status s = check_general(d, coef, x, y, zi, zf);
   if (s != TSIIR_OK || d->channels == 0) return s;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (d->io_dtype == TSIIR_F64)
-    return launch_general<double>(d, coef, x, y, zi, zf, nullptr, st);
-  return launch_general<float>(d, coef, x, y, zi, zf, nullptr, st);
+  return launch_general_io(d, coef, x, y, zi, zf, nullptr, st);
 }
 
 extern "C" size_t tsiir_lfilter_workspace_bytes(const tsiir_sosfilt_desc* d) {
@@ -436,6 +460,5 @@ extern "C" tsiir_status tsiir_lfilter_forward_ws(const tsiir_sosfilt_desc* d, co
   if (need > 0 && (!workspace || workspace_bytes < need)) return TSIIR_EWORKSPACE;
   hipStream_t st = static_cast<hipStream_t>(stream);
   void* ws = need > 0 ? workspace : nullptr;
-  if (d->io_dtype == TSIIR_F64) return launch_general<double>(d, coef, x, y, zi, zf, ws, st);
-  return launch_general<float>(d, coef, x, y, zi, zf, ws, st);
+  return launch_general_io(d, coef, x, y, zi, zf, ws, st);
 }

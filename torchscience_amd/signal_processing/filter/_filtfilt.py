@@ -164,9 +164,9 @@ def filtfilt(b: Tensor, a: Tensor, x: Tensor, axis: int = -1, padtype: PadType =
         raise ValueError("b must be a 1-D tensor")
     if a.ndim != 1:
         raise ValueError("a must be a 1-D tensor")
-    if (max(b.shape[0], a.shape[0]) in (2, 3) and b.is_floating_point() and
+    if (max(b.shape[0], a.shape[0]) >= 2 and b.is_floating_point() and
             a.is_floating_point() and not (_needs_grad(b, a, x) or x.is_complex())):
-        return _filtfilt_one_section(b, a, x, axis, padtype, padlen)
+        return _filtfilt_kernel_form(b, a, x, axis, padtype, padlen)
     if bool(a[0] != 1.0):
         b = b / a[0]
         a = a / a[0]
@@ -198,13 +198,16 @@ def filtfilt(b: Tensor, a: Tensor, x: Tensor, axis: int = -1, padtype: PadType =
     return torch.moveaxis(y, -1, axis)
 
 
-def _filtfilt_one_section(b: Tensor, a: Tensor, x: Tensor, axis: int, padtype: PadType,
+def _filtfilt_kernel_form(b: Tensor, a: Tensor, x: Tensor, axis: int, padtype: PadType,
                           padlen: Optional[int]) -> Tensor:
-    """filtfilt of a filter of order 1 or 2: lfilter runs it as one second-order
-    section, so its zero-phase form is sosfiltfilt's kernel form with the
-    section [b, a] (zero-padded to three taps each, normalised by a[0] as
-    _filtfilt.py:93-95 does), filtfilt's default padlen 3 * max(len(a),
-    len(b)) and zi = lfilter_zi(b, a), which is sosfilt_zi of the section."""
+    """filtfilt of order >= 1 without gradients, its pieces on the device.
+    b and a come to the host once and are normalised by a[0] there
+    (_filtfilt.py:93-95); padlen defaults to 3 * max(len(a), len(b)).
+    Orders 1 and 2: lfilter runs the filter as one second-order section, so
+    this is sosfiltfilt's kernel form with the section [b, a] (zi =
+    lfilter_zi(b, a) is sosfilt_zi of the section).  Higher orders: the same
+    pieces through the general-order kernel (tsiir_lfilter_forward_ws), the
+    backward passes in its reverse-time mode."""
     b_h, a_h = b.detach().cpu(), a.detach().cpu()
     if bool(a_h[0] != 1.0):
         b_h = b_h / a_h[0]
@@ -219,8 +222,49 @@ def _filtfilt_one_section(b: Tensor, a: Tensor, x: Tensor, axis: int, padtype: P
     pad = padlen if padtype is not None else 0
     if pad > 0 and padtype not in ("odd", "even", "constant"):
         raise ValueError(f"Unknown padtype: {padtype}")
-    sos_h = torch.zeros(1, 6, dtype=torch.promote_types(b_h.dtype, a_h.dtype))
-    sos_h[0, :n_b] = b_h
-    sos_h[0, 3:3 + n_a] = a_h
-    sos = sos_h.pin_memory().to(x.device, non_blocking=True) if x.is_cuda else sos_h.to(x.device)
-    return torch.moveaxis(_sosfiltfilt_rows(sos, sos_h, xm, padtype, pad), -1, axis)
+    if max(n_b, n_a) <= 3:
+        sos_h = torch.zeros(1, 6, dtype=torch.promote_types(b_h.dtype, a_h.dtype))
+        sos_h[0, :n_b] = b_h
+        sos_h[0, 3:3 + n_a] = a_h
+        sos = _to_device(sos_h, x.device)
+        return torch.moveaxis(_sosfiltfilt_rows(sos, sos_h, xm, padtype, pad), -1, axis)
+    return torch.moveaxis(_filtfilt_general_rows(b_h, a_h, xm, padtype, pad), -1, axis)
+
+
+def _to_device(t: Tensor, dev: torch.device) -> Tensor:
+    """A small host tensor on dev, through pinned memory (no synchronisation)."""
+    return t.pin_memory().to(dev, non_blocking=True) if dev.type == "cuda" else t.to(dev)
+
+
+def _filtfilt_general_rows(b_h: Tensor, a_h: Tensor, x: Tensor, padtype: PadType,
+                           padlen: int) -> Tensor:
+    """Both passes of filtfilt (order >= 3) over [left, x, right] with the
+    general-order kernel, time last, as _sosfiltfilt_rows does for sections:
+    the left extension for its state, the rows, the right extension forward
+    and then backward for the backward pass's state, the rows backward."""
+    from ._lfilter import _general_rows
+    _require_device(x)
+    out = torch.promote_types(torch.promote_types(b_h.dtype, a_h.dtype), x.dtype)
+    comp = torch.float64 if out == torch.float64 else torch.float32
+    n = max(b_h.shape[0], a_h.shape[0])
+    pad_to = lambda v: torch.nn.functional.pad(v.to(torch.float64), (0, n - v.shape[0]))
+    coef = _to_device(torch.cat([pad_to(b_h), pad_to(a_h)]).to(comp), x.device)
+    zi = _to_device(lfilter_zi(b_h.to(torch.float64), a_h.to(torch.float64)).to(comp),
+                    x.device)
+    batch, T = x.shape[:-1], x.shape[-1]
+    rows = x.reshape(-1, T).to(comp).contiguous()
+
+    def run(inp, z, want_zf, reverse=False):
+        return _general_rows(coef, inp, z.contiguous(), want_zf, False, reverse=reverse)
+
+    if padlen > 0:
+        left, right = (e.contiguous() for e in _extensions(rows, padtype, padlen))
+        _, z = run(left, zi.unsqueeze(0) * left[:, :1], True)
+        y1, z = run(rows, z, True)
+        y1r, _ = run(right, z, False)
+        _, z = run(y1r, zi.unsqueeze(0) * y1r[:, -1:], True, reverse=True)
+    else:
+        y1, _ = run(rows, zi.unsqueeze(0) * rows[:, :1], False)
+        z = zi.unsqueeze(0) * y1[:, -1:]
+    y2, _ = run(y1, z, False, reverse=True)
+    return y2.reshape(batch + (T,)).to(out)

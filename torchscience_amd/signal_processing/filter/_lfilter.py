@@ -241,10 +241,11 @@ def _lfilter_as_section(b, a, x, axis, zi, order):
 
 
 @torch.compiler.disable
-def _general_rows(coef, rows, zi_r, want_zf, zi_broadcast):
+def _general_rows(coef, rows, zi_r, want_zf, zi_broadcast, reverse=False):
     """DF2T of order N = len(coef) // 2 - 1 (coef = b then a, padded) over
     rows (B, T) through tsiir_lfilter_forward_ws (time-parallel for orders up
-    to 16).  Returns (y, zf or None)."""
+    to 16); reverse: along decreasing time (zi the state at the end, zf after
+    the first sample).  Returns (y, zf or None)."""
     B, T = rows.shape
     N = coef.numel() // 2 - 1
     dev = rows.device
@@ -254,6 +255,8 @@ def _general_rows(coef, rows, zi_r, want_zf, zi_broadcast):
         flags |= _lib.HAS_ZI | (_lib.ZI_BROADCAST if zi_broadcast else 0)
     if want_zf:
         flags |= _lib.WRITE_ZF
+    if reverse:
+        flags |= _lib.REVERSE
     zf = torch.empty((B, N), dtype=rows.dtype, device=dev) if want_zf else None
     if B == 0:
         return y, zf
