@@ -213,7 +213,9 @@ tsiir_status tsiir_sosfilt_forward_noalloc(const tsiir_sosfilt_desc* desc,
  * _lfilter.py:114-117 does) in coef_dtype == io_dtype (F32 or F64); they are
  * normalised by a[0] when a[0] != 1.  zi / zf are (channels, N).
  * Replaces the per-sample DF2T loop _lfilter.py:153-166 for orders above 2
- * (orders <= 2 go through tsiir_sosfilt_forward as one section). */
+ * (orders <= 2 go through tsiir_sosfilt_forward as one section).
+ * TSIIR_REVERSE runs along decreasing time (zi the state at the end, zf
+ * after the first sample), as for tsiir_sosfilt_forward. */
 tsiir_status tsiir_lfilter_forward(const tsiir_sosfilt_desc* desc, const void* coef,
                                    const void* x, void* y, const void* zi, void* zf,
                                    void* stream);
