@@ -78,7 +78,11 @@ def sosfiltfilt(sos: Tensor, x: Tensor, axis: int = -1, padtype: PadType = "odd"
     sos_h = sos.detach().cpu() if fast else sos
     if padlen is None:
         ntaps = 2 * S + 1
-        ntaps -= min(int((sos_h[:, 2] == 0).sum().item()), int((sos_h[:, 5] == 0).sum().item()))
+        if fast:
+            c = sos_h.to(torch.float64).tolist()
+            ntaps -= min(sum(r[2] == 0 for r in c), sum(r[5] == 0 for r in c))
+        else:
+            ntaps -= min(int((sos[:, 2] == 0).sum().item()), int((sos[:, 5] == 0).sum().item()))
         padlen = 3 * ntaps
     xm = torch.moveaxis(x, axis, -1)
     T = xm.shape[-1]

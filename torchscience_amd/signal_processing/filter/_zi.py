@@ -15,7 +15,6 @@ from __future__ import annotations
 
 from typing import Optional
 
-import numpy as np
 import torch
 import torch.nn.functional as F
 from torch import Tensor
@@ -68,29 +67,30 @@ def sosfilt_zi(sos: Tensor) -> Tensor:
 
 
 def sosfilt_zi_host(sos: Tensor) -> Tensor:
-    """sosfilt_zi of host coefficients (S, 6), in FP64 with numpy, as a CPU
-    tensor.  Each section's system (I - A^T) zi = b[1:] - a[1:] b[0]
-    (_lfilter_zi.py:100-128; for a section [[1 + a1, -1], [a2, 1]]) is
-    solved in closed form, then scaled by the DC gains sum(b)/sum(a) of the
-    sections before it (_sosfilt_zi.py:55-86).  Sections with a0 == 0 or a
-    singular system go through sosfilt_zi, with its errors."""
+    """sosfilt_zi of host coefficients (S, 6), in FP64 (Python floats: the
+    arrays are a few dozen values), as a CPU tensor.  Each section's system
+    (I - A^T) zi = b[1:] - a[1:] b[0] (_lfilter_zi.py:100-128; for a section
+    [[1 + a1, -1], [a2, 1]]) is solved in closed form, then scaled by the DC
+    gains sum(b)/sum(a) of the sections before it (_sosfilt_zi.py:55-86).
+    Sections with a0 == 0, a singular system or a zero sum(a) go through
+    sosfilt_zi, with its errors and infinities."""
     if sos.ndim != 2 or sos.shape[1] != 6:
         raise ValueError("sos must be shape (n_sections, 6)")
-    c = sos.detach().to(device="cpu", dtype=torch.float64).numpy()
-    b, a = c[:, :3], c[:, 3:]
-    with np.errstate(divide="ignore", invalid="ignore"):
-        if not bool((a[:, 0] != 0).all()):
-            return sosfilt_zi(torch.from_numpy(c))
-        bn, an = b / a[:, :1], a / a[:, :1]
-        r0 = bn[:, 1] - an[:, 1] * bn[:, 0]
-        r1 = bn[:, 2] - an[:, 2] * bn[:, 0]
-        det = (1.0 + an[:, 1]) + an[:, 2]
-        if not bool((det != 0).all()):
-            return sosfilt_zi(torch.from_numpy(c))
-        zi = np.stack([(r0 + r1) / det, ((1.0 + an[:, 1]) * r1 - an[:, 2] * r0) / det], axis=1)
-        gain = b.sum(axis=1) / a.sum(axis=1)
-        scale = np.concatenate([[1.0], np.cumprod(gain)[:-1]])
-    return torch.from_numpy(zi * scale[:, None])
+    c = sos.detach().to(device="cpu", dtype=torch.float64)
+    out = []
+    scale = 1.0
+    for b0, b1, b2, a0, a1, a2 in c.tolist():
+        sa = a0 + a1 + a2
+        if a0 == 0.0 or sa == 0.0:
+            return sosfilt_zi(c)
+        n0, n1, n2, m1, m2 = b0 / a0, b1 / a0, b2 / a0, a1 / a0, a2 / a0
+        det = (1.0 + m1) + m2
+        if det == 0.0:
+            return sosfilt_zi(c)
+        r0, r1 = n1 - m1 * n0, n2 - m2 * n0
+        out.append((scale * ((r0 + r1) / det), scale * (((1.0 + m1) * r1 - m2 * r0) / det)))
+        scale = scale * ((b0 + b1 + b2) / sa)
+    return torch.tensor(out, dtype=torch.float64).reshape(-1, 2)
 
 
 def lfiltic(b: Tensor, a: Tensor, y: Tensor, x: Optional[Tensor] = None) -> Tensor:
