@@ -141,6 +141,10 @@ __device__ __forceinline__ void load_coefficients(Cascade<TR>& q, const TC* f) {
   }
 }
 
+// Samples are read kEdgeBlk at a time before they are filtered, so the
+// loads of a block are in flight together rather than one per step.
+constexpr int kEdgeBlk = 8;
+
 // One thread per channel.  RIGHT = false: step 1 above; RIGHT: step 3.
 template <typename TIO, typename TR, typename TC, bool RIGHT>
 __global__ __launch_bounds__(kWave) void sos_edge_kernel(EdgeArgs e) {
@@ -155,8 +159,15 @@ __global__ __launch_bounds__(kWave) void sos_edge_kernel(EdgeArgs e) {
   double* zo = e.z_out + c * e.S * 2;
   if constexpr (!RIGHT) {
     q.start(zu, static_cast<TR>(e.padlen > 0 ? extension<TIO, false>(xr, e, 0) : rd(xr, 0)));
-    for (int64_t k = 0; k < e.padlen; ++k)
-      q.step(static_cast<TR>(extension<TIO, false>(xr, e, k)));
+    for (int64_t k0 = 0; k0 < e.padlen; k0 += kEdgeBlk) {
+      TR u[kEdgeBlk];
+#pragma unroll
+      for (int j = 0; j < kEdgeBlk; ++j)
+        u[j] = k0 + j < e.padlen ? static_cast<TR>(extension<TIO, false>(xr, e, k0 + j)) : TR(0);
+#pragma unroll
+      for (int j = 0; j < kEdgeBlk; ++j)
+        if (k0 + j < e.padlen) q.step(u[j]);
+    }
   } else {
     const double* zi = e.z_in + c * e.S * 2;
 #pragma unroll
@@ -167,16 +178,35 @@ __global__ __launch_bounds__(kWave) void sos_edge_kernel(EdgeArgs e) {
       }
     }
     TIO* yb = static_cast<TIO*>(e.ybuf) + c * (e.padlen > 0 ? e.padlen : 1);
-    for (int64_t k = 0; k < e.padlen; ++k) {
-      const TR v = q.step(static_cast<TR>(extension<TIO, true>(xr, e, k)));
-      yb[k] = IOTraits<TIO>::from_f(static_cast<io_float_t<TIO>>(v));
+    TR last = e.padlen > 0 ? TR(0)
+                           : static_cast<TR>(rd(static_cast<const TIO*>(e.y1) + c * e.y1_stride,
+                                                e.T - 1));
+    for (int64_t k0 = 0; k0 < e.padlen; k0 += kEdgeBlk) {
+      TR u[kEdgeBlk];
+#pragma unroll
+      for (int j = 0; j < kEdgeBlk; ++j)
+        u[j] = k0 + j < e.padlen ? static_cast<TR>(extension<TIO, true>(xr, e, k0 + j)) : TR(0);
+#pragma unroll
+      for (int j = 0; j < kEdgeBlk; ++j) {
+        if (k0 + j < e.padlen) {
+          // the output as the reference's tensor holds it (rounded to TIO)
+          const TIO o = IOTraits<TIO>::from_f(static_cast<io_float_t<TIO>>(q.step(u[j])));
+          yb[k0 + j] = o;
+          last = static_cast<TR>(IOTraits<TIO>::to_f(o));
+        }
+      }
     }
-    const TR last =
-        e.padlen > 0
-            ? static_cast<TR>(rd(yb, e.padlen - 1))
-            : static_cast<TR>(rd(static_cast<const TIO*>(e.y1) + c * e.y1_stride, e.T - 1));
     q.start(zu, last);
-    for (int64_t k = e.padlen - 1; k >= 0; --k) q.step(static_cast<TR>(rd(yb, k)));
+    // backward over the stored outputs, a block of them read ahead
+    for (int64_t k1 = e.padlen; k1 > 0; k1 -= kEdgeBlk) {
+      TR u[kEdgeBlk];
+#pragma unroll
+      for (int j = 0; j < kEdgeBlk; ++j)
+        u[j] = k1 - 1 - j >= 0 ? static_cast<TR>(rd(yb, k1 - 1 - j)) : TR(0);
+#pragma unroll
+      for (int j = 0; j < kEdgeBlk; ++j)
+        if (k1 - 1 - j >= 0) q.step(u[j]);
+    }
   }
   q.store(zo);
 }
