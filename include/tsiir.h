@@ -149,6 +149,8 @@ tsiir_status tsiir_workspace_init(void* workspace, size_t workspace_bytes,
  * (hipHostMalloc; pass its host address) or device memory; NULL unregisters
  * it.  A caller polls the mailbox without synchronising.  The header is
  * zeroed by tsiir_workspace_init, so register after it (stream-ordered).
+ * Bits 1 and 2 are look-back waits; tsiir_sosfiltfilt reports a filter
+ * without a steady state the same way (TSIIR_ERR_STEADY_STATE).
  * No counterpart in the reference (its Python loop cannot time out). */
 tsiir_status tsiir_workspace_set_error_mailbox(void* workspace, size_t workspace_bytes,
                                                uint32_t* mailbox, void* stream);
@@ -179,11 +181,25 @@ tsiir_status tsiir_sosfilt_forward(const tsiir_sosfilt_desc* desc,
  * workspace of desc (tsiir_sosfilt_workspace_bytes), used by both passes.
  * At most 8 sections and padlen * n_sections <= 16384, else
  * TSIIR_EUNSUPPORTED.  Replaces _sosfiltfilt.py:96-173 (the padded
- * concatenation, the two sosfilt calls and the flips). */
+ * concatenation, the two sosfilt calls and the flips).
+ *
+ * Without a copy of the coefficients on the host: padlen =
+ * TSIIR_PADLEN_DEFAULT takes each filter's default of _sosfiltfilt.py:100-110
+ * (3 * (2 n_sections + 1 - min(#b2 == 0, #a2 == 0))), which has to be below
+ * n_samples whatever the coefficients (3 * (2 n_sections + 1) < n_samples,
+ * else TSIIR_EINVAL); zi_unit = NULL forms sosfilt_zi on the device (equal
+ * bit for bit to the FP64 closed form of _sosfilt_zi.py:55-86 with
+ * _lfilter_zi.py:100-128).  A filter with no steady state there (a section
+ * whose system is singular: 1 + a1 / a0 + a2 / a0 == 0, or sum(a) == 0 when
+ * a0 == 0; the reference's solve raises) gives NaN outputs and sets
+ * TSIIR_ERR_STEADY_STATE in the workspace's error word and mailbox
+ * (tsiir_workspace_set_error_mailbox). */
 #define TSIIR_PAD_NONE 0
 #define TSIIR_PAD_ODD 1
 #define TSIIR_PAD_EVEN 2
 #define TSIIR_PAD_CONSTANT 3
+#define TSIIR_PADLEN_DEFAULT (-1)
+#define TSIIR_ERR_STEADY_STATE 4u
 size_t tsiir_sosfiltfilt_scratch_bytes(const tsiir_sosfilt_desc* desc, int64_t padlen);
 tsiir_status tsiir_sosfiltfilt(const tsiir_sosfilt_desc* desc, int padtype, int64_t padlen,
                                const void* x, void* y, const void* sos, const double* zi_unit,

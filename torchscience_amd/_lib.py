@@ -32,6 +32,8 @@ PAD_NONE, PAD_ODD, PAD_EVEN, PAD_CONSTANT = 0, 1, 2, 3
 # tsiir_sosfiltfilt's bounds (beyond them it returns EUNSUPPORTED)
 SOSFILTFILT_MAX_SECTIONS = 8
 SOSFILTFILT_MAX_WORK = 16384  # padlen x sections
+PADLEN_DEFAULT = -1  # the default padlen, formed on the device
+ERR_STEADY_STATE = 4  # workspace error bit: a filter with no steady state
 
 # Every symbol include/tsiir.h declares.
 EXPORTED = (
@@ -270,11 +272,21 @@ def io_code(dtype: torch.dtype) -> int:
     raise TypeError(f"unsupported I/O dtype {dtype}")
 
 
+def error_text(code: int) -> str:
+    """What a workspace error code means (the bits of its error word)."""
+    if code == ERR_STEADY_STATE:
+        return ("sosfiltfilt: a filter has no steady state (a section's system for "
+                "sosfilt_zi is singular; error 4): its outputs are NaN")
+    return (f"a look-back wait gave up (error {code}): the outputs of the affected "
+            "windows are NaN")
+
+
 def check_errors(device: torch.device | None = None) -> None:
-    """Synchronise `device` (or all devices) and raise TsiirError if a
-    look-back wait of any earlier call gave up (its outputs are NaN).  Calls
-    also report it themselves, without synchronising, on the next call on
-    the same stream."""
+    """Synchronise `device` (or all devices) and raise TsiirError if an
+    earlier call failed on the device: a look-back wait gave up, or
+    sosfiltfilt met a filter with no steady state (their outputs are NaN).
+    Calls also report it themselves, without synchronising, on the next call
+    on the same stream."""
     if device is not None:
         torch.cuda.synchronize(device)
     else:
@@ -282,5 +294,4 @@ def check_errors(device: torch.device | None = None) -> None:
             torch.cuda.synchronize(i)
     code = int(ops().take_error())
     if code != 0:
-        raise TsiirError(f"a look-back wait gave up (error {code}): the outputs of the "
-                         "affected windows are NaN")
+        raise TsiirError(error_text(code))

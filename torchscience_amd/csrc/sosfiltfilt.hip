@@ -14,6 +14,12 @@
 //      backward from zi_unit * y1r[-1]; keep only the final state zc;
 //   4. sosfilt over y1 in reverse time from zc -> y.
 //
+// With zi_unit null and padlen < 0 the edge kernels form the steady state
+// and the default padlen from the coefficients on the device, so a call
+// needs no copy of them to the host; a filter without a steady state gives
+// NaN outputs and sets kErrSteadyState in the workspace's error word and
+// mailbox instead of failing the call.
+//
 // The edge kernels run one thread per channel (the extensions are a few
 // dozen samples), every section of a sample before the next, in the DF2T
 // order of the window kernels (_lfilter.py:158-166 with fused multiply-adds)
@@ -33,15 +39,18 @@ namespace {
 constexpr int kEdgeMaxS = 8;
 constexpr int64_t kEdgeMaxWork = 16384;  // padlen x sections per thread
 constexpr int64_t kEdgeLine = 128;       // bytes: y1 rows start on L2 lines
+constexpr uint32_t kErrSteadyState = TSIIR_ERR_STEADY_STATE;
 
 struct EdgeArgs {
   const void* x;            // rows, time-ordered
   int64_t x_rows, x_stride;
   int64_t B, T, cpf;
   int S, padtype;
-  int64_t padlen;
+  int64_t padlen;           // < 0: each filter's default (default_padlen)
+  int64_t ybuf_stride;      // elements per channel in ybuf
   const void* sos;          // (n_filters, S, 6), coefficient type
-  const double* zi_unit;    // (n_filters, S, 2)
+  const double* zi_unit;    // (n_filters, S, 2), or null: steady_state() here
+  WsHeader* hdr;            // error word and mailbox of the call's workspace
   const double* z_in;       // right edge: the state after the pass over the rows
   double* z_out;            // (B, S, 2)
   void* ybuf;               // right edge: its forward outputs, (B, max(padlen, 1))
@@ -67,9 +76,9 @@ __device__ __forceinline__ io_float_t<TIO> to_io(io_float_t<TIO> v) {
 // x[T-2-k], even x[padlen - k] / x[T-2-k], constant x[0] / x[T-1].
 template <typename TIO, bool RIGHT>
 __device__ __forceinline__ io_float_t<TIO> extension(const TIO* xr, const EdgeArgs& e,
-                                                     int64_t k) {
+                                                     int64_t padlen, int64_t k) {
   const int64_t edge = RIGHT ? e.T - 1 : 0;
-  const int64_t mirror = RIGHT ? e.T - 2 - k : e.padlen - k;
+  const int64_t mirror = RIGHT ? e.T - 2 - k : padlen - k;
   const io_float_t<TIO> xe = rd(xr, edge);
   if (e.padtype == TSIIR_PAD_ODD) return to_io<TIO>(io_float_t<TIO>(2) * xe - rd(xr, mirror));
   if (e.padtype == TSIIR_PAD_EVEN) return rd(xr, mirror);
@@ -141,6 +150,51 @@ __device__ __forceinline__ void load_coefficients(Cascade<TR>& q, const TC* f) {
   }
 }
 
+// The reference's default padlen of one filter (_sosfiltfilt.py:100-110):
+// 3 * (2 S + 1 - min(sections with b2 == 0, sections with a2 == 0)).
+template <typename TC>
+__device__ __forceinline__ int64_t default_padlen(const TC* f, int S) {
+  int nb = 0, na = 0;
+  for (int s = 0; s < S; ++s) {
+    nb += f[6 * s + 2] == TC(0) ? 1 : 0;
+    na += f[6 * s + 5] == TC(0) ? 1 : 0;
+  }
+  return 3 * (2 * S + 1 - (nb < na ? nb : na));
+}
+
+// sosfilt_zi of one filter in FP64 (_sosfilt_zi.py:55-86 over
+// _lfilter_zi.py:100-128): each section's system [[1 + a1, -1], [a2, 1]] zi
+// = [b1 - a1 b0, b2 - a2 b0] (normalised by a0) in closed form, scaled by
+// the DC gains sum(b) / sum(a) of the sections before it.  No contraction,
+// so the states equal sosfilt_zi_host's bit for bit.  False where the
+// reference's solve raises: a singular system (det == 0), or, for a0 == 0
+// (whose leading zero it strips, leaving [[1 + a2 / a1, -1], [0, 1]] or the
+// identity), sum(a) == 0.  An a0 == 0 section otherwise passes: its states
+// are not finite here, but neither is any output of a pass that divides by
+// a0, the reference's included.
+template <typename TC>
+__device__ __forceinline__ bool steady_state(const TC* f, int S, double (&zu)[2 * kEdgeMaxS]) {
+#pragma clang fp contract(off)
+  double scale = 1.0;
+  bool ok = true;
+#pragma unroll
+  for (int s = 0; s < kEdgeMaxS; ++s) {
+    if (s < S) {
+      const double b0 = f[6 * s], b1 = f[6 * s + 1], b2 = f[6 * s + 2];
+      const double a0 = f[6 * s + 3], a1 = f[6 * s + 4], a2 = f[6 * s + 5];
+      const double sa = a0 + a1 + a2;
+      const double n0 = b0 / a0, n1 = b1 / a0, n2 = b2 / a0, m1 = a1 / a0, m2 = a2 / a0;
+      const double det = (1.0 + m1) + m2;
+      ok = ok && (a0 != 0.0 ? det != 0.0 : sa != 0.0);
+      const double r0 = n1 - m1 * n0, r1 = n2 - m2 * n0;
+      zu[2 * s] = scale * ((r0 + r1) / det);
+      zu[2 * s + 1] = scale * (((1.0 + m1) * r1 - m2 * r0) / det);
+      scale = scale * ((b0 + b1 + b2) / sa);
+    }
+  }
+  return ok;
+}
+
 // Samples are read kEdgeBlk at a time before they are filtered, so the
 // loads of a block are in flight together rather than one per step.
 constexpr int kEdgeBlk = 8;
@@ -153,20 +207,37 @@ __global__ __launch_bounds__(kWave) void sos_edge_kernel(EdgeArgs e) {
   const int64_t f = c / e.cpf;
   Cascade<TR> q;
   q.S = e.S;
-  load_coefficients<TR, TC>(q, static_cast<const TC*>(e.sos) + f * e.S * 6);
-  const double* zu = e.zi_unit + f * e.S * 2;
+  const TC* fc = static_cast<const TC*>(e.sos) + f * e.S * 6;
+  load_coefficients<TR, TC>(q, fc);
+  const int64_t padlen = e.padlen >= 0 ? e.padlen : default_padlen(fc, e.S);
+  double zu[2 * kEdgeMaxS];
+  if (e.zi_unit != nullptr) {
+#pragma unroll
+    for (int k = 0; k < 2 * kEdgeMaxS; ++k) zu[k] = k < 2 * e.S ? e.zi_unit[f * e.S * 2 + k] : 0.0;
+  } else if (!steady_state(fc, e.S, zu)) {
+    // reported through the workspace's error word and mailbox; the states
+    // (and so every output of the channel) are NaN
+    atomicOr(&e.hdr->error, kErrSteadyState);
+    if (e.hdr->mailbox != nullptr)
+      __hip_atomic_store(e.hdr->mailbox, kErrSteadyState, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_SYSTEM);
+#pragma unroll
+    for (int k = 0; k < 2 * kEdgeMaxS; ++k) zu[k] = __builtin_nan("");
+  }
   const TIO* xr = static_cast<const TIO*>(e.x) + (c % e.x_rows) * e.x_stride;
   double* zo = e.z_out + c * e.S * 2;
   if constexpr (!RIGHT) {
-    q.start(zu, static_cast<TR>(e.padlen > 0 ? extension<TIO, false>(xr, e, 0) : rd(xr, 0)));
-    for (int64_t k0 = 0; k0 < e.padlen; k0 += kEdgeBlk) {
+    q.start(zu, static_cast<TR>(padlen > 0 ? extension<TIO, false>(xr, e, padlen, 0)
+                                           : rd(xr, 0)));
+    for (int64_t k0 = 0; k0 < padlen; k0 += kEdgeBlk) {
       TR u[kEdgeBlk];
 #pragma unroll
       for (int j = 0; j < kEdgeBlk; ++j)
-        u[j] = k0 + j < e.padlen ? static_cast<TR>(extension<TIO, false>(xr, e, k0 + j)) : TR(0);
+        u[j] = k0 + j < padlen ? static_cast<TR>(extension<TIO, false>(xr, e, padlen, k0 + j))
+                               : TR(0);
 #pragma unroll
       for (int j = 0; j < kEdgeBlk; ++j)
-        if (k0 + j < e.padlen) q.step(u[j]);
+        if (k0 + j < padlen) q.step(u[j]);
     }
   } else {
     const double* zi = e.z_in + c * e.S * 2;
@@ -177,18 +248,19 @@ __global__ __launch_bounds__(kWave) void sos_edge_kernel(EdgeArgs e) {
         q.z1[s] = static_cast<TR>(zi[2 * s + 1]);
       }
     }
-    TIO* yb = static_cast<TIO*>(e.ybuf) + c * (e.padlen > 0 ? e.padlen : 1);
-    TR last = e.padlen > 0 ? TR(0)
+    TIO* yb = static_cast<TIO*>(e.ybuf) + c * e.ybuf_stride;
+    TR last = padlen > 0 ? TR(0)
                            : static_cast<TR>(rd(static_cast<const TIO*>(e.y1) + c * e.y1_stride,
                                                 e.T - 1));
-    for (int64_t k0 = 0; k0 < e.padlen; k0 += kEdgeBlk) {
+    for (int64_t k0 = 0; k0 < padlen; k0 += kEdgeBlk) {
       TR u[kEdgeBlk];
 #pragma unroll
       for (int j = 0; j < kEdgeBlk; ++j)
-        u[j] = k0 + j < e.padlen ? static_cast<TR>(extension<TIO, true>(xr, e, k0 + j)) : TR(0);
+        u[j] = k0 + j < padlen ? static_cast<TR>(extension<TIO, true>(xr, e, padlen, k0 + j))
+                               : TR(0);
 #pragma unroll
       for (int j = 0; j < kEdgeBlk; ++j) {
-        if (k0 + j < e.padlen) {
+        if (k0 + j < padlen) {
           // the output as the reference's tensor holds it (rounded to TIO)
           const TIO o = IOTraits<TIO>::from_f(static_cast<io_float_t<TIO>>(q.step(u[j])));
           yb[k0 + j] = o;
@@ -198,7 +270,7 @@ __global__ __launch_bounds__(kWave) void sos_edge_kernel(EdgeArgs e) {
     }
     q.start(zu, last);
     // backward over the stored outputs, a block of them read ahead
-    for (int64_t k1 = e.padlen; k1 > 0; k1 -= kEdgeBlk) {
+    for (int64_t k1 = padlen; k1 > 0; k1 -= kEdgeBlk) {
       TR u[kEdgeBlk];
 #pragma unroll
       for (int j = 0; j < kEdgeBlk; ++j)
@@ -245,14 +317,19 @@ size_t align256(size_t n) { return (n + 255) / 256 * 256; }
 // the line-aligned kernels), the right extension's outputs, and the FP64
 // states za, zb, zc.
 struct ScratchLayout {
-  int64_t y1_stride = 0;
+  int64_t y1_stride = 0, ybuf_stride = 1;
   size_t y1 = 0, ybuf = 0, za = 0, zb = 0, zc = 0, total = 0;
 };
 
+// The largest default padlen of an S-section filter.
+int64_t max_default_padlen(int64_t S) { return 3 * (2 * S + 1); }
+
+// padlen < 0: room for any filter's default.
 bool scratch_layout(const tsiir_sosfilt_desc* d, int64_t padlen, ScratchLayout* L) {
   const size_t esz = io_bytes(d->io_dtype);
-  if (esz == 0 || d->channels < 0 || d->n_samples < 1 || d->n_sections < 1 || padlen < 0)
-    return false;
+  if (esz == 0 || d->channels < 0 || d->n_samples < 1 || d->n_sections < 1) return false;
+  if (padlen < 0) padlen = max_default_padlen(d->n_sections);
+  L->ybuf_stride = padlen > 0 ? padlen : 1;
   const int64_t per_line = kEdgeLine / static_cast<int64_t>(esz);
   const size_t B = static_cast<size_t>(d->channels);
   const size_t zs = B * static_cast<size_t>(d->n_sections) * 2 * sizeof(double);
@@ -261,7 +338,7 @@ bool scratch_layout(const tsiir_sosfilt_desc* d, int64_t padlen, ScratchLayout* 
   L->y1 = off;
   off = align256(off + B * static_cast<size_t>(L->y1_stride) * esz);
   L->ybuf = off;
-  off = align256(off + B * static_cast<size_t>(padlen > 0 ? padlen : 1) * esz);
+  off = align256(off + B * static_cast<size_t>(L->ybuf_stride) * esz);
   L->za = off;
   off = align256(off + zs);
   L->zb = off;
@@ -292,13 +369,18 @@ tsiir_status tsiir_sosfiltfilt(const tsiir_sosfilt_desc* desc, int padtype, int6
   if (!desc) return TSIIR_EINVAL;
   const tsiir_sosfilt_desc d = *desc;
   if ((d.flags & ~TSIIR_FP32_DF2T) != 0 || d.mode != TSIIR_MODE_PLAIN) return TSIIR_EINVAL;
-  if (padtype < TSIIR_PAD_NONE || padtype > TSIIR_PAD_CONSTANT || padlen < 0) return TSIIR_EINVAL;
+  if (padtype < TSIIR_PAD_NONE || padtype > TSIIR_PAD_CONSTANT) return TSIIR_EINVAL;
+  if (padlen < 0 && padlen != TSIIR_PADLEN_DEFAULT) return TSIIR_EINVAL;
   if ((padtype == TSIIR_PAD_NONE) != (padlen == 0)) return TSIIR_EINVAL;
-  if (d.n_samples < 1 || padlen >= d.n_samples || d.n_sections < 1) return TSIIR_EINVAL;
+  if (d.n_samples < 1 || d.n_sections < 1) return TSIIR_EINVAL;
+  // a default padlen has to fit whatever the coefficients are
+  const int64_t pad_max = padlen >= 0 ? padlen : max_default_padlen(d.n_sections);
+  if (pad_max >= d.n_samples) return TSIIR_EINVAL;
   if (d.chans_per_filter < 1 || d.x_rows < 1) return TSIIR_EINVAL;
   if (d.channels == 0) return TSIIR_OK;
-  if (!x || !y || !sos || !zi_unit) return TSIIR_EINVAL;
-  if (d.n_sections > kEdgeMaxS || padlen * d.n_sections > kEdgeMaxWork) return TSIIR_EUNSUPPORTED;
+  if (!x || !y || !sos) return TSIIR_EINVAL;
+  if (!workspace || workspace_bytes < sizeof(WsHeader)) return TSIIR_EINVAL;
+  if (d.n_sections > kEdgeMaxS || pad_max * d.n_sections > kEdgeMaxWork) return TSIIR_EUNSUPPORTED;
   ScratchLayout L;
   if (!scratch_layout(&d, padlen, &L)) return TSIIR_EINVAL;
   if (!scratch || scratch_bytes < L.total) return TSIIR_EWORKSPACE;
@@ -319,8 +401,10 @@ tsiir_status tsiir_sosfiltfilt(const tsiir_sosfilt_desc* desc, int padtype, int6
   e.S = d.n_sections;
   e.padtype = padtype;
   e.padlen = padlen;
+  e.ybuf_stride = L.ybuf_stride;
   e.sos = sos;
   e.zi_unit = zi_unit;
+  e.hdr = static_cast<WsHeader*>(workspace);
   e.z_out = za;
   e.y1 = y1;
   e.y1_stride = L.y1_stride;

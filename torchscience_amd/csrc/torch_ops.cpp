@@ -30,6 +30,7 @@
 #include <cmath>
 #include <map>
 #include <mutex>
+#include <string>
 #include <tuple>
 
 #include "tsiir.h"
@@ -62,6 +63,15 @@ struct WsEntry {
 std::mutex g_ws_mu;
 std::map<std::tuple<int, hipStream_t, int>, WsEntry> g_ws;
 
+// What a workspace error code means (the bits of the header's error word).
+std::string error_text(int code) {
+  if (code == static_cast<int>(TSIIR_ERR_STEADY_STATE))
+    return "sosfiltfilt: a filter has no steady state (a section's system for sosfilt_zi is "
+           "singular; error 4): its outputs are NaN";
+  return "a look-back wait gave up (error " + std::to_string(code) +
+         "): the outputs of the affected windows are NaN";
+}
+
 int mailbox_value(const WsEntry& e) {
   return *static_cast<volatile const int32_t*>(e.mailbox.data_ptr());
 }
@@ -74,8 +84,7 @@ at::Tensor workspace(const at::Device& dev, hipStream_t stream, int kind, size_t
     const int code = mailbox_value(it->second);
     if (code != 0) {
       g_ws.erase(it);
-      TORCH_CHECK(false, "tsiir: a look-back wait of an earlier call on this stream gave up "
-                  "(error ", code, "): the outputs of the affected windows are NaN");
+      TORCH_CHECK(false, "tsiir: an earlier call on this stream failed: ", error_text(code));
     }
     if (static_cast<size_t>(it->second.buf.numel()) >= nbytes) return it->second.buf;
   }
@@ -350,26 +359,35 @@ Bwd sosfilt_backward_hip(const at::Tensor& x, const at::Tensor& y, const at::Ten
 
 // Zero-phase filtering of rows x (channels, samples) with one filter sos
 // (1, S, 6) and its steady state zi_unit (1, S, 2) in double: the C ABI's
-// tsiir_sosfiltfilt, the extensions never stored.  The sosfilt workspace of
-// the stream is shared with sosfilt_forward; the first pass's rows go to a
-// scratch tensor of the caching allocator.
-at::Tensor sosfiltfilt_meta(const at::Tensor& x, const at::Tensor& sos, const at::Tensor& zi_unit,
-                            int64_t padtype, int64_t padlen) {
+// tsiir_sosfiltfilt, the extensions never stored.  zi_unit None and padlen
+// -1 form the steady state and the default padlen on the device (no copy of
+// sos to the host; a filter without a steady state is reported through the
+// stream's error mailbox).  The sosfilt workspace of the stream is shared
+// with sosfilt_forward; the first pass's rows go to a scratch tensor of the
+// caching allocator.
+at::Tensor sosfiltfilt_meta(const at::Tensor& x, const at::Tensor& sos,
+                            const c10::optional<at::Tensor>& zi_unit, int64_t padtype,
+                            int64_t padlen) {
   check_rows(x, sos, x.size(0), x.size(0));
   return at::empty({x.size(0), x.size(1)}, x.options());
 }
 
-at::Tensor sosfiltfilt_hip(const at::Tensor& x, const at::Tensor& sos, const at::Tensor& zi_unit,
-                           int64_t padtype, int64_t padlen) {
+at::Tensor sosfiltfilt_hip(const at::Tensor& x, const at::Tensor& sos,
+                           const c10::optional<at::Tensor>& zi_unit, int64_t padtype,
+                           int64_t padlen) {
   check_rows(x, sos, x.size(0), x.size(0));
   c10::DeviceGuard guard(x.device());
-  TORCH_CHECK(sos.device() == x.device() && zi_unit.device() == x.device(),
-              "tsiir: sos and zi_unit must be on x's device");
+  TORCH_CHECK(sos.device() == x.device(), "tsiir: sos must be on x's device");
   TORCH_CHECK(sos.size(0) == 1, "tsiir: sosfiltfilt takes one filter");
   const int64_t S = sos.size(1), B = x.size(0), T = x.size(1);
-  TORCH_CHECK(zi_unit.scalar_type() == at::kDouble && zi_unit.is_contiguous() &&
-                  zi_unit.numel() == S * 2,
-              "tsiir: zi_unit must be contiguous double (1, n_sections, 2)");
+  const double* zu = nullptr;
+  if (zi_unit.has_value()) {
+    TORCH_CHECK(zi_unit->device() == x.device(), "tsiir: zi_unit must be on x's device");
+    TORCH_CHECK(zi_unit->scalar_type() == at::kDouble && zi_unit->is_contiguous() &&
+                    zi_unit->numel() == S * 2,
+                "tsiir: zi_unit must be contiguous double (1, n_sections, 2)");
+    zu = zi_unit->data_ptr<double>();
+  }
   const at::ScalarType io = x.scalar_type();
   at::Tensor y = at::empty({B, T}, x.options());
   if (B == 0) return y;
@@ -384,7 +402,7 @@ at::Tensor sosfiltfilt_hip(const at::Tensor& x, const at::Tensor& sos, const at:
   at::Tensor ws = workspace(x.device(), stream, 0, nws);
   at::Tensor sc = at::empty({static_cast<int64_t>(nsc)}, x.options().dtype(at::kByte));
   check_status(tsiir_sosfiltfilt(&d, static_cast<int>(padtype), padlen, x.data_ptr(),
-                                 y.data_ptr(), sos.data_ptr(), zi_unit.data_ptr<double>(),
+                                 y.data_ptr(), sos.data_ptr(), zu,
                                  sc.data_ptr(), static_cast<size_t>(sc.numel()), ws.data_ptr(),
                                  static_cast<size_t>(ws.numel()), stream),
                "tsiir_sosfiltfilt");
@@ -402,7 +420,7 @@ TORCH_LIBRARY(tsiir, m) {
       "sosfilt_backward(Tensor x, Tensor y, Tensor saved, Tensor sos, Tensor? zi, Tensor dy, "
       "Tensor? dzf, int chans_per_filter, bool zi_broadcast) -> (Tensor dx, Tensor dsos, "
       "Tensor dzi)");
-  m.def("sosfiltfilt(Tensor x, Tensor sos, Tensor zi_unit, int padtype, int padlen) -> Tensor");
+  m.def("sosfiltfilt(Tensor x, Tensor sos, Tensor? zi_unit, int padtype, int padlen) -> Tensor");
   m.def("take_error() -> int", &take_error);
   m.def("max_pole_radius(Tensor sos) -> float", &max_pole_radius);
 }

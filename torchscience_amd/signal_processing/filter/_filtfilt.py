@@ -72,7 +72,12 @@ def sosfiltfilt(sos: Tensor, x: Tensor, axis: int = -1, padtype: PadType = "odd"
         raise ValueError("sos must be shape (n_sections, 6)")
     S = sos.shape[0]
     fast = S > 0 and not (_needs_grad(sos, x) or x.is_complex() or sos.is_complex())
-    # The kernel form takes the coefficients to the host once: the default
+    xm = torch.moveaxis(x, axis, -1)
+    T = xm.shape[-1]
+    dev_pad = _device_padlen(sos, xm, padtype, padlen) if fast else None
+    if dev_pad is not None:
+        return torch.moveaxis(_sosfiltfilt_rows(sos, None, xm, padtype, dev_pad), -1, axis)
+    # Otherwise the kernel form takes the coefficients to the host once: the default
     # padlen and sosfilt_zi are small host computations there, where on the
     # device each of their scalar tests and solves would wait for the device.
     sos_h = sos.detach().cpu() if fast else sos
@@ -84,11 +89,10 @@ def sosfiltfilt(sos: Tensor, x: Tensor, axis: int = -1, padtype: PadType = "odd"
         else:
             ntaps -= min(int((sos[:, 2] == 0).sum().item()), int((sos[:, 5] == 0).sum().item()))
         padlen = 3 * ntaps
-    xm = torch.moveaxis(x, axis, -1)
-    T = xm.shape[-1]
     if padtype is not None and padlen >= T:
         raise ValueError(f"padlen ({padlen}) must be less than x.shape[axis] ({T})")
-    pad = padlen if padtype is not None else 0
+    # the reference pads (and trims) only for padlen > 0
+    pad = max(padlen, 0) if padtype is not None else 0
     if pad > 0 and padtype not in ("odd", "even", "constant"):
         raise ValueError(f"Unknown padtype: {padtype}")
     if fast:
@@ -113,12 +117,39 @@ def sosfiltfilt(sos: Tensor, x: Tensor, axis: int = -1, padtype: PadType = "odd"
     return torch.moveaxis(y, -1, axis)
 
 
-def _sosfiltfilt_rows(sos: Tensor, sos_h: Tensor, x: Tensor, padtype: PadType,
+def _device_padlen(sos: Tensor, x: Tensor, padtype: PadType,
+                   padlen: Optional[int]) -> Optional[int]:
+    """The padlen argument of the operator when it can form the default
+    padlen and sosfilt_zi from coefficients on the device (-1: the default),
+    so that the call never waits for the device; None where a host copy of
+    the coefficients is needed: sos or x not on the device, beyond the
+    operator's bounds, or a reference error that depends on the values
+    (padlen >= T for some default padlen, 3 (2 S + 1) >= T).  A filter with
+    no steady state then gives NaN rows and a TsiirError from the stream's
+    next call (or check_errors) where the reference raises at once."""
+    S, T = sos.shape[0], x.shape[-1]
+    if not (sos.is_cuda and x.is_cuda) or S > _lib.SOSFILTFILT_MAX_SECTIONS:
+        return None
+    if padtype is None:
+        return 0
+    if padlen is None:
+        pad, need = -1, 3 * (2 * S + 1)
+    else:
+        pad = need = max(padlen, 0)
+    if need >= T or need * S > _lib.SOSFILTFILT_MAX_WORK:
+        return None
+    if pad != 0 and padtype not in ("odd", "even", "constant"):
+        raise ValueError(f"Unknown padtype: {padtype}")
+    return pad
+
+
+def _sosfiltfilt_rows(sos: Tensor, sos_h: Optional[Tensor], x: Tensor, padtype: PadType,
                       padlen: int) -> Tensor:
     """Both passes on the HIP kernel over [left, x, right] (padlen samples
     of each extension, none when padlen is 0), time last; returns the
     central T samples of the second pass.  sos_h: the coefficients on the
-    host."""
+    host, or None for the operator's device-side sosfilt_zi (and padlen -1
+    for its default padlen, within the operator's bounds)."""
     plan = plan_sosfilt(sos.shape, sos.dtype, x.shape, x.dtype, -1, None, None, None)
     S, T, B = plan.n_sections, plan.n_samples, plan.channels
     comp = plan.compute_dtype
@@ -126,12 +157,15 @@ def _sosfiltfilt_rows(sos: Tensor, sos_h: Tensor, x: Tensor, padtype: PadType,
     rows = x.reshape(B, T).to(comp)
     if rows.stride(-1) != 1:
         rows = rows.contiguous()
-    if S <= _lib.SOSFILTFILT_MAX_SECTIONS and padlen * S <= _lib.SOSFILTFILT_MAX_WORK:
+    if sos_h is None or (S <= _lib.SOSFILTFILT_MAX_SECTIONS
+                         and padlen * S <= _lib.SOSFILTFILT_MAX_WORK):
         # one operator: edge kernels plus the two passes over the rows
-        zu = sosfilt_zi_host(sos_h).reshape(1, S, 2)
-        zu = zu.pin_memory().to(x.device, non_blocking=True) if x.is_cuda else zu.to(x.device)
+        zu = None
+        if sos_h is not None:
+            zu = sosfilt_zi_host(sos_h).reshape(1, S, 2)
+            zu = zu.pin_memory().to(x.device, non_blocking=True) if x.is_cuda else zu.to(x.device)
         pad = {None: _lib.PAD_NONE, "odd": _lib.PAD_ODD, "even": _lib.PAD_EVEN,
-               "constant": _lib.PAD_CONSTANT}[padtype if padlen > 0 else None]
+               "constant": _lib.PAD_CONSTANT}[padtype if padlen != 0 else None]
         _require_device(sos_c, rows)
         _lib.ops()
         y = torch.ops.tsiir.sosfiltfilt(rows, sos_c, zu, pad, padlen)
