@@ -50,6 +50,8 @@ struct EdgeArgs {
   int64_t ybuf_stride;      // elements per channel in ybuf
   const void* sos;          // (n_filters, S, 6), coefficient type
   const double* zi_unit;    // (n_filters, S, 2), or null: steady_state() here
+  double* zs;               // zi_unit null: the left kernel's steady states for
+                            // the right kernel, (n_filters, S, 2)
   WsHeader* hdr;            // error word and mailbox of the call's workspace
   const double* z_in;       // right edge: the state after the pass over the rows
   double* z_out;            // (B, S, 2)
@@ -183,7 +185,14 @@ __device__ __forceinline__ bool steady_state(const TC* f, int S, double (&zu)[2 
       const double b0 = f[6 * s], b1 = f[6 * s + 1], b2 = f[6 * s + 2];
       const double a0 = f[6 * s + 3], a1 = f[6 * s + 4], a2 = f[6 * s + 5];
       const double sa = a0 + a1 + a2;
-      const double n0 = b0 / a0, n1 = b1 / a0, n2 = b2 / a0, m1 = a1 / a0, m2 = a2 / a0;
+      double n0 = b0, n1 = b1, n2 = b2, m1 = a1, m2 = a2;  // x / 1 == x
+      if (a0 != 1.0) {
+        n0 = b0 / a0;
+        n1 = b1 / a0;
+        n2 = b2 / a0;
+        m1 = a1 / a0;
+        m2 = a2 / a0;
+      }
       const double det = (1.0 + m1) + m2;
       ok = ok && (a0 != 0.0 ? det != 0.0 : sa != 0.0);
       const double r0 = n1 - m1 * n0, r1 = n2 - m2 * n0;
@@ -200,20 +209,23 @@ __device__ __forceinline__ bool steady_state(const TC* f, int S, double (&zu)[2 
 constexpr int kEdgeBlk = 8;
 
 // One thread per channel.  RIGHT = false: step 1 above; RIGHT: step 3.
-template <typename TIO, typename TR, typename TC, bool RIGHT>
+// ONE: a single filter for every channel, so its coefficients, padlen and
+// steady state are uniform (scalar loads, scalar loop control).
+template <typename TIO, typename TR, typename TC, bool RIGHT, bool ONE>
 __global__ __launch_bounds__(kWave) void sos_edge_kernel(EdgeArgs e) {
   const int64_t c = static_cast<int64_t>(blockIdx.x) * kWave + threadIdx.x;
   if (c >= e.B) return;
-  const int64_t f = c / e.cpf;
+  const int64_t f = ONE ? 0 : c / e.cpf;
   Cascade<TR> q;
   q.S = e.S;
   const TC* fc = static_cast<const TC*>(e.sos) + f * e.S * 6;
   load_coefficients<TR, TC>(q, fc);
   const int64_t padlen = e.padlen >= 0 ? e.padlen : default_padlen(fc, e.S);
   double zu[2 * kEdgeMaxS];
-  if (e.zi_unit != nullptr) {
+  const double* zsrc = e.zi_unit != nullptr ? e.zi_unit : RIGHT ? e.zs : nullptr;
+  if (zsrc != nullptr) {
 #pragma unroll
-    for (int k = 0; k < 2 * kEdgeMaxS; ++k) zu[k] = k < 2 * e.S ? e.zi_unit[f * e.S * 2 + k] : 0.0;
+    for (int k = 0; k < 2 * kEdgeMaxS; ++k) zu[k] = k < 2 * e.S ? zsrc[f * e.S * 2 + k] : 0.0;
   } else if (!steady_state(fc, e.S, zu)) {
     // reported through the workspace's error word and mailbox; the states
     // (and so every output of the channel) are NaN
@@ -223,6 +235,11 @@ __global__ __launch_bounds__(kWave) void sos_edge_kernel(EdgeArgs e) {
                          __HIP_MEMORY_SCOPE_SYSTEM);
 #pragma unroll
     for (int k = 0; k < 2 * kEdgeMaxS; ++k) zu[k] = __builtin_nan("");
+  }
+  if (!RIGHT && zsrc == nullptr && c % e.cpf == 0) {
+#pragma unroll
+    for (int k = 0; k < 2 * kEdgeMaxS; ++k)
+      if (k < 2 * e.S) e.zs[f * e.S * 2 + k] = zu[k];
   }
   const TIO* xr = static_cast<const TIO*>(e.x) + (c % e.x_rows) * e.x_stride;
   double* zo = e.z_out + c * e.S * 2;
@@ -286,10 +303,18 @@ __global__ __launch_bounds__(kWave) void sos_edge_kernel(EdgeArgs e) {
 template <typename TIO, typename TR, bool RIGHT>
 void launch_edge(const EdgeArgs& e, int coef_dtype, hipStream_t stream) {
   const dim3 grid(static_cast<unsigned>((e.B + kWave - 1) / kWave)), block(kWave);
-  if (coef_dtype == TSIIR_F64)
-    hipLaunchKernelGGL((sos_edge_kernel<TIO, TR, double, RIGHT>), grid, block, 0, stream, e);
-  else
-    hipLaunchKernelGGL((sos_edge_kernel<TIO, TR, float, RIGHT>), grid, block, 0, stream, e);
+  const bool one = e.cpf >= e.B;
+  if (coef_dtype == TSIIR_F64) {
+    if (one)
+      hipLaunchKernelGGL((sos_edge_kernel<TIO, TR, double, RIGHT, true>), grid, block, 0, stream, e);
+    else
+      hipLaunchKernelGGL((sos_edge_kernel<TIO, TR, double, RIGHT, false>), grid, block, 0, stream, e);
+  } else {
+    if (one)
+      hipLaunchKernelGGL((sos_edge_kernel<TIO, TR, float, RIGHT, true>), grid, block, 0, stream, e);
+    else
+      hipLaunchKernelGGL((sos_edge_kernel<TIO, TR, float, RIGHT, false>), grid, block, 0, stream, e);
+  }
 }
 
 template <bool RIGHT>
@@ -314,11 +339,11 @@ size_t io_bytes(int io_dtype) {
 size_t align256(size_t n) { return (n + 255) / 256 * 256; }
 
 // Scratch: y1 (rows padded to whole L2 lines, so both passes over it take
-// the line-aligned kernels), the right extension's outputs, and the FP64
-// states za, zb, zc.
+// the line-aligned kernels), the right extension's outputs, the FP64
+// states za, zb, zc and the device-formed steady states zs.
 struct ScratchLayout {
   int64_t y1_stride = 0, ybuf_stride = 1;
-  size_t y1 = 0, ybuf = 0, za = 0, zb = 0, zc = 0, total = 0;
+  size_t y1 = 0, ybuf = 0, za = 0, zb = 0, zc = 0, zs = 0, total = 0;
 };
 
 // The largest default padlen of an S-section filter.
@@ -345,6 +370,12 @@ bool scratch_layout(const tsiir_sosfilt_desc* d, int64_t padlen, ScratchLayout* 
   off = align256(off + zs);
   L->zc = off;
   off = align256(off + zs);
+  L->zs = off;
+  const size_t n_filters = d->chans_per_filter > 0
+                               ? (B + static_cast<size_t>(d->chans_per_filter) - 1) /
+                                     static_cast<size_t>(d->chans_per_filter)
+                               : 0;
+  off = align256(off + n_filters * static_cast<size_t>(d->n_sections) * 2 * sizeof(double));
   L->total = off;
   return true;
 }
@@ -404,6 +435,7 @@ tsiir_status tsiir_sosfiltfilt(const tsiir_sosfilt_desc* desc, int padtype, int6
   e.ybuf_stride = L.ybuf_stride;
   e.sos = sos;
   e.zi_unit = zi_unit;
+  e.zs = reinterpret_cast<double*>(sc + L.zs);
   e.hdr = static_cast<WsHeader*>(workspace);
   e.z_out = za;
   e.y1 = y1;
