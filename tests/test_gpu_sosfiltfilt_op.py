@@ -112,12 +112,14 @@ def test_filtfilt_one_section(oracle_mod, io, padtype, ba):
 
 @pytest.mark.parametrize("io", [torch.float64, torch.float32])
 @pytest.mark.parametrize("order", [3, 8, 16, 20])
-@pytest.mark.parametrize("T", [300, 5000])
+@pytest.mark.parametrize("T", [300, 5000, 5001])
 def test_general_order_reverse_equals_flip(io, order, T):
     """The general-order kernel's reverse-time mode (serial form for short
     rows and order 20, chunked time-parallel form for 5000 samples up to
     order 16) equals the forward mode on flipped rows bit for bit, zi and zf
-    included."""
+    included.  5000 samples: whole 16-byte vectors both ways; 5001: the
+    reverse mode and the odd rows of the forward one take element
+    accesses."""
     import scipy.signal as ss
     from torchscience_amd.signal_processing.filter._lfilter import _general_rows
     dev = _dev()
@@ -251,3 +253,26 @@ def test_no_steady_state_next_call_raises():
         sosfiltfilt(torch.tensor(_lowpass5(), dtype=torch.float32, device=dev), x)
     y = sosfiltfilt(torch.tensor(_lowpass5(), dtype=torch.float32, device=dev), x)
     assert torch.isfinite(y).all()
+
+
+@pytest.mark.parametrize("io", [torch.float64, torch.float32])
+@pytest.mark.parametrize("order", [4, 8, 16])
+def test_general_order_unaligned_rows_equal_aligned(io, order):
+    """The chunked general-order form reads and writes rows in 16-byte
+    vectors through an LDS tile when they are aligned, element by element
+    otherwise: rows of 6001 samples, every other one starting off a 16-byte
+    boundary, give the same bits as each row alone in an aligned buffer, zf
+    included."""
+    import scipy.signal as ss
+    from torchscience_amd.signal_processing.filter._lfilter import _general_rows
+    dev = _dev()
+    b, a = ss.butter(order, 0.2)
+    coef = torch.tensor(np.concatenate([b, a]), dtype=io, device=dev)
+    rng = np.random.default_rng(order)
+    x = torch.tensor(rng.standard_normal((19, 6001)), dtype=io, device=dev)
+    zi = torch.tensor(rng.standard_normal((19, order)), dtype=io, device=dev)
+    y, zf = _general_rows(coef, x, zi, True, False)
+    for c in (0, 1, 2, 3, 18):
+        yc, zfc = _general_rows(coef, x[c:c + 1].contiguous(), zi[c:c + 1].contiguous(), True,
+                                False)
+        assert torch.equal(y[c], yc[0]) and torch.equal(zf[c], zfc[0]), c

@@ -46,6 +46,21 @@ __device__ inline void df2t_step(const TACC (&bn)[NMAX + 1], const TACC (&an)[NM
     if (k == N - 1) z[k] = bn[N] * xn - an[N] * yn;
 }
 
+// df2t_step at order NMAX over coefficients zero-padded past N, with no
+// test of N: the states past N - 1 stay zero, and the value of every other
+// state and output is the same (up to the sign of a zero: z[N - 1] gains a
+// "+ 0").  With N a run-time value the tests of df2t_step cost more than
+// the padded terms (branches and state moves around each update).
+template <typename TACC, int NMAX>
+__device__ __forceinline__ void df2t_step_padded(const TACC (&bn)[NMAX + 1],
+                                                 const TACC (&an)[NMAX + 1], TACC (&z)[NMAX],
+                                                 TACC xn, TACC& yn) {
+  yn = bn[0] * xn + z[0];
+#pragma unroll
+  for (int k = 0; k < NMAX - 1; ++k) z[k] = (bn[k + 1] * xn - an[k + 1] * yn) + z[k + 1];
+  z[NMAX - 1] = bn[NMAX] * xn - an[NMAX] * yn;
+}
+
 template <typename TACC, int NMAX>
 __device__ inline void load_coef(const TACC* __restrict__ coef, int N, TACC (&bn)[NMAX + 1],
                                  TACC (&an)[NMAX + 1]) {
@@ -144,10 +159,16 @@ __global__ __launch_bounds__(64) void lfilter_general_kernel(
 // and store the final state (FP64) in ws[(c M + j) N ..].  The scan kernel
 // replaces those with the chunk start states.  Second pass: filter the
 // chunk from its start state, writing y (and zf from the last chunk).
-// Full chunks advance together in blocks of BLK samples; in the second pass
-// each block's outputs go through an LDS tile and are stored as contiguous
-// BLK-sample segments (a store instruction covers 64 / BLK segments rather
-// than 64 scattered addresses).  A partial last chunk runs after that,
+// Full chunks advance together in blocks of BLK samples.  Each block's
+// inputs come through an LDS tile: the wave loads the 64 chunks' BLK-sample
+// segments as 16-byte vectors (an instruction covers 64 / (BLK / V)
+// segments rather than 64 scattered addresses) and each lane reads its own
+// row of the tile; the next block's vectors are in flight while the current
+// block runs through the recurrence.  In the second pass the outputs go out
+// the same way.  Along decreasing time (REV) a segment is a vector run of
+// memory read or written in reverse order.  Rows that are not 16-byte
+// aligned (or, with REV, whose length is not a multiple of V) take
+// per-lane element accesses instead.  A partial last chunk runs after that,
 // sample by sample.
 constexpr int kGenChunk = 256;
 
@@ -161,7 +182,10 @@ __global__ __launch_bounds__(64) void lfilter_chunk_kernel(
   // tile rows of BLK samples padded to a multiple of 4 (16-byte vectors)
   constexpr int TS = BLK + 4;
   constexpr int V = 16 / sizeof(TACC);            // samples per 16-byte vector
-  __shared__ __align__(16) TACC tile[SECOND ? 64 * TS : 1];
+  constexpr int VPR = BLK / V;                    // vectors per tile row
+  constexpr int NB = kGenChunk / BLK;             // blocks per chunk
+  static_assert(BLK % V == 0 && (TS * sizeof(TACC)) % 16 == 0, "tile rows of whole vectors");
+  __shared__ __align__(16) TACC tile[64 * TS];
   const int l = threadIdx.x;
   const int64_t c = blockIdx.x / G;
   const int64_t g = blockIdx.x - c * G;
@@ -176,60 +200,104 @@ __global__ __launch_bounds__(64) void lfilter_chunk_kernel(
 #pragma unroll
   for (int k = 0; k < NMAX; ++k)
     z[k] = (SECOND && valid && k < N) ? static_cast<TACC>(st[k]) : TACC(0);
-  const auto xr = row_ref<REV>(x + (c % x_rows) * x_stride, T);
-  const auto yr = row_ref<REV>(y + c * y_stride, T);
-  TACC cur[BLK], nxt[BLK];
-  if (full) {
+  const TACC* xrow = x + (c % x_rows) * x_stride;
+  TACC* yrow = y + c * y_stride;
+  const auto xr = row_ref<REV>(xrow, T);
+  const auto yr = row_ref<REV>(yrow, T);
+  // vector I/O: every segment of a full chunk starts on a 16-byte boundary
+  // (processing index n is memory n, or with REV memory T - 1 - n)
+  const bool t_ok = !REV || T % V == 0;
+  const bool xv_ok = t_ok && (reinterpret_cast<uintptr_t>(xrow) & 15) == 0;
+  const bool yv_ok =
+      t_ok && ((reinterpret_cast<uintptr_t>(yrow) | (y_stride * sizeof(TACC))) & 15) == 0;
+  // a tile row holds a block in memory order: the memory start of its
+  // vector `part` for chunk js's block sb (processing indices m .. m + BLK
+  // - 1, with REV the memory run T - m - BLK .. T - m - 1)
+  auto seg = [&](int64_t js, int sb, int part) __attribute__((always_inline)) -> int64_t {
+    const int64_t m = js * kGenChunk + sb * BLK;
+    return REV ? T - m - BLK + part * V : m + part * V;
+  };
+  u32x4 xin[VPR];
+  // the wave's loads of block sb: vector e = l + 64 q is vector e % VPR of
+  // chunk 64 g + e / VPR (only full chunks), one 16-byte load or, off a
+  // 16-byte boundary, V element loads
+  auto load_block = [&](int sb) __attribute__((always_inline)) {
 #pragma unroll
-    for (int i = 0; i < BLK; ++i) cur[i] = xr[n0 + i];
-  }
-  for (int sb = 0; sb < kGenChunk / BLK; ++sb) {
-    const int64_t m0 = n0 + sb * BLK;
-    if (full && sb + 1 < kGenChunk / BLK) {
+    for (int q = 0; q < VPR; ++q) {
+      const int e = l + 64 * q;
+      const int64_t js = g * 64 + e / VPR;
+      if ((js + 1) * kGenChunk <= T) {
+        const TACC* src = xrow + seg(js, sb, e % VPR);
+        if (xv_ok) {
+          xin[q] = *reinterpret_cast<const u32x4*>(src);
+        } else {
+          TACC v[V];
 #pragma unroll
-      for (int i = 0; i < BLK; ++i) nxt[i] = xr[m0 + BLK + i];
+          for (int i = 0; i < V; ++i) v[i] = src[i];
+          __builtin_memcpy(&xin[q], v, sizeof(v));
+        }
+      }
     }
+  };
+  // block sb's vectors into the tile in memory order, then each lane's row
+  // out in processing order
+  auto take_block = [&](TACC (&cur)[BLK]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int q = 0; q < VPR; ++q) {
+      const int e = l + 64 * q;
+      const int64_t js = g * 64 + e / VPR;
+      if ((js + 1) * kGenChunk <= T)
+        *reinterpret_cast<u32x4*>(&tile[(e / VPR) * TS + (e % VPR) * V]) = xin[q];
+    }
+    __syncthreads();
+    if (full) {
+#pragma unroll
+      for (int i = 0; i < BLK; ++i) cur[i] = tile[l * TS + (REV ? BLK - 1 - i : i)];
+    }
+    __syncthreads();
+  };
+  TACC cur[BLK];
+  load_block(0);
+  take_block(cur);
+  for (int sb = 0; sb < NB; ++sb) {
+    if (sb + 1 < NB) load_block(sb + 1);
     TACC yv[BLK];
     if (full) {
 #pragma unroll
-      for (int i = 0; i < BLK; ++i) df2t_step<TACC, NMAX>(bn, an, N, z, cur[i], yv[i]);
+      for (int i = 0; i < BLK; ++i) df2t_step_padded<TACC, NMAX>(bn, an, z, cur[i], yv[i]);
     }
     if constexpr (SECOND) {
       if (full) {
 #pragma unroll
-        for (int i = 0; i < BLK; ++i) tile[l * TS + i] = yv[i];
+        for (int i = 0; i < BLK; ++i) tile[l * TS + (REV && yv_ok ? BLK - 1 - i : i)] = yv[i];
       }
       __syncthreads();
-      // vector e = l + 64 q: source lane e / (BLK / V), its samples from
-      // V (e % (BLK / V)); stored whole when the row is 16-byte aligned
-      constexpr int VPR = BLK / V;                  // vectors per tile row
-      const bool vec_ok =
-          !REV && ((reinterpret_cast<uintptr_t>(y + c * y_stride) | (y_stride * sizeof(TACC))) &
-                   15) == 0;
+      // vector e = l + 64 q: source lane e / VPR, its samples from V (e %
+      // VPR); with yv_ok the tile row holds them in memory order
 #pragma unroll
       for (int q = 0; q < VPR; ++q) {
         const int e = l + 64 * q;
         const int src = e / VPR, part = e % VPR;
         const int64_t js = g * 64 + src;
         if ((js + 1) * kGenChunk > T) continue;
-        const int64_t k0 = js * kGenChunk + sb * BLK + part * V;
-        const TACC* from = tile + src * TS + part * V;
-        if (vec_ok) {
-          *reinterpret_cast<uint4*>(&yr[k0]) = *reinterpret_cast<const uint4*>(from);
+        if (yv_ok) {
+          *reinterpret_cast<u32x4*>(yrow + seg(js, sb, part)) =
+              *reinterpret_cast<const u32x4*>(tile + src * TS + part * V);
         } else {
+          const int64_t k0 = js * kGenChunk + sb * BLK + part * V;
+          const TACC* from = tile + src * TS + part * V;
 #pragma unroll
           for (int v = 0; v < V; ++v) yr[k0 + v] = from[v];
         }
       }
       __syncthreads();
     }
-#pragma unroll
-    for (int i = 0; i < BLK; ++i) cur[i] = nxt[i];
+    if (sb + 1 < NB) take_block(cur);
   }
   if (valid && !full) {
     for (int64_t n = n0; n < n1; ++n) {
       TACC yn;
-      df2t_step<TACC, NMAX>(bn, an, N, z, xr[n], yn);
+      df2t_step_padded<TACC, NMAX>(bn, an, z, xr[n], yn);
       if (SECOND) yr[n] = yn;
     }
   }
