@@ -319,9 +319,12 @@ __global__ __launch_bounds__(64) void lfilter_chunk_kernel(
 // forms column c of P = A^L by L products with the companion matrix (O(N)
 // each): repeated squaring loses the accuracy of this non-normal matrix
 // (relative error of A^256 for an order-16 Butterworth lowpass above 1,
-// against 4e-10 this way).  Then one lane per channel
-// walks its chunks, P in registers up to order 8 (read from LDS above),
-// the chunk states loaded in groups with the next group in flight.
+// against 4e-10 this way).  Then NMAX lanes per channel walk its chunks,
+// lane r holding row r of P and component r of the state: each step takes
+// the state's components from the channel's lanes (NMAX shuffles) and forms
+// S_{j+1}[r] = s_j[r] + sum_m P[r][m] S_j[m] in the order m = 0, 1, ...
+// (64 / NMAX channels per wave, the chunk states loaded kScanGroup chunks
+// ahead, a channel's lanes reading and writing one contiguous run).
 template <typename TACC, int NMAX>
 __global__ __launch_bounds__(64) void lfilter_scan_kernel(const TACC* __restrict__ coef, int N,
                                                           int64_t B, int64_t M,
@@ -354,62 +357,38 @@ __global__ __launch_bounds__(64) void lfilter_scan_kernel(const TACC* __restrict
       if (r < N) P[r * N + l] = v[r];
   }
   __syncthreads();
-  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + l;
-  if (c >= B) return;
-  constexpr bool kPinRegs = NMAX <= 8;
-  double Pr[kPinRegs ? NMAX : 1][kPinRegs ? NMAX : 1];
-  if constexpr (kPinRegs) {
+  constexpr int LPC = NMAX;                       // lanes per channel
+  const int r = l % LPC;
+  const int64_t c = (static_cast<int64_t>(blockIdx.x) * blockDim.x + l) / LPC;
+  const bool live = c < B;
+  const int base = l - r;                         // the channel's first lane
+  double Pr[NMAX];
 #pragma unroll
-    for (int r = 0; r < NMAX; ++r)
-#pragma unroll
-      for (int m = 0; m < NMAX; ++m) Pr[r][m] = (r < N && m < N) ? P[r * N + m] : 0.0;
-  }
-  double S[NMAX];
+  for (int m = 0; m < NMAX; ++m) Pr[m] = (r < N && m < N) ? P[r * N + m] : 0.0;
   const int64_t zrow = (flags & TSIIR_ZI_BROADCAST) ? 0 : c;
-  double* w = ws + c * M * N;
+  double* w = ws + (live ? c : 0) * M * N;
+  const bool mine = live && r < N;                // this lane's component is stored
+  double S = (mine && (flags & TSIIR_HAS_ZI)) ? static_cast<double>(zi[zrow * N + r]) : 0.0;
+  constexpr int kScanGroup = 8;
+  double cur[kScanGroup], nxt[kScanGroup];
 #pragma unroll
-  for (int k = 0; k < NMAX; ++k)
-    S[k] = (k < N && (flags & TSIIR_HAS_ZI)) ? static_cast<double>(zi[zrow * N + k]) : 0.0;
-  // chunk states in groups of kScanGroup, the next group's loads in flight
-  constexpr int kScanGroup = 4;
-  double cur[kScanGroup][NMAX], nxt[kScanGroup][NMAX];
-#pragma unroll
-  for (int q = 0; q < kScanGroup; ++q)
-#pragma unroll
-    for (int k = 0; k < NMAX; ++k) cur[q][k] = (q < M && k < N) ? w[q * N + k] : 0.0;
+  for (int q = 0; q < kScanGroup; ++q) cur[q] = (mine && q < M) ? w[q * N + r] : 0.0;
   for (int64_t j0 = 0; j0 < M; j0 += kScanGroup) {
 #pragma unroll
     for (int q = 0; q < kScanGroup; ++q)
-#pragma unroll
-      for (int k = 0; k < NMAX; ++k)
-        nxt[q][k] = (j0 + kScanGroup + q < M && k < N) ? w[(j0 + kScanGroup + q) * N + k] : 0.0;
+      nxt[q] = (mine && j0 + kScanGroup + q < M) ? w[(j0 + kScanGroup + q) * N + r] : 0.0;
 #pragma unroll
     for (int q = 0; q < kScanGroup; ++q) {
       if (j0 + q < M) {
+        if (mine) w[(j0 + q) * N + r] = S;
+        double acc = cur[q];
 #pragma unroll
-        for (int k = 0; k < NMAX; ++k)
-          if (k < N) w[(j0 + q) * N + k] = S[k];
-        double Sn[NMAX];
-#pragma unroll
-        for (int r = 0; r < NMAX; ++r) {
-          double acc = cur[q][r];
-#pragma unroll
-          for (int m = 0; m < NMAX; ++m) {
-            if constexpr (kPinRegs)
-              acc += Pr[r][m] * S[m];
-            else if (r < N && m < N)
-              acc += P[r * N + m] * S[m];
-          }
-          Sn[r] = acc;
-        }
-#pragma unroll
-        for (int r = 0; r < NMAX; ++r) S[r] = Sn[r];
+        for (int m = 0; m < NMAX; ++m) acc += Pr[m] * __shfl(S, base + m);
+        S = acc;
       }
     }
 #pragma unroll
-    for (int q = 0; q < kScanGroup; ++q)
-#pragma unroll
-      for (int k = 0; k < NMAX; ++k) cur[q][k] = nxt[q][k];
+    for (int q = 0; q < kScanGroup; ++q) cur[q] = nxt[q];
   }
 }
 
@@ -432,7 +411,9 @@ static tsiir_status launch_general(const tsiir_sosfilt_desc* d, const void* coef
     const int64_t G = (M + 63) / 64;   // workgroups per channel
     if (d->channels * G > 0x7fffffff) return TSIIR_EUNSUPPORTED;
     const unsigned cblocks = static_cast<unsigned>(d->channels * G);
-    const unsigned sblocks = static_cast<unsigned>((d->channels + 63) / 64);
+    // the scan: N rounded up to the kernel's order (4, 8 or 16) lanes per channel
+    const int64_t lpc = N <= 4 ? 4 : (N <= 8 ? 8 : 16);
+    const unsigned sblocks = static_cast<unsigned>((d->channels * lpc + 63) / 64);
     auto go = [&](auto first, auto scan, auto second) {
       hipLaunchKernelGGL(first, dim3(cblocks), dim3(64), 0, st, static_cast<const TACC*>(coef),
                          N, static_cast<const TACC*>(x), static_cast<TACC*>(y),
